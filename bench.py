@@ -1,0 +1,169 @@
+"""Benchmark: substeps/s of the FLIP/PIC substep on one MI355X.
+
+Workload (default): dam break on a 128^3 MAC grid, dx = 1/128, 1-cell solid
+walls, fluid x <= 0.4, y <= 0.5 (cell centres) -> 396,900 fluid cells and
+3,175,200 particles (8 per cell, one per 2x2x2 sub-cell, SplitMix64 seed 1),
+gravity -9.81, FLIP 0.95, fixed dt = 1/240, PCG to 1e-6 relative (inf-norm)
+residual.  One step = one substep (density, material, save, forces, pressure
+solve, apply, extrapolation, G2P + RK4 advection, P2G).  The timed run uses the
+mixed-precision PCG; the plain FP64 PCG runs the same substeps beside it.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: (ni, nj, nk, dx, rd fractions, seed, pcg_max_iter)
+    "dam128": (128, 128, 128, 1.0 / 128, (0.4, 0.5, 1.0), 1, 200),
+    "dam64x32x32": (64, 32, 32, 0.125, (0.35, 0.6, 1.0), 0, 200),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_sim(xf, cfg, precision, sweep_kernel=0):
+    ni, nj, nk, dx, frac, seed, maxit = CONFIGS[cfg]
+    p = xf.default_params(flip_ratio=0.95, pcg_tol=1e-6, pcg_max_iter=maxit, precision=precision,
+                          sweep_kernel=sweep_kernel, mixed_inner_tol=1e-4)
+    s = xf.FluidSimulator(ni, nj, nk, p)
+    s.setCellWidth(dx)
+    s.setSolidBoundary()
+    L = (ni * dx, nj * dx, nk * dx)
+    s.addCubicFluid((0.0, 0.0, 0.0), (frac[0] * L[0], frac[1] * L[1], frac[2] * L[2]))
+    s.initParticles(xf.SEED_SUBCELL, seed)
+    s.addConstantForce(0.0, -9.81, 0.0)
+    return s
+
+
+def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0):
+    s = make_sim(xf, cfg, precision, sweep_kernel)
+    for _ in range(warmup):
+        s.timeStep(dt)
+    s.synchronize()
+    s.set_timing(True)
+    stage_ms = [0.0] * 9
+    solve_ms, iters, outer, worst = 0.0, 0, 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rc = s.timeStep(dt)
+        st = s.last_solve()
+        solve_ms += st.ms
+        iters += st.iterations
+        outer += st.outer
+        worst = max(worst, st.residual / max(st.rhs_norm, 1e-300))
+        t = s.last_timing()
+        for i in range(9):
+            stage_ms[i] += t.ms[i]
+        if rc != 0:
+            log(f"warning: substep solve status {rc}")
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    n = s.num_particles()
+    s.close()
+    return dict(elapsed=elapsed, solve_ms=solve_ms, iters=iters, outer=outer, worst_rel_res=worst,
+                stage_ms=stage_ms, particles=n)
+
+
+def cpu_baseline(cfg, dt):
+    """The oracle (FP64 C++ restatement, reference-faithful threading) on one substep."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from scenes import oracle_dam_break
+    from oracle_lib import default_params
+    ni, nj, nk, dx, frac, seed, maxit = CONFIGS[cfg]
+    p = default_params(flip_ratio=0.95, pcg_tol=1e-6, pcg_max_iter=maxit)
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, p)
+    t0 = time.perf_counter()
+    o.time_step(dt)
+    el = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return dict(value=1.0 / el, unit="substeps/s", cores=min(cores, 6), kind="port",
+                sample=f"1 substep of {cfg} from rest ({o.lib.xo_num_particles(o.h)} particles), "
+                       f"oracle in reference-faithful mode (serial assembly/MIC(0)/PCG/P2G, 6-chunk OpenMP "
+                       f"density and G2P), {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="dam128", choices=list(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--sweep", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the substep on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+    import xfluid_amd as xf
+    if xf.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device")
+    dt = 1.0 / 240
+    ni, nj, nk = CONFIGS[args.config][:3]
+    N = ni * nj * nk
+    log(f"[bench] mixed precision, {args.config}, warmup {args.warmup}, steps {args.steps}")
+    mixed = run_gpu(xf, args.config, xf.PREC_MIXED, args.steps, args.warmup, dt, args.sweep)
+    log(f"[bench] mixed: {mixed['elapsed']:.3f} s, solve {mixed['solve_ms']:.1f} ms, iters {mixed['iters']}")
+    fp64 = None
+    if not args.no_fp64:
+        log("[bench] fp64 PCG")
+        fp64 = run_gpu(xf, args.config, xf.PREC_FP64, args.steps, args.warmup, dt, args.sweep)
+        log(f"[bench] fp64: {fp64['elapsed']:.3f} s, solve {fp64['solve_ms']:.1f} ms, iters {fp64['iters']}")
+    K = args.steps
+    value = K / mixed["elapsed"]
+    # roofline of the pressure solve (algorithmic bytes per PCG iteration: FP64
+    # 131 B/cell, FP32 bulk 67 B/cell over the full N-cell vectors)
+    it_m = max(mixed["iters"], 1)
+    ach_m = 67.0 * N * it_m / (mixed["solve_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "substeps/s (128^3 dam break, mixed-precision PCG to 1e-6 rel. residual)",
+        "value": round(value, 4),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["elapsed"] / K, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (PCG bulk f32 + f64 dots/refinement)",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config} dam break, {mixed['particles']} particles, dt=1/240, FLIP 0.95",
+                   "grid": [ni, nj, nk], "particles": mixed["particles"]},
+        "pcg_ms_per_solve_mixed": round(mixed["solve_ms"] / K, 3),
+        "pcg_iters_per_solve_mixed": round(mixed["iters"] / K, 2),
+        "pcg_outer_per_solve_mixed": round(mixed["outer"] / K, 2),
+        "worst_rel_residual_mixed": mixed["worst_rel_res"],
+        "particles_per_s": round(mixed["particles"] * value, 1),
+        "stage_ms_per_step_mixed": [round(x / K, 3) for x in mixed["stage_ms"]],
+        "roofline": {"bound": "hbm", "kernel": "pcg_solve (mixed)", "achieved": round(ach_m, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_m / HBM_PEAK_GBS, 5),
+                     "traffic": None},
+    }
+    if fp64:
+        it_f = max(fp64["iters"], 1)
+        out["substeps_per_s_fp64"] = round(K / fp64["elapsed"], 4)
+        out["pcg_ms_per_solve_fp64"] = round(fp64["solve_ms"] / K, 3)
+        out["pcg_iters_per_solve_fp64"] = round(fp64["iters"] / K, 2)
+        out["mixed_vs_fp64_solve_rate"] = round(fp64["solve_ms"] / max(mixed["solve_ms"], 1e-9), 3)
+        out["pcg_fp64_achieved_GBs"] = round(131.0 * N * it_f / (fp64["solve_ms"] * 1e-3) / 1e9, 2)
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle, 1 substep)")
+        out["cpu_baseline"] = cpu_baseline(args.config, dt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

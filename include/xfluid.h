@@ -170,6 +170,13 @@ int  xf_sim_frame_count(const xf_sim* s);
  *      `cap` entries are written. */
 int64_t xf_sim_surface_cells(xf_sim* s, int32_t* cells, uint8_t* cube_index, int64_t cap);
 
+/* ---- operator diagnostics on the last assembled system (parity hooks):
+ *      y = A x (FixedSparseMatrix multiply, sparse_matrix.h:266-276) and
+ *      z = (L L^T)^-1 r (PCGSolver::apply_preconditioner, pcg_solver.h:305-309).
+ *      Host buffers of ni*nj*nk doubles. */
+int  xf_sim_apply_matrix(xf_sim* s, const double* x, double* y);
+int  xf_sim_apply_precond(xf_sim* s, const double* r, double* z);
+
 /* ---- standalone projection: VariationalPressureSolver setters + solve() +
  *      applyPressure() (variational_pressuresolver.h:35-44).  Solves on the
  *      given material/face velocities and updates u,v,w in place; pressure

@@ -1,0 +1,204 @@
+"""GPU parity: every stage of the substep on the HIP library against the FP64
+oracle on identical inputs.
+
+Bit-exact where the arithmetic order is fixed (material, forces, right-hand
+side, MIC(0) pivots, SpMV, preconditioner sweeps, apply-pressure,
+extrapolation, G2P + advection, surface classification).  Tolerances where the
+GPU sums in another order: P2G and density (relative 1e-12), the PCG solution
+(relative residual of the solve, and pressure within 1e-5 of the FP64 oracle's).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle_lib import FIELD, STAGE, oracle
+from scenes import copy_state, gpu_like, oracle_dam_break, small_params
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # (ni, nj, nk, dx, fluid fractions, seed)
+    "c1_small": (24, 16, 12, 0.125, (0.35, 0.6, 1.0), 0),
+    "c1_quarter": (32, 16, 16, 0.125, (0.35, 0.6, 1.0), 0),
+    "odd": (19, 13, 11, 0.1, (0.5, 0.55, 1.0), 7),
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def xf():
+    import xfluid_amd
+    if xfluid_amd.device_count() < 1:
+        pytest.fail("no HIP device visible to libxfluid_hip.so")
+    return xfluid_amd
+
+
+def _advance(o, steps, dt):
+    for _ in range(steps):
+        o.time_step(dt)
+
+
+def _exact(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    if not np.array_equal(a, b):
+        d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+        raise AssertionError(f"{what}: {np.count_nonzero(d)} entries differ, max {d.max():.3e}")
+
+
+def _close(a, b, rtol, what):
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max() / scale
+    assert err <= rtol, f"{what}: relative error {err:.3e} > {rtol:.1e}"
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_stages_bitexact(xf, case):
+    ni, nj, nk, dx, frac, seed = CASES[case]
+    say(f"[parity] case={case} grid={ni}x{nj}x{nk}")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    g = gpu_like(o)
+    dt = 1.0 / 60
+    for step in range(3):
+        # material
+        copy_state(o, g)
+        o.stage(STAGE["MATERIAL"]); g.stage(STAGE["MATERIAL"])
+        _exact(g.get_field(FIELD["MATERIAL"]), o.get(FIELD["MATERIAL"]), f"{case} step{step} material")
+        o.stage(STAGE["SAVE"]); g.stage(STAGE["SAVE"])
+        for f in ("USAVED", "VSAVED", "WSAVED"):
+            _exact(g.get_field(FIELD[f]), o.get(FIELD[f]), f"{case} step{step} {f}")
+        o.stage(STAGE["FORCES"], dt); g.stage(STAGE["FORCES"], dt)
+        for f in ("U", "V", "W"):
+            _exact(g.get_field(FIELD[f]), o.get(FIELD[f]), f"{case} step{step} forces {f}")
+        # pressure solve: RHS and pivots exact, solution to tolerance
+        so = o.stage(STAGE["SOLVE"]); sg = g.stage(STAGE["SOLVE"])
+        _exact(g.get_field(FIELD["RHS"]), o.get(FIELD["RHS"]), f"{case} step{step} rhs")
+        _exact(g.get_field(FIELD["MIC_INV"]), o.get(FIELD["MIC_INV"]), f"{case} step{step} mic0")
+        assert so == 0 and sg == 0
+        st_o, st_g = o.last_solve(), g.last_solve()
+        assert abs(st_o.iterations - st_g.iterations) <= 2, (st_o.iterations, st_g.iterations)
+        assert st_g.residual <= 1e-6 * st_g.rhs_norm
+        _close(g.get_field(FIELD["PRESSURE"]), o.get(FIELD["PRESSURE"]), 1e-5, f"{case} step{step} pressure")
+        # apply with the oracle's pressure: exact
+        g.set_field(FIELD["PRESSURE"], o.get(FIELD["PRESSURE"]))
+        o.stage(STAGE["APPLY"]); g.stage(STAGE["APPLY"])
+        for f in ("U", "V", "W"):
+            _exact(g.get_field(FIELD[f]), o.get(FIELD[f]), f"{case} step{step} apply {f}")
+        o.stage(STAGE["EXTRAPOLATE"]); g.stage(STAGE["EXTRAPOLATE"])
+        for f in ("U", "V", "W"):
+            _exact(g.get_field(FIELD[f]), o.get(FIELD[f]), f"{case} step{step} extrapolate {f}")
+        o.stage(STAGE["ADVECT"], dt); g.stage(STAGE["ADVECT"], dt)
+        Po, Vo = o.particles()
+        Pg, Vg = g.particles()
+        _exact(Pg, Po, f"{case} step{step} advect positions")
+        _exact(Vg, Vo, f"{case} step{step} advect velocities")
+        o.stage(STAGE["P2G"]); g.stage(STAGE["P2G"])
+        for f in ("U", "V", "W"):
+            _close(g.get_field(FIELD[f]), o.get(FIELD[f]), 1e-12, f"{case} step{step} p2g {f}")
+        o.stage(STAGE["DENSITY"]); g.stage(STAGE["DENSITY"])
+        _close(g.get_field(FIELD["PHI"]), o.get(FIELD["PHI"]), 1e-12, f"{case} step{step} density")
+
+
+@pytest.mark.parametrize("case", ["c1_small", "odd"])
+def test_operators_bitexact(xf, case):
+    """SpMV and the MIC(0) preconditioner (both sweeps) on random vectors."""
+    ni, nj, nk, dx, frac, seed = CASES[case]
+    say(f"[operators] case={case}")
+    O = oracle()
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    o.stage(STAGE["MATERIAL"])
+    o.stage(STAGE["FORCES"], 1.0 / 60)
+    g = gpu_like(o)
+    copy_state(o, g)
+    g.stage(STAGE["SOLVE"])
+    N = ni * nj * nk
+    mat = o.get(FIELD["MATERIAL"]).ravel()
+    A7 = np.empty(7 * N)
+    b = np.empty(N)
+    O.xo_assemble(ni, nj, nk, mat, o.get(FIELD["U"]).ravel(), o.get(FIELD["V"]).ravel(),
+                  o.get(FIELD["W"]).ravel(), A7, b)
+    inv = np.empty(N)
+    O.xo_mic0(ni, nj, nk, A7, 0.97, 0.25, inv)
+    _exact(g.get_field(FIELD["MIC_INV"]).ravel(), inv, "mic0")
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(N)
+    x[inv == 0] = 0.0
+    y_o = np.empty(N)
+    O.xo_spmv(ni, nj, nk, A7, x, y_o)
+    _exact(g.apply_matrix(x), y_o, "spmv")
+    q = np.empty(N)
+    z = np.empty(N)
+    O.xo_sweep_fwd(ni, nj, nk, A7, inv, x, q)
+    O.xo_sweep_bwd(ni, nj, nk, A7, inv, q, z)
+    _exact(g.apply_precond(x), z, "precond (per-plane sweeps)")
+
+
+def test_surface_classification_exact(xf):
+    """Same density field -> same surface sub-cell list (reference order) and cube cases."""
+    ni, nj, nk, dx, frac, seed = CASES["c1_quarter"]
+    say("[surface] c1_quarter")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params(iso=0.5, blob_radius=1.0 * dx))
+    o.stage(STAGE["DENSITY"])
+    g = gpu_like(o)
+    g.set_field(FIELD["PHI"], o.get(FIELD["PHI"]))
+    co, qo = o.surface_cells()
+    cg, qg = g.surface_cells()
+    assert len(co) > 100
+    _exact(cg, co, "surface cells")
+    _exact(qg, qo, "cube index")
+
+
+def test_trajectory_tolerance(xf):
+    """Five full substeps run independently on GPU and oracle stay within tolerance."""
+    ni, nj, nk, dx, frac, seed = CASES["c1_small"]
+    say("[trajectory] c1_small 5 substeps")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    g = gpu_like(o)
+    dt = 1.0 / 60
+    for _ in range(5):
+        assert o.time_step(dt) == 0
+        assert g.timeStep(dt) == 0
+    Po, Vo = o.particles()
+    Pg, Vg = g.particles()
+    _close(Pg, Po, 1e-9, "positions after 5 substeps")
+    _close(Vg, Vo, 1e-6, "velocities after 5 substeps")
+    _exact(g.get_field(FIELD["MATERIAL"]), o.get(FIELD["MATERIAL"]), "material after 5 substeps")
+
+
+def test_mixed_precision_solve(xf):
+    """FP32-bulk PCG with FP64 refinement reaches the FP64 tolerance on the same system."""
+    ni, nj, nk, dx, frac, seed = CASES["c1_quarter"]
+    say("[mixed] c1_quarter")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    for _ in range(2):
+        o.time_step(1.0 / 60)
+    o.stage(STAGE["MATERIAL"]); o.stage(STAGE["SAVE"]); o.stage(STAGE["FORCES"], 1.0 / 60)
+    g = gpu_like(o)
+    copy_state(o, g)
+    o.stage(STAGE["SOLVE"])
+    p = g.params
+    p.precision = xf.PREC_MIXED
+    g.set_params(p)
+    assert g.stage(STAGE["SOLVE"]) == 0
+    st = g.last_solve()
+    assert st.residual <= 1e-6 * st.rhs_norm
+    assert st.outer >= 1
+    _close(g.get_field(FIELD["PRESSURE"]), o.get(FIELD["PRESSURE"]), 1e-4, "mixed pressure vs FP64 oracle")
+
+
+def test_frame_step_runs(xf):
+    ni, nj, nk, dx, frac, seed = CASES["c1_small"]
+    say("[frame] c1_small frameStep(1/30)")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    g = gpu_like(o)
+    assert o.frame_step(1.0 / 30) == 0
+    assert g.frameStep(1.0 / 30) == 0
+    Po, _ = o.particles()
+    Pg, _ = g.particles()
+    _close(Pg, Po, 1e-8, "positions after one frame")

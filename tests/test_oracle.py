@@ -1,0 +1,387 @@
+"""CPU tests of the oracle (the FP64 restatement), pinned three ways:
+
+1. against the reference's own code (oracle/_ref/libxfref.so, built from the
+   reference sources in place) stage by stage, bit for bit -- when present;
+2. against committed golden fixtures written by that build (tests/golden/);
+3. against closed forms and invariants (dense linear algebra, kernel values,
+   affine fields) that a plausible restatement mistake would break.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import FIELD, STAGE, oracle, ref_available
+from scenes import oracle_dam_break, ref_like, small_params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref/libxfref.so not built")
+
+FIELDS_CMP = ("U", "V", "W", "USAVED", "VSAVED", "WSAVED", "MATERIAL", "PHI")
+
+
+def _same(a, b, what):
+    if not np.array_equal(a, b):
+        d = np.abs(np.asarray(a, float) - np.asarray(b, float))
+        raise AssertionError(f"{what}: {np.count_nonzero(d)} differ, max {d.max():.3e}")
+
+
+# ---------------------------------------------------------------------------
+# 1. against the reference's own build
+@needs_ref
+@pytest.mark.parametrize("case", [
+    ((16, 12, 10), 0.125, (0.35, 0.6, 1.0), 0, {}),
+    ((13, 11, 9), 0.1, (0.5, 0.5, 1.0), 3, {"flip_ratio": 0.5}),           # n % 6 != 0: tail quirk
+    ((12, 12, 12), 0.1, (0.6, 0.7, 1.0), 5, {"blob_radius": 0.1, "iso": 0.5}),
+])
+def test_stages_match_reference(case):
+    dims, dx, frac, seed, extra = case
+    p = small_params(**extra)
+    o = oracle_dam_break(*dims, dx, frac, seed, p)
+    r = ref_like(o)
+    dt = 1.0 / 60
+    for step in range(4):
+        for s in range(9):
+            so = o.stage(s, dt)
+            sr = r.stage(s, dt)
+            assert so == sr
+            for f in FIELDS_CMP:
+                _same(o.get(FIELD[f]), r.get(FIELD[f]), f"step {step} stage {s} field {f}")
+            Po, Vo = o.particles()
+            Pr, Vr = r.particles()
+            _same(Po, Pr, f"step {step} stage {s} positions")
+            _same(Vo, Vr, f"step {step} stage {s} velocities")
+            if s == STAGE["SOLVE"]:
+                for f in ("PRESSURE", "RHS", "MIC_INV"):
+                    _same(o.get(FIELD[f]), r.get(FIELD[f]), f"step {step} {f}")
+                assert o.last_solve().iterations == r.last_solve()[0]
+            if s == STAGE["DENSITY"]:
+                _same(o.surface_cells()[0], r.surface_cells(), f"step {step} surface cells")
+
+
+@needs_ref
+def test_collisions_match_reference():
+    """Fast particles driven into the walls exercise the voxel walk and ray-box pull-back."""
+    p = small_params(flip_ratio=1.0)
+    o = oracle_dam_break(14, 12, 10, 0.1, (0.6, 0.6, 1.0), 11, p)
+    P, V = o.particles()
+    rng = np.random.default_rng(1)
+    V = rng.uniform(-40, 40, V.shape)
+    o.set_particles(P, V)
+    r = ref_like(o)
+    # large velocities on the grid too
+    for f, shape in (("U", None), ("V", None), ("W", None)):
+        a = rng.uniform(-30, 30, o.get(FIELD[f]).shape)
+        o.set(FIELD[f], a)
+        r.set(FIELD[f], a)
+    dt = 1.0 / 30
+    for s in (STAGE["MATERIAL"], STAGE["SAVE"], STAGE["ADVECT"]):
+        o.stage(s, dt)
+        r.stage(s, dt)
+    Po, Vo = o.particles()
+    Pr, Vr = r.particles()
+    # The reference indexes the material of the pulled-back position without a
+    # range check (FluidSimulator.cpp:713-715), undefined behaviour when it lies
+    # outside the grid; the restatement rejects such positions.  Those are the
+    # only particles allowed to differ.
+    cells = np.floor_divide(Pr, 0.1).astype(int)
+    outside = np.any((cells < 0) | (cells >= np.array([14, 12, 10])), axis=1)
+    diff = np.any(Po != Pr, axis=1)
+    assert np.array_equal(diff, diff & outside), "differences outside the undefined-behaviour case"
+    assert np.array_equal(Po[outside], P[outside]), "rejected moves keep the old position"
+    _same(Po[~outside], Pr[~outside], "positions after collision handling")
+    _same(Vo, Vr, "velocities")
+    moved_back = np.count_nonzero(np.any(Po != P, axis=1))
+    assert moved_back > 0
+
+
+@needs_ref
+def test_frame_step_matches_reference():
+    p = small_params()
+    o = oracle_dam_break(16, 12, 10, 0.125, (0.35, 0.6, 1.0), 2, p)
+    r = ref_like(o)
+    for _ in range(2):
+        o.frame_step(1.0 / 30)
+        r.frame_step(1.0 / 30)
+    Po, Vo = o.particles()
+    Pr, Vr = r.particles()
+    _same(Po, Pr, "positions after 2 frames")
+    _same(Vo, Vr, "velocities after 2 frames")
+
+
+@needs_ref
+def test_pressure_pieces_match_reference():
+    """MIC(0) pivots and one preconditioner application against Bridson's
+    factor_modified_incomplete_cholesky0 / solve_lower(_transpose) themselves."""
+    O = oracle()
+    p = small_params()
+    o = oracle_dam_break(14, 11, 9, 0.1, (0.5, 0.6, 1.0), 4, p)
+    for _ in range(2):
+        o.time_step(1.0 / 60)
+    o.stage(STAGE["MATERIAL"]); o.stage(STAGE["FORCES"], 1.0 / 60)
+    r = ref_like(o)
+    for f in ("U", "V", "W", "MATERIAL"):
+        r.set(FIELD[f], o.get(FIELD[f]))
+    ni, nj, nk = o.ni, o.nj, o.nk
+    N = ni * nj * nk
+    A_r, b_r, inv_r, z_r = np.empty(7 * N), np.empty(N), np.empty(N), np.empty(N)
+    x = np.random.default_rng(0).standard_normal(N)
+    assert r.lib.xr_pressure_pieces(r.h, A_r, b_r, inv_r, x, z_r) == 0
+    A_o, b_o, inv_o = np.empty(7 * N), np.empty(N), np.empty(N)
+    O.xo_assemble(ni, nj, nk, o.get(FIELD["MATERIAL"]).ravel(), o.get(FIELD["U"]).ravel(),
+                  o.get(FIELD["V"]).ravel(), o.get(FIELD["W"]).ravel(), A_o, b_o)
+    _same(A_o, A_r, "matrix")
+    _same(b_o, b_r, "rhs")
+    O.xo_mic0(ni, nj, nk, A_o, 0.97, 0.25, inv_o)
+    _same(inv_o, inv_r, "MIC(0) pivots")
+    q, z = np.empty(N), np.empty(N)
+    O.xo_sweep_fwd(ni, nj, nk, A_o, inv_o, x, q)
+    O.xo_sweep_bwd(ni, nj, nk, A_o, inv_o, q, z)
+    _same(z, z_r, "preconditioner")
+
+
+# ---------------------------------------------------------------------------
+# 2. golden fixtures written by the reference build
+def _golden(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not generated")
+    return np.load(path, allow_pickle=False)
+
+
+def test_golden_substeps():
+    g = _golden("dam16x12x10_substeps.npz")
+    dims = tuple(int(x) for x in g["dims"])
+    dx = float(g["dx"])
+    p = small_params()
+    o = oracle_dam_break(*dims, dx, tuple(g["frac"]), int(g["seed"]), p)
+    Pi, _ = o.particles()
+    _same(Pi, g["p0"], "seeded initial positions")
+    for step in range(int(g["steps"])):
+        o.time_step(float(g["dt"]))
+    P, V = o.particles()
+    _same(P, g["p_final"], "positions")
+    _same(V, g["v_final"], "velocities")
+    for f in ("U", "V", "W", "MATERIAL", "PHI"):
+        _same(o.get(FIELD[f]), g[f], f)
+
+
+def test_golden_pressure():
+    g = _golden("dam14x11x9_pressure.npz")
+    O = oracle()
+    ni, nj, nk = (int(x) for x in g["dims"])
+    N = ni * nj * nk
+    A, b, inv = np.empty(7 * N), np.empty(N), np.empty(N)
+    O.xo_assemble(ni, nj, nk, g["material"].ravel(), g["u"].ravel(), g["v"].ravel(), g["w"].ravel(), A, b)
+    _same(A, g["A7"], "matrix")
+    _same(b, g["b"], "rhs")
+    O.xo_mic0(ni, nj, nk, A, 0.97, 0.25, inv)
+    _same(inv, g["inv"], "pivots")
+    q, z = np.empty(N), np.empty(N)
+    O.xo_sweep_fwd(ni, nj, nk, A, inv, g["r"], q)
+    O.xo_sweep_bwd(ni, nj, nk, A, inv, q, z)
+    _same(z, g["z"], "preconditioner")
+
+
+# ---------------------------------------------------------------------------
+# 3. closed forms and invariants
+def _dense(A7, ni, nj, nk):
+    N = ni * nj * nk
+    nn = ni * nj
+    off = [-nn, -ni, -1, 0, 1, ni, nn]
+    D = np.zeros((N, N))
+    for e in range(7):
+        for c in range(N):
+            a = A7[e * N + c]
+            if a != 0:
+                D[c, c + off[e]] = a
+    return D
+
+
+def _random_system(seed=0, dims=(7, 6, 5)):
+    O = oracle()
+    ni, nj, nk = dims
+    rng = np.random.default_rng(seed)
+    mat = np.where(rng.random((nk, nj, ni)) < 0.7, 1, 0).astype(np.uint8)
+    mat[0, :, :] = mat[-1, :, :] = 2
+    mat[:, 0, :] = mat[:, -1, :] = 2
+    mat[:, :, 0] = mat[:, :, -1] = 2
+    mat[2, 2, 2:4] = 2
+    u = rng.standard_normal((nk, nj, ni + 1))
+    v = rng.standard_normal((nk, nj + 1, ni))
+    w = rng.standard_normal((nk + 1, nj, ni))
+    N = ni * nj * nk
+    A, b = np.empty(7 * N), np.empty(N)
+    O.xo_assemble(ni, nj, nk, mat.ravel(), u.ravel(), v.ravel(), w.ravel(), A, b)
+    return O, dims, mat, (u, v, w), A, b
+
+
+def test_matrix_closed_form():
+    """Symmetric; fluid diagonal = 6 - 0.5*(#solid) + (#air); off-diagonals -1 / -0.5;
+    the RHS of a fluid row is the negative discrete divergence with solid faces halved."""
+    O, (ni, nj, nk), mat, (u, v, w), A, b = _random_system(1)
+    D = _dense(A, ni, nj, nk)
+    assert np.array_equal(D, D.T)
+    m = mat.ravel()
+    for c in np.nonzero(m == 1)[0]:
+        k, rem = divmod(c, ni * nj)
+        j, i = divmod(rem, ni)
+        nb = [mat[k, j, i + 1], mat[k, j, i - 1], mat[k, j + 1, i], mat[k, j - 1, i], mat[k + 1, j, i],
+              mat[k - 1, j, i]]
+        exp = sum(1.0 if x == 1 else (2.0 if x == 0 else 0.5) for x in nb)
+        assert D[c, c] == exp
+        faces = [-u[k, j, i + 1], u[k, j, i], -v[k, j + 1, i], v[k, j, i], -w[k + 1, j, i], w[k, j, i]]
+        wts = [0.5 if x == 2 else 1.0 for x in nb]
+        assert np.isclose(b[c], sum(f * t for f, t in zip(faces, wts)), rtol=0, atol=1e-12)
+    # SpMV == dense product
+    x = np.random.default_rng(2).standard_normal(ni * nj * nk)
+    y = np.empty_like(x)
+    O.xo_spmv(ni, nj, nk, A, x, y)
+    assert np.allclose(y, D @ x, rtol=1e-14, atol=1e-13)
+
+
+def test_mic0_reduces_to_cholesky_on_a_line():
+    """A single fluid line in air, capped by one solid cell, gives a tridiagonal
+    matrix whose lexicographic elimination has no fill: with tau = 0 the
+    incomplete factor is the exact Cholesky factor (L L^T = A)."""
+    O = oracle()
+    ni, nj, nk = 9, 3, 3
+    mat = np.zeros((nk, nj, ni), np.uint8)
+    mat[1, 1, 0] = 2
+    mat[1, 1, 1:7] = 1
+    N = ni * nj * nk
+    z = np.zeros((nk, nj, ni + 1)), np.zeros((nk, nj + 1, ni)), np.zeros((nk + 1, nj, ni))
+    A, b = np.empty(7 * N), np.empty(N)
+    O.xo_assemble(ni, nj, nk, mat.ravel(), z[0].ravel(), z[1].ravel(), z[2].ravel(), A, b)
+    inv = np.empty(N)
+    O.xo_mic0(ni, nj, nk, A, 0.0, 0.0, inv)
+    D = _dense(A, ni, nj, nk)
+    rows = np.nonzero(np.diag(D))[0]
+    Ds = D[np.ix_(rows, rows)]
+    Lc = np.linalg.cholesky(Ds)
+    assert np.allclose(1.0 / np.diag(Lc), inv[rows], rtol=1e-14)
+    # the preconditioner is then an exact solve
+    r = np.zeros(N)
+    r[rows] = np.random.default_rng(3).standard_normal(len(rows))
+    q, zz = np.empty(N), np.empty(N)
+    O.xo_sweep_fwd(ni, nj, nk, A, inv, r, q)
+    O.xo_sweep_bwd(ni, nj, nk, A, inv, q, zz)
+    assert np.allclose(D[np.ix_(rows, rows)] @ zz[rows], r[rows], rtol=1e-12, atol=1e-12)
+
+
+def test_pcg_against_dense_solve():
+    O, (ni, nj, nk), mat, uvw, A, b = _random_system(4)
+    D = _dense(A, ni, nj, nk)
+    rows = np.nonzero(np.diag(D))[0]
+    x = np.empty_like(b)
+    it, res = np.ctypeslib.ctypes.c_int(), np.ctypeslib.ctypes.c_double()
+    import ctypes as C
+    it, res = C.c_int(), C.c_double()
+    st = O.xo_pcg(ni, nj, nk, A, b, x, 1e-12, 500, 0.97, 0.25, C.byref(it), C.byref(res))
+    assert st == 0
+    Ds = D[np.ix_(rows, rows)]
+    xs = np.linalg.lstsq(Ds, b[rows], rcond=None)[0]
+    # singular components (closed fluid pockets) make x unique only up to a null vector; compare residuals
+    assert np.abs(Ds @ x[rows] - b[rows]).max() <= 1e-10 * max(np.abs(b).max(), 1)
+    assert np.abs(Ds @ xs - b[rows]).max() <= 1e-8
+
+
+def test_kernel_closed_forms():
+    """W(0) = 1 and W(R^2) = 0 (1 - 4/9 + 17/9 - 22/9 = 0): a lone particle's density at
+    its own sample point is 1 and falls to 0 at the radius; P2G of uniform velocity is uniform."""
+    p = small_params(blob_radius=0.3)
+    o = oracle_dam_break(10, 10, 10, 0.1, (0.0, 0.0, 0.0), 0, p)
+    d = 2
+    # particle exactly on sub-vertex (4,4,4): x = (4+0.5)*0.05
+    x = 4.5 * 0.05
+    o.set_particles(np.array([[x, x, x]]), np.array([[1.0, -2.0, 0.5]]))
+    o.stage(STAGE["DENSITY"])
+    phi = o.get(FIELD["PHI"])
+    assert phi[4, 4, 4] == 1.0
+    assert phi.max() == 1.0
+    # all samples beyond R are exactly 0
+    idx = np.argwhere(phi > 0)
+    dist = np.sqrt((((idx + 0.5) * 0.05 - x) ** 2).sum(1))
+    assert dist.max() < 0.3
+    # uniform particle velocity -> uniform face velocity wherever weight > 1e-7
+    o2 = oracle_dam_break(10, 10, 10, 0.1, (0.7, 0.7, 1.0), 1, small_params())
+    P, V = o2.particles()
+    V[:] = (0.25, -1.5, 3.0)
+    o2.set_particles(P, V)
+    o2.stage(STAGE["P2G"])
+    for f, val in (("U", 0.25), ("V", -1.5), ("W", 3.0)):
+        a = o2.get(FIELD[f])
+        nz = a[a != 0]
+        assert nz.size > 100
+        assert np.allclose(nz, val, rtol=1e-13)
+
+
+def test_advection_affine_and_constant():
+    """Trilinear sampling reproduces affine fields; a constant field advects exactly."""
+    p = small_params(flip_ratio=0.0, tail_skip=0)
+    o = oracle_dam_break(12, 12, 12, 0.1, (0.7, 0.7, 1.0), 2, p)
+    o.stage(STAGE["MATERIAL"])
+    u = np.full(o.get(FIELD["U"]).shape, 0.3)
+    v = np.full(o.get(FIELD["V"]).shape, -0.2)
+    w = np.full(o.get(FIELD["W"]).shape, 0.1)
+    for f, a in (("U", u), ("V", v), ("W", w), ("USAVED", u), ("VSAVED", v), ("WSAVED", w)):
+        o.set(FIELD[f], a)
+    P0, _ = o.particles()
+    dt = 0.01
+    o.stage(STAGE["ADVECT"], dt)
+    P1, V1 = o.particles()
+    assert np.allclose(V1, [0.3, -0.2, 0.1], rtol=1e-14)
+    # particles whose straight move stays out of the walls advect exactly by v*dt
+    dest = np.floor((P0 + np.array([0.3, -0.2, 0.1]) * dt) / 0.1).astype(int)
+    free = np.all((dest >= 1) & (dest <= 10), axis=1)
+    assert free.sum() > 1000
+    assert np.allclose(P1[free] - P0[free], np.array([0.3, -0.2, 0.1]) * dt, rtol=0, atol=1e-14)
+    # the others were stopped at a wall and remain inside the fluid box
+    assert np.all(P1[~free] >= 0.1) and np.all(P1[~free] <= 1.1)
+
+
+def test_extrapolation_single_cell():
+    """One fluid cell: layer 1 faces take the mean of their layer-0 neighbours."""
+    o = oracle_dam_break(7, 7, 7, 0.1, (0.0, 0.0, 0.0), 0, small_params(cfl=1.0))
+    mat = o.get(FIELD["MATERIAL"])
+    mat[3, 3, 3] = 1
+    o.set(FIELD["MATERIAL"], mat)
+    u = np.zeros(o.get(FIELD["U"]).shape)
+    u[3, 3, 3] = 1.0
+    u[3, 3, 4] = 3.0
+    o.set(FIELD["U"], u)
+    o.stage(STAGE["EXTRAPOLATE"])
+    e = o.get(FIELD["U"])
+    assert e[3, 3, 2] == 1.0 and e[3, 3, 5] == 3.0
+    assert e[3, 4, 3] == 1.0 and e[4, 3, 4] == 3.0
+
+
+def test_surface_against_brute_force():
+    """Surface list = sub-cells whose 8 corners straddle iso, in coarse i-major order."""
+    o = oracle_dam_break(8, 7, 6, 0.1, (0.5, 0.5, 1.0), 3, small_params(iso=0.5))
+    o.stage(STAGE["DENSITY"])
+    phi = o.get(FIELD["PHI"])  # (k, j, i)
+    cells, cube = o.surface_cells()
+    d = 2
+    exp = []
+    for i in range(8):
+        for j in range(7):
+            for k in range(6):
+                blk = phi[k * d:k * d + d + 1, j * d:j * d + d + 1, i * d:i * d + d + 1]
+                if not ((blk >= 0.5).any() and (blk < 0.5).any()):
+                    continue
+                for a in range(i * d, i * d + d):
+                    for bb in range(j * d, j * d + d):
+                        for c in range(k * d, k * d + d):
+                            cb = phi[c:c + 2, bb:bb + 2, a:a + 2]
+                            if (cb >= 0.5).any() and (cb < 0.5).any():
+                                exp.append(a + 16 * (bb + 14 * c))
+    assert list(cells) == exp
+    # cube index bit b from corner order (i,j,k),(i+1,j,k),(i+1,j,k+1),(i,j,k+1), then +j
+    corners = [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)]
+    for cell, q in zip(cells[:200], cube[:200]):
+        a, rem = cell % 16, cell // 16
+        bb, c = rem % 14, rem // 14
+        ci = sum(1 << t for t, (di, dj, dk) in enumerate(corners) if phi[c + dk, bb + dj, a + di] > 0.5)
+        assert ci == q

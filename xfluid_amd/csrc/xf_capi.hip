@@ -1,0 +1,741 @@
+// xf_capi.hip -- the C ABI (include/xfluid.h): device state of one simulator
+// and the host orchestration of FluidSimulator::frameStep / timeStep
+// (FluidSimulator.cpp:224-335) over the gfx950 kernels.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "../../include/xfluid.h"
+#include "xf_internal.h"
+
+namespace {
+thread_local std::string g_err;
+int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return set_err(-2, "%s: %s", #x, hipGetErrorString(e_));     \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+inline uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double u01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+}  // namespace
+
+struct xf_sim {
+    Geo g{};
+    xf_params prm{};
+    int dev = 0;
+    hipStream_t st = nullptr;
+    std::vector<uint8_t> h_mat;
+    std::vector<double> forces;
+    double *u = nullptr, *v = nullptr, *w = nullptr, *us = nullptr, *vs = nullptr, *ws = nullptr;
+    uint8_t* mat = nullptr;
+    uint16_t* code = nullptr;
+    int8_t* marker = nullptr;
+    double* phi = nullptr;
+    long nphi = 0;
+    int phi_d = 0;
+    Particles P{};
+    CellIndex ci{};
+    bool sorted = false;
+    PcgWork pw{};
+    double* red = nullptr;
+    double* h_red = nullptr;
+    int32_t* surf_cnt = nullptr;
+    int32_t* surf_start = nullptr;
+    xf_solve_stats last{};
+    int frame = 0;
+    bool timing = false;
+    xf_step_timing tlast{};
+    hipEvent_t ev[XF_STAGE_COUNT + 1]{};
+    bool inited = false;
+
+    long nu() const { return (long)(g.ni + 1) * g.nj * g.nk; }
+    long nv() const { return (long)g.ni * (g.nj + 1) * g.nk; }
+    long nw() const { return (long)g.ni * g.nj * (g.nk + 1); }
+    long maxface() const { return std::max(nu(), std::max(nv(), nw())); }
+};
+
+namespace {
+const int RED_N = 1024;
+
+long phi_count(const Geo& g, int d) { return (long)(d * g.ni + 1) * (d * g.nj + 1) * (d * g.nk + 1); }
+
+int ensure_phi(xf_sim* s) {
+    int d = s->prm.sub_division;
+    if (s->phi && s->phi_d == d) return 0;
+    dfree(s->phi);
+    s->nphi = phi_count(s->g, d);
+    HIPCHK(dalloc(&s->phi, s->nphi));
+    HIPCHK(hipMemsetAsync(s->phi, 0, s->nphi * 8, s->st));
+    s->phi_d = d;
+    return 0;
+}
+
+int ensure_particles(xf_sim* s, int64_t n) {
+    Particles& P = s->P;
+    if (n <= P.cap) { P.n = n; return 0; }
+    double** d[] = {&P.px, &P.py, &P.pz, &P.vx, &P.vy, &P.vz, &P.tx, &P.ty, &P.tz, &P.tvx, &P.tvy, &P.tvz};
+    for (auto* p : d) dfree(*p);
+    dfree(P.pid); dfree(P.tpid); dfree(P.key); dfree(P.perm);
+    for (auto* p : d) HIPCHK(dalloc(p, n));
+    HIPCHK(dalloc(&P.pid, n)); HIPCHK(dalloc(&P.tpid, n)); HIPCHK(dalloc(&P.key, n)); HIPCHK(dalloc(&P.perm, n));
+    P.cap = n;
+    P.n = n;
+    return 0;
+}
+
+int upload_mat(xf_sim* s) {
+    HIPCHK(hipMemcpyAsync(s->mat, s->h_mat.data(), s->g.N, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int download_mat(xf_sim* s) {
+    HIPCHK(hipMemcpyAsync(s->h_mat.data(), s->mat, s->g.N, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+double particle_radius(const xf_sim* s) {  // FluidSimulator.cpp:380-385
+    double dx = s->g.dx;
+    double volume = dx * dx * dx / 8.0;
+    double pi = 3.141592653;
+    return std::pow(3 * volume / (4 * pi), 1.0 / 3.0);
+}
+double blob_radius(const xf_sim* s) {
+    return s->prm.blob_radius > 0 ? s->prm.blob_radius : particle_radius(s) * 3.0;
+}
+
+int ensure_sorted(xf_sim* s) {
+    if (s->sorted) return 0;
+    sort_particles(s->g, s->P, s->ci, s->st);
+    HIPCHK(hipGetLastError());
+    s->sorted = true;
+    return 0;
+}
+
+// ---- stages ----
+int st_density(xf_sim* s) {
+    int rc = ensure_phi(s);
+    if (rc) return rc;
+    rc = ensure_sorted(s);
+    if (rc) return rc;
+    launch_density(s->g, s->P, s->ci, s->phi, s->prm.sub_division, blob_radius(s), s->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+int st_material(xf_sim* s) {
+    int rc = ensure_sorted(s);
+    if (rc) return rc;
+    launch_material(s->g, s->mat, s->ci.count, s->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+int st_save(xf_sim* s) {
+    HIPCHK(hipMemcpyAsync(s->us, s->u, s->nu() * 8, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->vs, s->v, s->nv() * 8, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->ws, s->w, s->nw() * 8, hipMemcpyDeviceToDevice, s->st));
+    return 0;
+}
+int st_forces(xf_sim* s, double dt) {  // FluidSimulator.cpp:429-499
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (size_t f = 0; f < s->forces.size() / 3; ++f) {
+        ax += s->forces[3 * f]; ay += s->forces[3 * f + 1]; az += s->forces[3 * f + 2];
+    }
+    double dv[3] = {ax * dt, ay * dt, az * dt};
+    double* fa[3] = {s->u, s->v, s->w};
+    const double err = 1e-10;
+    for (int a = 0; a < 3; ++a)
+        if (dv[a] > err || dv[a] < -err) launch_forces(s->g, s->mat, fa[a], a, dv[a], s->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+int st_solve(xf_sim* s) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s->st));
+    launch_stencil_rhs(s->g, s->mat, s->u, s->v, s->w, s->code, s->pw.b, s->st);
+    const xf_params& p = s->prm;
+    PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
+                            p.mixed_inner_tol, p.mixed_max_outer, p.sweep_kernel, s->st);
+    HIPCHK(hipEventRecord(e1, s->st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    s->last.status = r.status;
+    s->last.iterations = r.iterations;
+    s->last.outer = r.outer;
+    s->last.residual = r.residual;
+    s->last.rhs_norm = r.rhs_norm;
+    s->last.ms = ms;
+    return r.status;
+}
+int st_apply(xf_sim* s) {
+    double* fa[3] = {s->u, s->v, s->w};
+    for (int a = 0; a < 3; ++a) launch_apply_pressure(s->g, s->mat, s->pw.x, fa[a], a, s->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+int st_extrapolate(xf_sim* s) {  // FluidSimulator.cpp:511-519
+    int layers = (int)(s->prm.cfl + 5);
+    if (layers > 120) return set_err(-1, "extrapolation layers %d exceed 120", layers);
+    double* fa[3] = {s->u, s->v, s->w};
+    for (int a = 0; a < 3; ++a) launch_extrapolate(s->g, s->mat, fa[a], s->marker, a, layers, s->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+int st_advect(xf_sim* s, double dt) {
+    int64_t n = s->P.n;
+    int64_t limit = s->prm.tail_skip ? (n / 6) * 6 : n;  // FluidSimulator.cpp:654-663
+    launch_g2p_advect(s->g, s->P, s->mat, s->u, s->v, s->w, s->us, s->vs, s->ws, dt, s->prm.flip_ratio,
+                      s->prm.integrator, limit, s->st);
+    HIPCHK(hipGetLastError());
+    s->sorted = false;
+    return 0;
+}
+int st_p2g(xf_sim* s) {
+    int rc = ensure_sorted(s);
+    if (rc) return rc;
+    double* fa[3] = {s->u, s->v, s->w};
+    for (int a = 0; a < 3; ++a) launch_p2g(s->g, s->P, s->ci, fa[a], a, s->prm.p2g_kernel, s->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int run_stage(xf_sim* s, int stage, double dt) {
+    switch (stage) {
+        case XF_STAGE_DENSITY: return st_density(s);
+        case XF_STAGE_MATERIAL: return st_material(s);
+        case XF_STAGE_SAVE: return st_save(s);
+        case XF_STAGE_FORCES: return st_forces(s, dt);
+        case XF_STAGE_SOLVE: return st_solve(s);
+        case XF_STAGE_APPLY: return st_apply(s);
+        case XF_STAGE_EXTRAPOLATE: return st_extrapolate(s);
+        case XF_STAGE_ADVECT: return st_advect(s, dt);
+        case XF_STAGE_P2G: return st_p2g(s);
+    }
+    return set_err(-1, "unknown stage %d", stage);
+}
+
+int check(const xf_sim* s) {
+    if (!s) return set_err(-1, "null handle");
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char* xf_last_error(void) { return g_err.c_str(); }
+
+void xf_params_default(xf_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->flip_ratio = 0.95; p->cfl = 1.0; p->max_dt = 0.5; p->pcg_tol = 1e-5; p->pcg_max_iter = 100;
+    p->sub_division = 2; p->mic_tau = 0.97; p->mic_sigma = 0.25; p->blob_radius = 0.0; p->iso = 0.4;
+    p->precision = XF_PREC_FP64; p->integrator = XF_INTEGRATOR_RK4; p->p2g_kernel = XF_P2G_WYVILL;
+    p->tail_skip = 1; p->density_every_substep = 1; p->sweep_kernel = 0;
+    p->mixed_inner_tol = 1e-3; p->mixed_max_outer = 20;
+}
+
+int xf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
+    if (!out) return set_err(-1, "null out");
+    *out = nullptr;
+    if (ni < 3 || nj < 3 || nk < 3) return set_err(-1, "grid must be at least 3x3x3 (got %d,%d,%d)", ni, nj, nk);
+    if ((long)ni * nj * nk >= (1L << 30)) return set_err(-1, "grid too large");
+    if (xf_device_count() < 1) return set_err(-3, "no HIP device: the xfluid HIP library needs an MI355X (gfx950)");
+    xf_sim* s = new xf_sim();
+    s->g.ni = ni; s->g.nj = nj; s->g.nk = nk;
+    s->g.nn = (long)ni * nj;
+    s->g.N = s->g.nn * nk;
+    s->g.dx = 0.0001;  // FluidSimulator.h:175
+    xf_params_default(&s->prm);
+    hipGetDevice(&s->dev);
+    auto fail = [&](int rc) { xf_sim_destroy(s); return rc; };
+    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(set_err(-2, "stream"));
+    const long N = s->g.N;
+    bool ok = dalloc(&s->u, s->nu()) == hipSuccess && dalloc(&s->v, s->nv()) == hipSuccess &&
+              dalloc(&s->w, s->nw()) == hipSuccess && dalloc(&s->us, s->nu()) == hipSuccess &&
+              dalloc(&s->vs, s->nv()) == hipSuccess && dalloc(&s->ws, s->nw()) == hipSuccess &&
+              dalloc(&s->mat, N) == hipSuccess && dalloc(&s->code, N) == hipSuccess &&
+              dalloc(&s->marker, s->maxface()) == hipSuccess && dalloc(&s->ci.count, N + 1) == hipSuccess &&
+              dalloc(&s->ci.start, N + 2) == hipSuccess && dalloc(&s->ci.cursor, N + 1) == hipSuccess &&
+              dalloc(&s->pw.x, N) == hipSuccess && dalloc(&s->pw.r, N) == hipSuccess &&
+              dalloc(&s->pw.s, N) == hipSuccess && dalloc(&s->pw.z, N) == hipSuccess &&
+              dalloc(&s->pw.q, N) == hipSuccess && dalloc(&s->pw.b, N) == hipSuccess &&
+              dalloc(&s->pw.inv, N) == hipSuccess && dalloc(&s->pw.x32, N) == hipSuccess &&
+              dalloc(&s->pw.r32, N) == hipSuccess && dalloc(&s->pw.s32, N) == hipSuccess &&
+              dalloc(&s->pw.z32, N) == hipSuccess && dalloc(&s->pw.q32, N) == hipSuccess &&
+              dalloc(&s->pw.inv32, N) == hipSuccess && dalloc(&s->pw.partial, RED_N) == hipSuccess &&
+              dalloc(&s->pw.S, 2) == hipSuccess && dalloc(&s->red, RED_N) == hipSuccess &&
+              dalloc(&s->surf_cnt, N + 1) == hipSuccess && dalloc(&s->surf_start, N + 2) == hipSuccess;
+    if (!ok) return fail(set_err(-2, "device allocation failed"));
+    s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
+    if (dalloc(&s->ci.scan_tmp, s->ci.scan_tmp_n) != hipSuccess) return fail(set_err(-2, "alloc scan"));
+    s->pw.npartial = RED_N;
+    s->pw.nflags = 0;
+    if (hipHostMalloc((void**)&s->pw.S_host, 2 * sizeof(PcgScalars)) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_red, RED_N * sizeof(double)) != hipSuccess)
+        return fail(set_err(-2, "pinned alloc"));
+    std::memset(s->pw.S_host, 0, 2 * sizeof(PcgScalars));
+    hipMemsetAsync(s->u, 0, s->nu() * 8, s->st);
+    hipMemsetAsync(s->v, 0, s->nv() * 8, s->st);
+    hipMemsetAsync(s->w, 0, s->nw() * 8, s->st);
+    hipMemsetAsync(s->us, 0, s->nu() * 8, s->st);
+    hipMemsetAsync(s->vs, 0, s->nv() * 8, s->st);
+    hipMemsetAsync(s->ws, 0, s->nw() * 8, s->st);
+    hipMemsetAsync(s->pw.x, 0, N * 8, s->st);
+    hipMemsetAsync(s->pw.b, 0, N * 8, s->st);
+    hipMemsetAsync(s->pw.inv, 0, N * 8, s->st);
+    hipMemsetAsync(s->code, 0, N * 2, s->st);
+    s->h_mat.assign(N, 0);
+    if (upload_mat(s)) return fail(-2);
+    if (ensure_phi(s)) return fail(-2);
+    for (auto& e : s->ev) hipEventCreate(&e);
+    if (hipStreamSynchronize(s->st) != hipSuccess) return fail(set_err(-2, "init sync"));
+    *out = s;
+    return 0;
+}
+
+void xf_sim_destroy(xf_sim* s) {
+    if (!s) return;
+    if (s->st) hipStreamSynchronize(s->st);
+    double** ds[] = {&s->u, &s->v, &s->w, &s->us, &s->vs, &s->ws, &s->phi, &s->pw.x, &s->pw.r, &s->pw.s, &s->pw.z,
+                     &s->pw.q, &s->pw.b, &s->pw.inv, &s->pw.partial, &s->red, &s->P.px, &s->P.py, &s->P.pz,
+                     &s->P.vx, &s->P.vy, &s->P.vz, &s->P.tx, &s->P.ty, &s->P.tz, &s->P.tvx, &s->P.tvy, &s->P.tvz};
+    for (auto* p : ds) dfree(*p);
+    float** fs[] = {&s->pw.x32, &s->pw.r32, &s->pw.s32, &s->pw.z32, &s->pw.q32, &s->pw.inv32};
+    for (auto* p : fs) dfree(*p);
+    int32_t** is[] = {&s->ci.count, &s->ci.start, &s->ci.cursor, &s->ci.scan_tmp, &s->P.pid, &s->P.tpid,
+                      &s->P.key, &s->P.perm, &s->surf_cnt, &s->surf_start};
+    for (auto* p : is) dfree(*p);
+    dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
+    if (s->pw.S_host) hipHostFree(s->pw.S_host);
+    if (s->h_red) hipHostFree(s->h_red);
+    for (auto& e : s->ev) if (e) hipEventDestroy(e);
+    if (s->st) hipStreamDestroy(s->st);
+    delete s;
+}
+
+int xf_sim_set_cell_width(xf_sim* s, double dx) {
+    if (check(s)) return -1;
+    if (!(dx > 0)) return set_err(-1, "dx must be positive");
+    s->g.dx = dx;
+    return 0;
+}
+int xf_sim_set_params(xf_sim* s, const xf_params* p) {
+    if (check(s) || !p) return set_err(-1, "null argument");
+    if (p->sub_division < 1 || p->sub_division > 4) return set_err(-1, "sub_division must be 1..4");
+    if (p->pcg_max_iter < 0) return set_err(-1, "pcg_max_iter < 0");
+    s->prm = *p;
+    return ensure_phi(s);
+}
+int xf_sim_get_params(const xf_sim* s, xf_params* p) {
+    if (check(s) || !p) return set_err(-1, "null argument");
+    *p = s->prm;
+    return 0;
+}
+
+// FluidSimulator.cpp:64-96
+int xf_sim_set_solid_boundary(xf_sim* s) {
+    if (check(s)) return -1;
+    if (download_mat(s)) return -2;
+    const Geo& g = s->g;
+    for (int k = 0; k < g.nk; ++k)
+        for (int j = 0; j < g.nj; ++j)
+            for (int i = 0; i < g.ni; ++i)
+                if (i == 0 || j == 0 || k == 0 || i == g.ni - 1 || j == g.nj - 1 || k == g.nk - 1)
+                    s->h_mat[cidx(g, i, j, k)] = XF_SOLID;
+    return upload_mat(s);
+}
+// FluidSimulator.cpp:103-121 (r2 held in float as in the reference)
+int xf_sim_add_sphere_fluid(xf_sim* s, double x, double y, double z, double r) {
+    if (check(s)) return -1;
+    if (download_mat(s)) return -2;
+    const Geo& g = s->g;
+    float r2 = r * r;
+    for (int i = 1; i < g.ni - 1; ++i)
+        for (int j = 1; j < g.nj - 1; ++j)
+            for (int k = 1; k < g.nk - 1; ++k) {
+                double ex = x - (i + 0.5) * g.dx, ey = y - (j + 0.5) * g.dx, ez = z - (k + 0.5) * g.dx;
+                if (ex * ex + ey * ey + ez * ez <= r2) s->h_mat[cidx(g, i, j, k)] = XF_FLUID;
+            }
+    return upload_mat(s);
+}
+// FluidSimulator.cpp:123-140
+int xf_sim_add_cubic_fluid(xf_sim* s, const double lu[3], const double rd[3]) {
+    if (check(s) || !lu || !rd) return set_err(-1, "null argument");
+    if (download_mat(s)) return -2;
+    const Geo& g = s->g;
+    for (int i = 1; i < g.ni - 1; ++i)
+        for (int j = 1; j < g.nj - 1; ++j)
+            for (int k = 1; k < g.nk - 1; ++k) {
+                double gx = (i + 0.5) * g.dx, gy = (j + 0.5) * g.dx, gz = (k + 0.5) * g.dx;
+                if ((gx <= rd[0] && gx >= lu[0]) && (gy <= rd[1] && gy >= lu[1]) && (gz <= rd[2] && gz >= lu[2]))
+                    s->h_mat[cidx(g, i, j, k)] = XF_FLUID;
+            }
+    return upload_mat(s);
+}
+int xf_sim_add_constant_force(xf_sim* s, double fx, double fy, double fz) {
+    if (check(s)) return -1;
+    s->forces.push_back(fx); s->forces.push_back(fy); s->forces.push_back(fz);
+    return 0;
+}
+
+static int upload_particles(xf_sim* s, int64_t n, const double* px, const double* py, const double* pz,
+                            const double* vx, const double* vy, const double* vz) {
+    if (n < 0 || n >= (1LL << 31)) return set_err(-1, "particle count out of range");
+    int rc = ensure_particles(s, n);
+    if (rc) return rc;
+    Particles& P = s->P;
+    const double* src[6] = {px, py, pz, vx, vy, vz};
+    double* dst[6] = {P.px, P.py, P.pz, P.vx, P.vy, P.vz};
+    for (int a = 0; a < 6; ++a)
+        if (n) HIPCHK(hipMemcpyAsync(dst[a], src[a], n * 8, hipMemcpyHostToDevice, s->st));
+    std::vector<int32_t> ids(n);
+    for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)i;
+    if (n) HIPCHK(hipMemcpyAsync(P.pid, ids.data(), n * 4, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->sorted = false;
+    return 0;
+}
+
+// FluidSimulator.cpp:168-184, 347-370 with SplitMix64 jitter
+int xf_sim_seed_particles(xf_sim* s, int mode, uint64_t seed) {
+    if (check(s)) return -1;
+    if (mode != XF_SEED_REFERENCE && mode != XF_SEED_SUBCELL) return set_err(-1, "unknown seed mode");
+    if (download_mat(s)) return -2;
+    const Geo& g = s->g;
+    const double dx = g.dx, q = 0.25 * dx;
+    const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1},
+                          {-1, 1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, 1, 1}};
+    std::vector<double> P[3];
+    for (int i = 0; i < g.ni; ++i)
+        for (int j = 0; j < g.nj; ++j)
+            for (int k = 0; k < g.nk; ++k) {
+                if (s->h_mat[cidx(g, i, j, k)] != XF_FLUID) continue;
+                double c[3] = {(i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx};
+                for (int t = 0; t < 8; ++t)
+                    for (int a = 0; a < 3; ++a) {
+                        double r = u01(seed);
+                        double v;
+                        if (mode == XF_SEED_REFERENCE) {
+                            double pt = sg[t][a] < 0 ? c[a] - q : c[a] + q;
+                            v = pt + (-0.01 + r * (0.01 - -0.01));
+                        } else {
+                            double lo = sg[t][a] < 0 ? c[a] - 0.5 * dx : c[a];
+                            v = lo + r * (0.5 * dx);
+                        }
+                        P[a].push_back(v);
+                    }
+            }
+    int64_t n = (int64_t)P[0].size();
+    std::vector<double> zero(n, 0.0);
+    return upload_particles(s, n, P[0].data(), P[1].data(), P[2].data(), zero.data(), zero.data(), zero.data());
+}
+
+// FluidSimulator::init (FluidSimulator.cpp:186-222): walls, then particles
+// (seeded from the material when none were given).
+int xf_sim_init(xf_sim* s) {
+    if (check(s)) return -1;
+    int rc = xf_sim_set_solid_boundary(s);
+    if (rc) return rc;
+    if (s->P.n == 0) {
+        rc = xf_sim_seed_particles(s, XF_SEED_REFERENCE, 0);
+        if (rc) return rc;
+    }
+    rc = ensure_phi(s);
+    s->inited = true;
+    return rc;
+}
+
+int xf_sim_stage(xf_sim* s, int stage, double dt) {
+    if (check(s)) return -1;
+    int rc = run_stage(s, stage, dt);
+    if (rc < 0) return rc;
+    hipError_t e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) return set_err(-2, "stage %d: %s", stage, hipGetErrorString(e));
+    return rc;
+}
+
+// FluidSimulator::timeStep (FluidSimulator.cpp:265-335)
+int xf_sim_time_step(xf_sim* s, double dt, int output_mesh) {
+    if (check(s)) return -1;
+    const int order[] = {XF_STAGE_DENSITY, XF_STAGE_MATERIAL, XF_STAGE_SAVE, XF_STAGE_FORCES, XF_STAGE_SOLVE,
+                         XF_STAGE_APPLY, XF_STAGE_EXTRAPOLATE, XF_STAGE_ADVECT, XF_STAGE_P2G};
+    int status = 0;
+    if (s->timing) hipEventRecord(s->ev[XF_STAGE_COUNT], s->st);
+    hipEvent_t prev = s->ev[XF_STAGE_COUNT];
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    for (int stg : order) {
+        if (stg == XF_STAGE_DENSITY && !(s->prm.density_every_substep || output_mesh)) continue;
+        if (stg == XF_STAGE_ADVECT) {
+            // sort + P2G below need the positions after advection
+        }
+        int rc = run_stage(s, stg, dt);
+        if (rc < 0) return rc;
+        if (rc > 0) status = rc;
+        if (s->timing) {
+            hipEventRecord(s->ev[stg], s->st);
+            marks.push_back({stg, s->ev[stg]});
+        }
+    }
+    hipError_t e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) return set_err(-2, "time step: %s", hipGetErrorString(e));
+    if (s->timing) {
+        std::memset(&s->tlast, 0, sizeof(s->tlast));
+        for (auto& m : marks) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, prev, m.second);
+            s->tlast.ms[m.first] = ms;
+            prev = m.second;
+        }
+        float tot = 0;
+        hipEventElapsedTime(&tot, s->ev[XF_STAGE_COUNT], prev);
+        s->tlast.total_ms = tot;
+    }
+    return status;
+}
+
+// FluidSimulator::frameStep (FluidSimulator.cpp:224-262) + _maxVelocity (:387-400)
+int xf_sim_frame_step(xf_sim* s, double dt) {
+    if (check(s)) return -1;
+    double left = dt;
+    int worst = 0;
+    while (left > 0.0) {
+        double maxv = 0.0;
+        if (s->P.n > 0) {
+            launch_maxv2(s->P, s->red, RED_N, s->st);
+            HIPCHK(hipMemcpyAsync(s->h_red, s->red, RED_N * 8, hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+            for (int i = 0; i < RED_N; ++i) maxv = s->h_red[i] > maxv ? s->h_red[i] : maxv;
+        }
+        maxv = std::sqrt(maxv);
+        double cur = maxv == 0.0 ? s->prm.max_dt : s->prm.cfl * s->g.dx / maxv;
+        cur = cur < s->prm.max_dt ? cur : s->prm.max_dt;
+        if (left <= cur) { cur = left; left = 0.0; } else left -= cur;
+        int rc = xf_sim_time_step(s, cur, left <= 0.0);
+        if (rc < 0) return rc;
+        if (rc > worst) worst = rc;
+    }
+    ++s->frame;
+    return worst;
+}
+
+int xf_sim_last_solve(const xf_sim* s, xf_solve_stats* st) {
+    if (check(s) || !st) return set_err(-1, "null argument");
+    *st = s->last;
+    return 0;
+}
+int xf_sim_set_timing(xf_sim* s, int on) {
+    if (check(s)) return -1;
+    s->timing = on != 0;
+    return 0;
+}
+int xf_sim_last_timing(const xf_sim* s, xf_step_timing* t) {
+    if (check(s) || !t) return set_err(-1, "null argument");
+    *t = s->tlast;
+    return 0;
+}
+int xf_sim_synchronize(xf_sim* s) {
+    if (check(s)) return -1;
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_frame_count(const xf_sim* s) { return s ? s->frame : -1; }
+
+int64_t xf_sim_num_particles(const xf_sim* s) { return s ? s->P.n : -1; }
+
+int xf_sim_set_particles(xf_sim* s, int64_t n, const double* px, const double* py, const double* pz,
+                         const double* vx, const double* vy, const double* vz) {
+    if (check(s)) return -1;
+    if (n > 0 && (!px || !py || !pz || !vx || !vy || !vz)) return set_err(-1, "null particle buffer");
+    return upload_particles(s, n, px, py, pz, vx, vy, vz);
+}
+
+// Particles come back in their original order (the device keeps them sorted by cell).
+int xf_sim_get_particles(xf_sim* s, double* px, double* py, double* pz, double* vx, double* vy, double* vz) {
+    if (check(s)) return -1;
+    const int64_t n = s->P.n;
+    if (n == 0) return 0;
+    std::vector<double> buf(n);
+    std::vector<int32_t> ids(n);
+    HIPCHK(hipMemcpyAsync(ids.data(), s->P.pid, n * 4, hipMemcpyDeviceToHost, s->st));
+    double* src[6] = {s->P.px, s->P.py, s->P.pz, s->P.vx, s->P.vy, s->P.vz};
+    double* dst[6] = {px, py, pz, vx, vy, vz};
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (int a = 0; a < 6; ++a) {
+        if (!dst[a]) continue;
+        HIPCHK(hipMemcpyAsync(buf.data(), src[a], n * 8, hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        for (int64_t i = 0; i < n; ++i) dst[a][ids[i]] = buf[i];
+    }
+    return 0;
+}
+
+static void* field_dev(xf_sim* s, int which, size_t* bytes) {
+    const long N = s->g.N;
+    switch (which) {
+        case XF_FIELD_U: *bytes = s->nu() * 8; return s->u;
+        case XF_FIELD_V: *bytes = s->nv() * 8; return s->v;
+        case XF_FIELD_W: *bytes = s->nw() * 8; return s->w;
+        case XF_FIELD_USAVED: *bytes = s->nu() * 8; return s->us;
+        case XF_FIELD_VSAVED: *bytes = s->nv() * 8; return s->vs;
+        case XF_FIELD_WSAVED: *bytes = s->nw() * 8; return s->ws;
+        case XF_FIELD_MATERIAL: *bytes = N; return s->mat;
+        case XF_FIELD_PHI: ensure_phi(s); *bytes = s->nphi * 8; return s->phi;
+        case XF_FIELD_PRESSURE: *bytes = N * 8; return s->pw.x;
+        case XF_FIELD_RHS: *bytes = N * 8; return s->pw.b;
+        case XF_FIELD_MIC_INV: *bytes = N * 8; return s->pw.inv;
+    }
+    *bytes = 0;
+    return nullptr;
+}
+size_t xf_sim_field_bytes(const xf_sim* s, int which) {
+    size_t b = 0;
+    if (!s) return 0;
+    field_dev(const_cast<xf_sim*>(s), which, &b);
+    return b;
+}
+int xf_sim_set_field(xf_sim* s, int which, const void* host, size_t bytes) {
+    if (check(s) || !host) return set_err(-1, "null argument");
+    size_t b = 0;
+    void* d = field_dev(s, which, &b);
+    if (!d) return set_err(-1, "unknown field %d", which);
+    if (b != bytes) return set_err(-1, "field %d holds %zu bytes, got %zu", which, b, bytes);
+    HIPCHK(hipMemcpyAsync(d, host, b, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (which == XF_FIELD_MATERIAL) std::memcpy(s->h_mat.data(), host, b);
+    return 0;
+}
+int xf_sim_get_field(xf_sim* s, int which, void* host, size_t bytes) {
+    if (check(s) || !host) return set_err(-1, "null argument");
+    size_t b = 0;
+    void* d = field_dev(s, which, &b);
+    if (!d) return set_err(-1, "unknown field %d", which);
+    if (b != bytes) return set_err(-1, "field %d holds %zu bytes, got %zu", which, b, bytes);
+    HIPCHK(hipMemcpyAsync(host, d, b, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int64_t xf_sim_surface_cells(xf_sim* s, int32_t* cells, uint8_t* cube_index, int64_t cap) {
+    if (check(s)) return -1;
+    if (ensure_phi(s)) return -2;
+    int32_t* dc = nullptr;
+    uint8_t* dq = nullptr;
+    int64_t total = surface_cells(s->g, s->phi, s->prm.sub_division, s->prm.iso, nullptr, nullptr, 0, s->surf_cnt,
+                                  s->surf_start, s->ci.scan_tmp, s->ci.scan_tmp_n, s->st);
+    if (cells && cube_index && cap > 0 && total > 0) {
+        int64_t m = std::min(cap, total);
+        if (dalloc(&dc, m) != hipSuccess || dalloc(&dq, m) != hipSuccess) {
+            dfree(dc); dfree(dq);
+            return set_err(-2, "surface alloc");
+        }
+        surface_cells(s->g, s->phi, s->prm.sub_division, s->prm.iso, dc, dq, m, s->surf_cnt, s->surf_start,
+                      s->ci.scan_tmp, s->ci.scan_tmp_n, s->st);
+        hipMemcpyAsync(cells, dc, m * 4, hipMemcpyDeviceToHost, s->st);
+        hipMemcpyAsync(cube_index, dq, m, hipMemcpyDeviceToHost, s->st);
+        hipStreamSynchronize(s->st);
+        dfree(dc); dfree(dq);
+    }
+    HIPCHK(hipGetLastError());
+    return total;
+}
+
+int xf_sim_apply_matrix(xf_sim* s, const double* x, double* y) {
+    if (check(s) || !x || !y) return set_err(-1, "null argument");
+    const long N = s->g.N;
+    HIPCHK(hipMemcpyAsync(s->pw.s, x, N * 8, hipMemcpyHostToDevice, s->st));
+    pcg_spmv(s->g, s->code, s->pw.s, s->pw.z, s->st);
+    HIPCHK(hipMemcpyAsync(y, s->pw.z, N * 8, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_apply_precond(xf_sim* s, const double* r, double* z) {
+    if (check(s) || !r || !z) return set_err(-1, "null argument");
+    const long N = s->g.N;
+    HIPCHK(hipMemcpyAsync(s->pw.s, r, N * 8, hipMemcpyHostToDevice, s->st));
+    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, s->prm.sweep_kernel, s->pw.sweep_flags, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(z, s->pw.z, N * 8, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int xf_project(int ni, int nj, int nk, const uint8_t* material, double* u, double* v, double* w, double* pressure,
+               const xf_params* p, xf_solve_stats* st) {
+    if (!material || !u || !v || !w) return set_err(-1, "null argument");
+    xf_sim* s = nullptr;
+    int rc = xf_sim_create(ni, nj, nk, &s);
+    if (rc) return rc;
+    xf_params prm;
+    if (p) prm = *p; else xf_params_default(&prm);
+    rc = xf_sim_set_params(s, &prm);
+    const long N = (long)ni * nj * nk;
+    // the reference's assembly has no range checks (variational_pressuresolver.cpp:252-292):
+    // fluid must not touch the grid boundary
+    for (int k = 0; k < nk && rc == 0; ++k)
+        for (int j = 0; j < nj && rc == 0; ++j)
+            for (int i = 0; i < ni; ++i)
+                if ((i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1) &&
+                    material[i + (long)ni * (j + (long)nj * k)] == XF_FLUID) {
+                    rc = set_err(-1, "fluid cell on the grid boundary at (%d,%d,%d)", i, j, k);
+                    break;
+                }
+    if (!rc) rc = xf_sim_set_field(s, XF_FIELD_MATERIAL, material, N);
+    if (!rc) rc = xf_sim_set_field(s, XF_FIELD_U, u, s->nu() * 8);
+    if (!rc) rc = xf_sim_set_field(s, XF_FIELD_V, v, s->nv() * 8);
+    if (!rc) rc = xf_sim_set_field(s, XF_FIELD_W, w, s->nw() * 8);
+    int status = 0;
+    if (!rc) {
+        status = xf_sim_stage(s, XF_STAGE_SOLVE, 0.0);
+        if (status < 0) rc = status;
+    }
+    if (!rc) rc = xf_sim_stage(s, XF_STAGE_APPLY, 0.0);
+    if (!rc) rc = xf_sim_get_field(s, XF_FIELD_U, u, s->nu() * 8);
+    if (!rc) rc = xf_sim_get_field(s, XF_FIELD_V, v, s->nv() * 8);
+    if (!rc) rc = xf_sim_get_field(s, XF_FIELD_W, w, s->nw() * 8);
+    if (!rc && pressure) rc = xf_sim_get_field(s, XF_FIELD_PRESSURE, pressure, N * 8);
+    if (st) *st = s->last;
+    xf_sim_destroy(s);
+    return rc ? rc : status;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// xf_device.h -- shared device helpers for the gfx950 substep kernels.
+//
+// All parity kernels are compiled with -ffp-contract=off: every expression
+// below is evaluated in the operand order of the reference expression it
+// restates, with no fused multiply-add, so per-element results are bit-exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define XF_AIR 0
+#define XF_FLUID 1
+#define XF_SOLID 2
+
+// Grid geometry passed by value to every kernel.
+struct Geo {
+    int ni, nj, nk;
+    long N;      // ni*nj*nk
+    long nn;     // ni*nj
+    double dx;
+};
+
+__host__ __device__ inline long cidx(const Geo& g, int i, int j, int k) { return i + (long)g.ni * j + g.nn * k; }
+__host__ __device__ inline bool cin(const Geo& g, int i, int j, int k) {
+    return i >= 0 && j >= 0 && k >= 0 && i < g.ni && j < g.nj && k < g.nk;
+}
+
+// xfluidutil.cpp:3-21
+__device__ inline int pos2idx(double p, double dx) { return (int)(p / dx); }
+__device__ inline double idx2pos(int i, double dx) { return (i + 0.5) * dx; }
+
+// xfluidutil.cpp:53-65
+__device__ inline double trilerp8(const double* p, double x, double y, double z) {
+    double v0 = (1.0 - x) * p[0] + x * p[1];
+    double v1 = (1.0 - x) * p[2] + x * p[3];
+    double v2 = (1.0 - x) * p[4] + x * p[5];
+    double v3 = (1.0 - x) * p[6] + x * p[7];
+    v0 = (1.0 - y) * v0 + y * v1;
+    v1 = (1.0 - y) * v2 + y * v3;
+    return (1.0 - z) * v0 + z * v1;
+}
+
+// Wyvill "soft object" kernel, FluidSimulator.cpp:917-932 and
+// directeddistancefield.cpp:27-37,246-256.
+struct Wyvill {
+    double r2, c0, c1, c2;
+};
+__host__ __device__ inline Wyvill make_wyvill(double r) {
+    Wyvill w;
+    w.r2 = r * r;
+    w.c0 = (4.0 / 9.0) * (1.0 / (w.r2 * w.r2 * w.r2));
+    w.c1 = (17.0 / 9.0) * (1.0 / (w.r2 * w.r2));
+    w.c2 = (22.0 / 9.0) * (1.0 / (w.r2));
+    return w;
+}
+__device__ inline double wyvill(const Wyvill& w, double s) {
+    return s < w.r2 ? 1.0 - w.c0 * s * s * s + w.c1 * s * s - w.c2 * s : 0.0;
+}
+
+// Stencil code of a pressure row: 2 bits per neighbour direction giving the
+// off-diagonal coefficient class (0: none, 1: -0.5, 2: -1.0), plus the cell's
+// material in bits 12-13.  Direction order of the bit fields:
+//   0: -k (c-nn), 1: -j (c-ni), 2: -i (c-1), 3: +i (c+1), 4: +j (c+ni), 5: +k (c+nn)
+// i.e. the CSR column order of sparse_matrix.h:266-276 with the diagonal taken out.
+enum { D_ZM = 0, D_YM = 1, D_XM = 2, D_XP = 3, D_YP = 4, D_ZP = 5 };
+__host__ __device__ inline int code_dir(uint16_t c, int d) { return (c >> (2 * d)) & 3; }
+__host__ __device__ inline int code_mat(uint16_t c) { return (c >> 12) & 3; }
+__host__ __device__ inline double code_coef(int cls) { return cls == 2 ? -1.0 : (cls == 1 ? -0.5 : 0.0); }
+// Diagonal: fluid rows sum fluid 1 / air 2 / solid 0.5 over the six faces,
+// solid rows 0.5 per fluid neighbour (variational_pressuresolver.cpp:294-321).
+// All terms are exact halves, so the order of the sum does not matter.
+__host__ __device__ inline double code_diag(uint16_t c) {
+    int m = code_mat(c);
+    double s = 0.0;
+    if (m == XF_FLUID) {
+        for (int d = 0; d < 6; ++d) {
+            int cl = code_dir(c, d);
+            s += cl == 2 ? 1.0 : (cl == 1 ? 0.5 : 2.0);
+        }
+    } else if (m == XF_SOLID) {
+        for (int d = 0; d < 6; ++d) s += code_dir(c, d) == 1 ? 0.5 : 0.0;
+    }
+    return s;
+}
+
+template <class T>
+__device__ inline T warp_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ inline T warp_max(T v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_down(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Block reduction (blockDim multiple of 64, <= 1024) with a fixed tree order,
+// so the result is deterministic for a fixed launch shape.
+template <class T, bool MAX>
+__device__ inline T block_reduce(T v, T* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = MAX ? warp_max(v) : warp_sum(v);
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        v = lane < nw ? sh[lane] : (T)0;
+        v = MAX ? warp_max(v) : warp_sum(v);
+    }
+    __syncthreads();
+    return v;  // valid in thread 0
+}

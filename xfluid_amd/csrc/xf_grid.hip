@@ -1,0 +1,204 @@
+// xf_grid.hip -- per-cell and per-face stage kernels of the substep.
+//
+// Each kernel is a gather (one thread per output cell or face), so results are
+// deterministic and, compiled with -ffp-contract=off, bit-identical to the
+// serial reference loops they restate.
+#include "xf_internal.h"
+
+// ---------------------------------------------------------------------------
+// _updateMaterial (FluidSimulator.cpp:402-427): every non-solid cell becomes
+// air, then any cell holding a particle becomes fluid.  The particle count per
+// cell comes from the cell sort, so no atomics touch the material.
+__global__ void k_material(Geo g, uint8_t* mat, const int32_t* cnt) {
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    uint8_t m = mat[c];
+    if (m != XF_SOLID) mat[c] = cnt[c] > 0 ? XF_FLUID : XF_AIR;
+}
+void launch_material(const Geo& g, uint8_t* mat, const int32_t* cell_count, hipStream_t st) {
+    k_material<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, mat, cell_count);
+}
+
+__device__ inline uint8_t mat_at(const Geo& g, const uint8_t* mat, int i, int j, int k, uint8_t outside) {
+    return cin(g, i, j, k) ? mat[cidx(g, i, j, k)] : outside;
+}
+
+// ---------------------------------------------------------------------------
+// _applyForces (FluidSimulator.cpp:429-499): faces bordering a fluid cell get
+// += a*dt along the axis.
+__global__ void k_forces(Geo g, const uint8_t* mat, double* f, int axis, double dv) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nf = (long)si * sj * sk;
+    if (t >= nf) return;
+    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int oi = axis == 0, oj = axis == 1, ok = axis == 2;
+    bool a = mat_at(g, mat, i, j, k, XF_AIR) == XF_FLUID;
+    bool b = mat_at(g, mat, i - oi, j - oj, k - ok, XF_AIR) == XF_FLUID;
+    if (a || b) f[t] += dv;
+}
+void launch_forces(const Geo& g, const uint8_t* mat, double* f, int axis, double dv, hipStream_t st) {
+    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+    k_forces<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, f, axis, dv);
+}
+
+// ---------------------------------------------------------------------------
+// Stencil codes + right-hand side, variational_pressuresolver.cpp:231-326.
+// The reference visits fluid cells with i slowest and scatters into its own
+// row and into solid neighbours' rows.  As a gather:
+//  * a fluid row sums its six face terms in neighbour order +i,-i,+j,-j,+k,-k;
+//  * a solid row receives one term per fluid neighbour, in the order the
+//    reference visits those neighbours (i-major loop): -i,-j,-k,+k,+j,+i.
+// Direction classes stored in the code (2 bits each, see xf_device.h):
+//   fluid row: 0 air neighbour, 1 solid (-0.5), 2 fluid (-1), 3 outside grid;
+//   solid row: 1 fluid neighbour (-0.5), else 0.
+__device__ inline double face_u(const Geo& g, const double* u, int i, int j, int k) {
+    return u[i + (long)(g.ni + 1) * (j + (long)g.nj * k)];
+}
+__device__ inline double face_v(const Geo& g, const double* v, int i, int j, int k) {
+    return v[i + (long)g.ni * (j + (long)(g.nj + 1) * k)];
+}
+__device__ inline double face_w(const Geo& g, const double* w, int i, int j, int k) {
+    return w[i + (long)g.ni * (j + (long)g.nj * k)];
+}
+
+__global__ void k_stencil_rhs(Geo g, const uint8_t* mat, const double* u, const double* v, const double* w,
+                              uint16_t* code, double* b) {
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    uint8_t m = mat[c];
+    const uint8_t OUT = 3;
+    // neighbour materials in code direction order: -k,-j,-i,+i,+j,+k
+    uint8_t nb[6];
+    nb[D_ZM] = mat_at(g, mat, i, j, k - 1, OUT);
+    nb[D_YM] = mat_at(g, mat, i, j - 1, k, OUT);
+    nb[D_XM] = mat_at(g, mat, i - 1, j, k, OUT);
+    nb[D_XP] = mat_at(g, mat, i + 1, j, k, OUT);
+    nb[D_YP] = mat_at(g, mat, i, j + 1, k, OUT);
+    nb[D_ZP] = mat_at(g, mat, i, j, k + 1, OUT);
+    // face velocity on each side
+    double fc[6];
+    fc[D_ZM] = face_w(g, w, i, j, k);
+    fc[D_ZP] = face_w(g, w, i, j, k + 1);
+    fc[D_YM] = face_v(g, v, i, j, k);
+    fc[D_YP] = face_v(g, v, i, j + 1, k);
+    fc[D_XM] = face_u(g, u, i, j, k);
+    fc[D_XP] = face_u(g, u, i + 1, j, k);
+    uint16_t cd = (uint16_t)((m & 3) << 12);
+    double bb = 0;
+    if (m == XF_FLUID) {
+        for (int d = 0; d < 6; ++d) {
+            int cl = nb[d] == XF_FLUID ? 2 : nb[d] == XF_SOLID ? 1 : nb[d] == OUT ? 3 : 0;
+            cd |= (uint16_t)(cl << (2 * d));
+        }
+        // reference neighbour order +i,-i,+j,-j,+k,-k; vsign -1 on the + side
+        const int ord[6] = {D_XP, D_XM, D_YP, D_YM, D_ZP, D_ZM};
+        for (int e = 0; e < 6; ++e) {
+            int d = ord[e];
+            double vsign = (e & 1) ? 1.0 : -1.0;
+            uint8_t nm = nb[d];
+            if (nm == XF_FLUID || nm == XF_AIR) bb += vsign * fc[d];
+            else bb += vsign * fc[d] * 0.5;
+        }
+    } else if (m == XF_SOLID) {
+        for (int d = 0; d < 6; ++d)
+            if (nb[d] == XF_FLUID) cd |= (uint16_t)(1 << (2 * d));
+        // fluid neighbour f sees this cell on its opposite side: its vsign is
+        // -1 when this cell is f's + neighbour (f on our - side), +1 otherwise
+        const int ord[6] = {D_XM, D_YM, D_ZM, D_ZP, D_YP, D_XP};
+        for (int e = 0; e < 6; ++e) {
+            int d = ord[e];
+            if (nb[d] != XF_FLUID) continue;
+            double vsign_f = (d == D_XM || d == D_YM || d == D_ZM) ? -1.0 : 1.0;
+            bb += -vsign_f * fc[d] * 0.5;
+        }
+    }
+    code[c] = cd;
+    b[c] = bb;
+}
+void launch_stencil_rhs(const Geo& g, const uint8_t* mat, const double* u, const double* v, const double* w,
+                        uint16_t* code, double* b, hipStream_t st) {
+    k_stencil_rhs<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, mat, u, v, w, code, b);
+}
+
+// ---------------------------------------------------------------------------
+// applyPressure (variational_pressuresolver.cpp:120-204), factor 1.0; cells
+// outside the grid count as solid; air ghost pressure = -p(fluid).
+__global__ void k_apply_pressure(Geo g, const uint8_t* mat, const double* p, double* f, int axis) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nf = (long)si * sj * sk;
+    if (t >= nf) return;
+    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
+    uint8_t ml = mat_at(g, mat, i + di, j + dj, k + dk, XF_SOLID), mr = mat_at(g, mat, i, j, k, XF_SOLID);
+    if (ml != XF_FLUID && mr != XF_FLUID) return;
+    double p1, p2;
+    if (ml == XF_FLUID && mr == XF_FLUID) {
+        p2 = p[cidx(g, i, j, k)];
+        p1 = p[cidx(g, i + di, j + dj, k + dk)];
+    } else if (ml == XF_FLUID) {
+        p1 = p[cidx(g, i + di, j + dj, k + dk)];
+        p2 = (mr == XF_SOLID) ? p[cidx(g, i, j, k)] : -p1;
+    } else {
+        p2 = p[cidx(g, i, j, k)];
+        p1 = (ml == XF_SOLID) ? p[cidx(g, i + di, j + dj, k + dk)] : -p2;
+    }
+    f[t] = f[t] - 1.0 * (p2 - p1);
+}
+void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st) {
+    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+    k_apply_pressure<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, p, f, axis);
+}
+
+// ---------------------------------------------------------------------------
+// Velocity extrapolation, FluidSimulator.cpp:511-650.  Layer 0 = faces of
+// fluid cells.  The reference grows the layers breadth-first; a face of layer
+// L is a still-unmarked face next to a layer L-1 face, and its value is the
+// mean over its layer L-1 neighbours (order +i,-i,+j,-j,+k,-k).  Each layer
+// reads only finalised layer L-1 values, so one gather kernel per layer gives
+// the same values.  In-place marking is safe: an unmarked face only looks for
+// neighbours marked L-1, and markers change from -1 to L within the launch.
+__global__ void k_extrap_init(Geo g, const uint8_t* mat, int8_t* mark, int axis) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nf = (long)si * sj * sk;
+    if (t >= nf) return;
+    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int oi = axis == 0, oj = axis == 1, ok = axis == 2;
+    bool a = mat_at(g, mat, i, j, k, XF_AIR) == XF_FLUID;
+    bool b = mat_at(g, mat, i - oi, j - oj, k - ok, XF_AIR) == XF_FLUID;
+    mark[t] = (a || b) ? 0 : -1;
+}
+
+__global__ void k_extrap_layer(Geo g, double* f, int8_t* mark, int axis, int L) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nf = (long)si * sj * sk;
+    if (t >= nf) return;
+    if (mark[t] != -1) return;
+    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    const long sij = (long)si * sj;
+    double sum = 0.0;
+    int count = 0;
+    const int8_t want = (int8_t)(L - 1);
+    // +i,-i,+j,-j,+k,-k
+    if (i + 1 < si && mark[t + 1] == want) { sum += f[t + 1]; count += 1; }
+    if (i - 1 >= 0 && mark[t - 1] == want) { sum += f[t - 1]; count += 1; }
+    if (j + 1 < sj && mark[t + si] == want) { sum += f[t + si]; count += 1; }
+    if (j - 1 >= 0 && mark[t - si] == want) { sum += f[t - si]; count += 1; }
+    if (k + 1 < sk && mark[t + sij] == want) { sum += f[t + sij]; count += 1; }
+    if (k - 1 >= 0 && mark[t - sij] == want) { sum += f[t - sij]; count += 1; }
+    if (count > 0) {
+        f[t] = sum / (double)count;
+        mark[t] = (int8_t)L;
+    }
+}
+
+void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
+                        hipStream_t st) {
+    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+    k_extrap_init<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, marker, axis);
+    for (int L = 1; L <= layers; ++L) k_extrap_layer<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, f, marker, axis, L);
+}

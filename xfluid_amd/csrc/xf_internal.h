@@ -1,0 +1,82 @@
+// xf_internal.h -- host-side launchers shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "xf_device.h"
+
+#define XF_BLOCK 256
+
+inline unsigned xf_blocks(long n, int per = XF_BLOCK) { return (unsigned)((n + per - 1) / per); }
+
+// Scalars of one PCG solve, resident in device memory (no host round trip per
+// iteration).  `done` stops all later kernels of the solve.
+struct PcgScalars {
+    double rho, sz, alpha, beta, res, tol, rho_new, res0;
+    int done, iters, status, it;
+};
+
+// ---------------- grid stage kernels (xf_grid.hip) ----------------
+void launch_material(const Geo& g, uint8_t* mat, const int32_t* cell_count, hipStream_t st);
+void launch_forces(const Geo& g, const uint8_t* mat, double* f, int axis, double dv, hipStream_t st);
+void launch_stencil_rhs(const Geo& g, const uint8_t* mat, const double* u, const double* v, const double* w,
+                        uint16_t* code, double* b, hipStream_t st);
+void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st);
+void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
+                        hipStream_t st);
+
+// ---------------- pressure solve kernels (xf_pcg.hip) ----------------
+struct PcgWork {
+    // FP64 vectors
+    double *x, *r, *s, *z, *q, *b, *inv;
+    // FP32 copies for the mixed-precision inner solve
+    float *x32, *r32, *s32, *z32, *q32, *inv32;
+    double* partial;     // per-block partial sums / maxima
+    int npartial;
+    PcgScalars* S;       // device scalars
+    PcgScalars* S_host;  // pinned mirror
+    int* sweep_flags;    // progress counters of the pipelined sweep
+    int nflags;
+};
+struct PcgResult {
+    int status, iterations, outer;
+    double residual, rhs_norm;
+};
+// Assemble the stencil codes and right-hand side, factor, and solve A x = b.
+PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,
+                    double sigma, int precision, double inner_tol, int max_outer, int sweep_kernel, hipStream_t st);
+// Individual pieces, for parity checks.
+void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, hipStream_t st);
+void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
+                 int sweep_kernel, int* flags, hipStream_t st);
+void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st);
+
+// ---------------- particle kernels (xf_particles.hip) ----------------
+struct Particles {
+    int64_t n, cap;
+    double *px, *py, *pz, *vx, *vy, *vz;
+    int32_t* pid;
+    // sort scratch
+    double *tx, *ty, *tz, *tvx, *tvy, *tvz;
+    int32_t *tpid, *key, *perm;
+};
+struct CellIndex {
+    int32_t* count;   // N+1 (last bin: out of range)
+    int32_t* start;   // N+2
+    int32_t* cursor;  // N+1
+    int32_t* scan_tmp;
+    int scan_tmp_n;
+};
+void sort_particles(const Geo& g, Particles& P, CellIndex& ci, hipStream_t st);
+void launch_g2p_advect(const Geo& g, Particles& P, const uint8_t* mat, const double* u, const double* v,
+                       const double* w, const double* us, const double* vs, const double* ws, double dt, double flip,
+                       int integrator, int64_t limit_id, hipStream_t st);
+void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f, int axis, int kernel,
+                hipStream_t st);
+void launch_density(const Geo& g, const Particles& P, const CellIndex& ci, double* phi, int d, double radius,
+                    hipStream_t st);
+int64_t surface_cells(const Geo& g, const double* phi, int d, double iso, int32_t* cells, uint8_t* cube,
+                      int64_t cap, int32_t* count_tmp, int32_t* start_tmp, int32_t* scan_tmp, int scan_tmp_n,
+                      hipStream_t st);
+void launch_maxv2(const Particles& P, double* partial, int nblocks, hipStream_t st);
+void exclusive_scan_i32(const int32_t* in, int32_t* out, long n, int32_t* tmp, int tmp_n, hipStream_t st);
+long scan_tmp_size(long n);

@@ -1,0 +1,554 @@
+// xf_particles.hip -- particle kernels: cell sort, G2P + advection, P2G and
+// density gathers, surface classification with prefix-sum compaction.
+//
+// Particles live as SoA FP64 arrays, kept sorted by (cell, original index)
+// after every advection, so that the P2G and density gathers read each cell's
+// particles contiguously and sum them in a fixed order (deterministic results,
+// tolerance parity with the reference's particle-order sums).
+#include "xf_internal.h"
+
+// ---------------------------------------------------------------------------
+// Exclusive prefix sum of int32 with wavefront (DPP-shuffle) scans inside each
+// block: 256 threads x 4 items per block, then a single-block scan of block
+// totals, then the add-back.
+#define SCAN_ITEMS 4
+#define SCAN_TILE (XF_BLOCK * SCAN_ITEMS)
+
+__device__ inline int32_t wave_incl_scan(int32_t v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        int32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+// Block exclusive scan of one value per thread; returns the prefix and the total.
+__device__ inline int32_t block_excl_scan(int32_t v, int32_t* sh, int32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int32_t inc = wave_incl_scan(v);
+    if (lane == 63) sh[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        int32_t s = lane < nw ? sh[lane] : 0;
+        int32_t si = wave_incl_scan(s);
+        if (lane < nw) sh[lane] = si - s;
+        if (lane == nw - 1) sh[16] = si;
+    }
+    __syncthreads();
+    int32_t r = inc - v + sh[wid];
+    *total = sh[16];
+    __syncthreads();
+    return r;
+}
+
+__global__ void k_scan_tiles(const int32_t* in, long n, int32_t* tile_sum) {
+    __shared__ int32_t sh[17];
+    long base = blockIdx.x * (long)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int32_t s = 0;
+    for (int t = 0; t < SCAN_ITEMS; ++t) if (base + t < n) s += in[base + t];
+    int32_t tot;
+    block_excl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+__global__ void k_scan_sums(int32_t* tile_sum, int ntiles) {
+    __shared__ int32_t sh[17];
+    // one block of 1024: each thread owns a contiguous run
+    int per = (ntiles + blockDim.x - 1) / blockDim.x;
+    int b0 = threadIdx.x * per;
+    int32_t s = 0;
+    for (int t = 0; t < per; ++t) if (b0 + t < ntiles) s += tile_sum[b0 + t];
+    int32_t tot;
+    int32_t pre = block_excl_scan(s, sh, &tot);
+    for (int t = 0; t < per; ++t)
+        if (b0 + t < ntiles) { int32_t v = tile_sum[b0 + t]; tile_sum[b0 + t] = pre; pre += v; }
+}
+__global__ void k_scan_apply(const int32_t* in, int32_t* out, long n, const int32_t* tile_sum) {
+    __shared__ int32_t sh[17];
+    long base = blockIdx.x * (long)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int32_t v[SCAN_ITEMS];
+    int32_t s = 0;
+    for (int t = 0; t < SCAN_ITEMS; ++t) { v[t] = base + t < n ? in[base + t] : 0; s += v[t]; }
+    int32_t tot;
+    int32_t pre = block_excl_scan(s, sh, &tot) + tile_sum[blockIdx.x];
+    for (int t = 0; t < SCAN_ITEMS; ++t) {
+        if (base + t < n) out[base + t] = pre;
+        pre += v[t];
+    }
+    // out[n] = total, written by the block that owns element n-1
+    if (base <= n - 1 && n - 1 < base + SCAN_ITEMS) out[n] = pre;
+}
+
+long scan_tmp_size(long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// out has n+1 entries; out[n] = sum of all.
+void exclusive_scan_i32(const int32_t* in, int32_t* out, long n, int32_t* tmp, int tmp_n, hipStream_t st) {
+    int tiles = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
+    (void)tmp_n;
+    k_scan_tiles<<<tiles, XF_BLOCK, 0, st>>>(in, n, tmp);
+    k_scan_sums<<<1, 1024, 0, st>>>(tmp, tiles);
+    k_scan_apply<<<tiles, XF_BLOCK, 0, st>>>(in, out, n, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// Counting sort by cell: key = pos2Index cell (xfluidutil.cpp:3-6) or N when
+// outside the grid; within a cell, ascending original index.
+__global__ void k_keys(Geo g, long n, const double* px, const double* py, const double* pz, int32_t* key,
+                       int32_t* cnt) {
+    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int i = pos2idx(px[p], g.dx), j = pos2idx(py[p], g.dx), k = pos2idx(pz[p], g.dx);
+    int32_t kk = cin(g, i, j, k) ? (int32_t)cidx(g, i, j, k) : (int32_t)g.N;
+    key[p] = kk;
+    atomicAdd(&cnt[kk], 1);
+}
+__global__ void k_scatter(long n, const int32_t* key, const int32_t* start, int32_t* cursor, int32_t* perm) {
+    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t kk = key[p];
+    perm[start[kk] + atomicAdd(&cursor[kk], 1)] = (int32_t)p;
+}
+__global__ void k_sort_cells(long nbins, const int32_t* start, int32_t* perm, const int32_t* pid) {
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= nbins) return;
+    int32_t a = start[c], b = start[c + 1];
+    for (int32_t x = a + 1; x < b; ++x) {
+        int32_t v = perm[x];
+        int32_t key = pid[v];
+        int32_t y = x - 1;
+        while (y >= a && pid[perm[y]] > key) { perm[y + 1] = perm[y]; --y; }
+        perm[y + 1] = v;
+    }
+}
+__global__ void k_gather(long n, const int32_t* perm, const double* px, const double* py, const double* pz,
+                         const double* vx, const double* vy, const double* vz, const int32_t* pid, double* tx,
+                         double* ty, double* tz, double* tvx, double* tvy, double* tvz, int32_t* tpid) {
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int32_t p = perm[s];
+    tx[s] = px[p]; ty[s] = py[p]; tz[s] = pz[p];
+    tvx[s] = vx[p]; tvy[s] = vy[p]; tvz[s] = vz[p];
+    tpid[s] = pid[p];
+}
+template <class T>
+__global__ void k_zero(long n, T* a) {
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s < n) a[s] = 0;
+}
+
+void sort_particles(const Geo& g, Particles& P, CellIndex& ci, hipStream_t st) {
+    const long nb = g.N + 1;
+    k_zero<int32_t><<<xf_blocks(nb), XF_BLOCK, 0, st>>>(nb, ci.count);
+    k_zero<int32_t><<<xf_blocks(nb), XF_BLOCK, 0, st>>>(nb, ci.cursor);
+    if (P.n > 0) k_keys<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(g, P.n, P.px, P.py, P.pz, P.key, ci.count);
+    exclusive_scan_i32(ci.count, ci.start, nb, ci.scan_tmp, ci.scan_tmp_n, st);
+    if (P.n == 0) return;
+    k_scatter<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.key, ci.start, ci.cursor, P.perm);
+    k_sort_cells<<<xf_blocks(nb), XF_BLOCK, 0, st>>>(nb, ci.start, P.perm, P.pid);
+    k_gather<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.perm, P.px, P.py, P.pz, P.vx, P.vy, P.vz, P.pid, P.tx,
+                                                   P.ty, P.tz, P.tvx, P.tvy, P.tvz, P.tpid);
+    std::swap(P.px, P.tx); std::swap(P.py, P.ty); std::swap(P.pz, P.tz);
+    std::swap(P.vx, P.tvx); std::swap(P.vy, P.tvy); std::swap(P.vz, P.tvz);
+    std::swap(P.pid, P.tpid);
+}
+
+// ---------------------------------------------------------------------------
+// _getVelocity (FluidSimulator.cpp:814-894): staggered trilinear sample with
+// the reference's clamps; outside the box the velocity is 0.
+struct Faces {
+    const double *u, *v, *w;
+};
+__device__ inline void sample(const Geo& g, const Faces& F, const double* p, double* out) {
+    const double dx = g.dx;
+    if (!((p[0] >= 0.0) && (p[1] >= 0.0) && (p[2] >= 0.0) && (p[0] <= dx * g.ni) && (p[1] <= dx * g.nj) &&
+          (p[2] <= dx * g.nk))) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double invdx = 1.0 / dx;
+    double t[8], x, y, z;
+    int i, j, k;
+    x = p[0] * invdx; y = (p[1] - 0.5 * dx) * invdx; z = (p[2] - 0.5 * dx) * invdx;
+    i = (int)x;
+    j = y < 0.0 ? 0 : (int)y; j = j > (g.nj - 2) ? (g.nj - 2) : j;
+    k = z < 0.0 ? 0 : (int)z; k = k > (g.nk - 2) ? (g.nk - 2) : k;
+    x -= i; y -= j; z -= k;
+    {
+        const long si = g.ni + 1, sij = si * g.nj;
+        const double* b = F.u + i + si * j + sij * k;
+        t[0] = b[0]; t[1] = b[1]; t[2] = b[si]; t[3] = b[si + 1];
+        t[4] = b[sij]; t[5] = b[sij + 1]; t[6] = b[sij + si]; t[7] = b[sij + si + 1];
+    }
+    out[0] = trilerp8(t, x, y, z);
+    x = (p[0] - 0.5 * dx) * invdx; y = p[1] * invdx; z = (p[2] - 0.5 * dx) * invdx;
+    j = (int)y;
+    i = x < 0.0 ? 0 : (int)x; i = i > (g.ni - 2) ? (g.ni - 2) : i;
+    k = z < 0.0 ? 0 : (int)z; k = k > (g.nk - 2) ? (g.nk - 2) : k;
+    x -= i; y -= j; z -= k;
+    {
+        const long si = g.ni, sij = si * (g.nj + 1);
+        const double* b = F.v + i + si * j + sij * k;
+        t[0] = b[0]; t[1] = b[1]; t[2] = b[si]; t[3] = b[si + 1];
+        t[4] = b[sij]; t[5] = b[sij + 1]; t[6] = b[sij + si]; t[7] = b[sij + si + 1];
+    }
+    out[1] = trilerp8(t, x, y, z);
+    z = p[2] * invdx; y = (p[1] - 0.5 * dx) * invdx; x = (p[0] - 0.5 * dx) * invdx;
+    k = (int)z;
+    j = y < 0.0 ? 0 : (int)y; j = j > (g.nj - 2) ? (g.nj - 2) : j;
+    i = x < 0.0 ? 0 : (int)x; i = i > (g.ni - 2) ? (g.ni - 2) : i;
+    x -= i; y -= j; z -= k;
+    {
+        const long si = g.ni, sij = si * g.nj;
+        const double* b = F.w + i + si * j + sij * k;
+        t[0] = b[0]; t[1] = b[1]; t[2] = b[si]; t[3] = b[si + 1];
+        t[4] = b[sij]; t[5] = b[sij + 1]; t[6] = b[sij + si]; t[7] = b[sij + si + 1];
+    }
+    out[2] = trilerp8(t, x, y, z);
+}
+
+// collision.cpp:293-385 (voxel walk to the first solid cell)
+__device__ bool voxel_hit(const Geo& g, const uint8_t* mat, const double* a, const double* b, int* vox) {
+    const double invdx = 1.0 / g.dx;
+    double p0[3] = {a[0] * invdx, a[1] * invdx, a[2] * invdx};
+    double p1[3] = {b[0] * invdx, b[1] * invdx, b[2] * invdx};
+    int g0[3], g1[3], sd[3], gg[3];
+    double gp[3], vv[3];
+    for (int t = 0; t < 3; ++t) {
+        g0[t] = (int)floor(p0[t]); g1[t] = (int)floor(p1[t]);
+        sd[t] = g1[t] > g0[t] ? 1 : g1[t] < g0[t] ? -1 : 0;
+        gg[t] = g0[t];
+        gp[t] = g0[t] + (g1[t] > g0[t] ? 1 : 0);
+        vv[t] = p1[t] == p0[t] ? 1 : p1[t] - p0[t];
+    }
+    double vxvy = vv[0] * vv[1], vxvz = vv[0] * vv[2], vyvz = vv[1] * vv[2];
+    double errx = (gp[0] - p0[0]) * vyvz, erry = (gp[1] - p0[1]) * vxvz, errz = (gp[2] - p0[2]) * vxvy;
+    double derrx = sd[0] * vyvz, derry = sd[1] * vxvz, derrz = sd[2] * vxvy;
+    // the walk ends after |dg| steps; bound it for safety
+    int budget = abs(g1[0] - g0[0]) + abs(g1[1] - g0[1]) + abs(g1[2] - g0[2]) + 1;
+    for (int step = 0; step <= budget; ++step) {
+        if (cin(g, gg[0], gg[1], gg[2]) && mat[cidx(g, gg[0], gg[1], gg[2])] == XF_SOLID) {
+            vox[0] = gg[0]; vox[1] = gg[1]; vox[2] = gg[2];
+            return true;
+        }
+        if (gg[0] == g1[0] && gg[1] == g1[1] && gg[2] == g1[2]) break;
+        double xr = fabs(errx), yr = fabs(erry), zr = fabs(errz);
+        if (sd[0] != 0 && (sd[1] == 0 || xr < yr) && (sd[2] == 0 || xr < zr)) { gg[0] += sd[0]; errx += derrx; }
+        else if (sd[1] != 0 && (sd[2] == 0 || yr < zr)) { gg[1] += sd[1]; erry += derry; }
+        else if (sd[2] != 0) { gg[2] += sd[2]; errz += derrz; }
+    }
+    return false;
+}
+
+// collision.cpp:396-439, including its repeated dir.x guard
+__device__ bool ray_box(const double* p0, const double* din, const double* bmin, const double* bmax, double* hit) {
+    double dir[3] = {din[0], din[1], din[2]};
+    const double eps = 1e-10;
+    if (fabs(dir[0]) < eps) dir[0] = dir[0] < 0 ? -eps : eps;
+    if (fabs(dir[1]) < eps) dir[1] = dir[1] < 0 ? -eps : eps;
+    if (fabs(dir[0]) < eps) dir[2] = dir[2] < 0 ? -eps : eps;
+    double inv[3] = {1.0 / dir[0], 1.0 / dir[1], 1.0 / dir[2]};
+    double t1 = (bmin[0] - p0[0]) * inv[0], t2 = (bmax[0] - p0[0]) * inv[0];
+    double tmin = fmin(t1, t2), tmax = fmax(t1, t2);
+    t1 = (bmin[1] - p0[1]) * inv[1]; t2 = (bmax[1] - p0[1]) * inv[1];
+    tmin = fmax(tmin, fmin(t1, t2)); tmax = fmin(tmax, fmax(t1, t2));
+    t1 = (bmin[2] - p0[2]) * inv[2]; t2 = (bmax[2] - p0[2]) * inv[2];
+    tmin = fmax(tmin, fmin(t1, t2)); tmax = fmin(tmax, fmax(t1, t2));
+    if (tmax > fmax(tmin, 0.0)) {
+        for (int a = 0; a < 3; ++a) hit[a] = p0[a] + dir[a] * tmin;
+        return true;
+    }
+    return false;
+}
+
+// _advectParticles (FluidSimulator.cpp:652-723): FLIP/PIC blend, RK4 (or RK2
+// midpoint) advection, collision resolution.  Particles whose original index
+// is >= limit_id are left untouched (the reference's 6-chunk tail).
+__global__ void k_g2p_advect(Geo g, long n, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
+                             const int32_t* pid, const uint8_t* mat, Faces Fn, Faces Fo, double dt, double flip,
+                             int integrator, long limit_id) {
+    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (pid[p] >= limit_id) return;
+    double pos[3] = {px[p], py[p], pz[p]};
+    double vn[3], vo[3];
+    sample(g, Fn, pos, vn);
+    sample(g, Fo, pos, vo);
+    double vp[3] = {vx[p], vy[p], vz[p]};
+    for (int a = 0; a < 3; ++a) {
+        double vpic = vn[a];
+        double vflip = vp[a] + vn[a] - vo[a];
+        vp[a] = vpic * (1.0 - flip) + vflip * flip;
+    }
+    vx[p] = vp[0]; vy[p] = vp[1]; vz[p] = vp[2];
+
+    double np_[3], q[3], k1[3], k2[3];
+    sample(g, Fn, pos, k1);
+    for (int a = 0; a < 3; ++a) q[a] = pos[a] + k1[a] * (0.5 * dt);
+    sample(g, Fn, q, k2);
+    if (integrator == 1) {
+        for (int a = 0; a < 3; ++a) np_[a] = pos[a] + k2[a] * dt;
+    } else {
+        double k3[3], k4[3];
+        for (int a = 0; a < 3; ++a) q[a] = pos[a] + k2[a] * (0.5 * dt);
+        sample(g, Fn, q, k3);
+        for (int a = 0; a < 3; ++a) q[a] = pos[a] + k3[a] * dt;
+        sample(g, Fn, q, k4);
+        for (int a = 0; a < 3; ++a) np_[a] = pos[a] + (k1[a] + k2[a] * 2.0 + k3[a] * 2.0 + k4[a]) * (dt / 6.0);
+    }
+    int gi = pos2idx(np_[0], g.dx), gj = pos2idx(np_[1], g.dx), gk = pos2idx(np_[2], g.dx);
+    if (cin(g, gi, gj, gk) && mat[cidx(g, gi, gj, gk)] != XF_SOLID) {
+        px[p] = np_[0]; py[p] = np_[1]; pz[p] = np_[2];
+        return;
+    }
+    int vox[3];
+    if (!voxel_hit(g, mat, pos, np_, vox)) return;
+    double dv[3] = {np_[0] - pos[0], np_[1] - pos[1], np_[2] - pos[2]};
+    double l = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    double rn[3] = {dv[0] / l, dv[1] / l, dv[2] / l};
+    double bmin[3] = {idx2pos(vox[0], g.dx), idx2pos(vox[1], g.dx), idx2pos(vox[2], g.dx)};
+    double bmax[3] = {bmin[0] + g.dx, bmin[1] + g.dx, bmin[2] + g.dx};
+    double cp[3];
+    if (!ray_box(pos, rn, bmin, bmax, cp)) return;
+    const double back = (double)(float)(0.05 * g.dx);
+    double res[3] = {cp[0] - rn[0] * back, cp[1] - rn[1] * back, cp[2] - rn[2] * back};
+    int ri = pos2idx(res[0], g.dx), rj = pos2idx(res[1], g.dx), rk = pos2idx(res[2], g.dx);
+    if (cin(g, ri, rj, rk) && mat[cidx(g, ri, rj, rk)] != XF_SOLID) {
+        px[p] = res[0]; py[p] = res[1]; pz[p] = res[2];
+    }
+}
+
+void launch_g2p_advect(const Geo& g, Particles& P, const uint8_t* mat, const double* u, const double* v,
+                       const double* w, const double* us, const double* vs, const double* ws, double dt, double flip,
+                       int integrator, int64_t limit_id, hipStream_t st) {
+    if (P.n == 0) return;
+    Faces Fn{u, v, w}, Fo{us, vs, ws};
+    k_g2p_advect<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(g, P.n, P.px, P.py, P.pz, P.vx, P.vy, P.vz, P.pid, mat, Fn, Fo,
+                                                       dt, flip, integrator, limit_id);
+}
+
+// ---------------------------------------------------------------------------
+// P2G, _updateVelocityByParticles (FluidSimulator.cpp:909-1085), as a gather:
+// face f = sum_p W(|q_p - c_f|^2) v_p / sum_p W, q_p = p + dx/2 e_axis,
+// c_f = idx2Pos(f) -- the same per-pair arithmetic as the reference; only the
+// summation order differs (cells in k,j,i order, particles by index).
+// Faces with sum W <= 1e-7 get 0.  Candidate cells span the kernel support.
+__global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, const double* __restrict__ py,
+                      const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
+                      double* __restrict__ out) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nf = (long)si * sj * sk;
+    if (t >= nf) return;
+    int fi = (int)(t % si), fj = (int)((t / si) % sj), fk = (int)(t / ((long)si * sj));
+    const double dx = g.dx;
+    const bool tri = kern == 1;
+    const double r = tri ? dx : 1.5 * dx;
+    const Wyvill W = make_wyvill(r);
+    // support in cells: Wyvill r=1.5dx -> [f-2, f+1] along the axis, [f-1, f+1] across;
+    // trilinear r=dx -> [f-1, f] along, [f-1, f+1] across
+    int lo[3], hi[3];
+    int fc[3] = {fi, fj, fk};
+    for (int a = 0; a < 3; ++a) {
+        if (a == axis) { lo[a] = fc[a] - (tri ? 1 : 2); hi[a] = fc[a] + (tri ? 0 : 1); }
+        else { lo[a] = fc[a] - 1; hi[a] = fc[a] + 1; }
+    }
+    if (lo[0] < 0) lo[0] = 0;
+    if (lo[1] < 0) lo[1] = 0;
+    if (lo[2] < 0) lo[2] = 0;
+    if (hi[0] > g.ni - 1) hi[0] = g.ni - 1;
+    if (hi[1] > g.nj - 1) hi[1] = g.nj - 1;
+    if (hi[2] > g.nk - 1) hi[2] = g.nk - 1;
+    const double cx = idx2pos(fi, dx), cy = idx2pos(fj, dx), cz = idx2pos(fk, dx);
+    const double sx = axis == 0 ? 0.5 * dx : 0.0, sy = axis == 1 ? 0.5 * dx : 0.0, sz = axis == 2 ? 0.5 * dx : 0.0;
+    double wsum = 0.0, vsum = 0.0;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j) {
+            long row = (long)g.ni * j + g.nn * k;
+            int a = start[row + lo[0]], b = start[row + hi[0] + 1];
+            for (int p = a; p < b; ++p) {
+                double ex = (px[p] + sx) - cx, ey = (py[p] + sy) - cy, ez = (pz[p] + sz) - cz;
+                double cw;
+                if (tri) {
+                    double ax = fabs(ex) / dx, ay = fabs(ey) / dx, az = fabs(ez) / dx;
+                    cw = (ax < 1.0 && ay < 1.0 && az < 1.0) ? (1.0 - ax) * (1.0 - ay) * (1.0 - az) : 0.0;
+                } else {
+                    double s = ex * ex + ey * ey + ez * ez;
+                    cw = wyvill(W, s);
+                }
+                if (cw != 0.0) {
+                    wsum += cw;
+                    vsum += vc[p] * cw;
+                }
+            }
+        }
+    out[t] = wsum > 0.0000001 ? vsum / wsum : 0.0;
+}
+
+void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f, int axis, int kernel,
+                hipStream_t st) {
+    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+    const double* vc = axis == 0 ? P.vx : axis == 1 ? P.vy : P.vz;
+    k_p2g<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, axis, kernel, P.px, P.py, P.pz, vc, ci.start, f);
+}
+
+// ---------------------------------------------------------------------------
+// Particle-blob density, DirectedDistanceField::updateFeild
+// (directeddistancefield.cpp:122-169, 300-328): phi(v) = sum_p W(|x_v - p|^2)
+// over the (d*n+1)^3 sub-vertex lattice, x_v = (v + 1/2) dx/d.  One thread
+// gathers the d^3 vertices owned by one coarse cell; the per-axis squared
+// offsets are computed once per particle and summed in the reference's
+// x,y,z order, so every pair weight equals the reference's.
+template <int D>
+__global__ void k_density(Geo g, const double* __restrict__ px, const double* __restrict__ py,
+                          const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi,
+                          double R) {
+    const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n1) return;
+    const int ci = (int)(t % (g.ni + 1)), cj = (int)((t / (g.ni + 1)) % (g.nj + 1)),
+              ck = (int)(t / ((long)(g.ni + 1) * (g.nj + 1)));
+    const int vi = D * g.ni + 1, vj = D * g.nj + 1, vk = D * g.nk + 1;
+    const int ni_v = (ci == g.ni) ? 1 : D, nj_v = (cj == g.nj) ? 1 : D, nk_v = (ck == g.nk) ? 1 : D;
+    const double dx = g.dx, subdx = dx / (double)D;
+    const Wyvill W = make_wyvill(R);
+    double xs[D], ys[D], zs[D];
+    for (int a = 0; a < D; ++a) {
+        xs[a] = idx2pos(D * ci + a, subdx);
+        ys[a] = idx2pos(D * cj + a, subdx);
+        zs[a] = idx2pos(D * ck + a, subdx);
+    }
+    // candidate cells: those overlapping [x_lo - R, x_hi + R] (tiny margin for rounding at cell edges)
+    auto range = [&](double lo_pos, double hi_pos, int n, int* lo, int* hi) {
+        *lo = (int)floor((lo_pos - R) / dx - 1e-7);
+        *hi = (int)floor((hi_pos + R) / dx + 1e-7);
+        if (*lo < 0) *lo = 0;
+        if (*hi > n - 1) *hi = n - 1;
+    };
+    int li, hi_, lj, hj, lk, hk;
+    range(xs[0], xs[ni_v - 1], g.ni, &li, &hi_);
+    range(ys[0], ys[nj_v - 1], g.nj, &lj, &hj);
+    range(zs[0], zs[nk_v - 1], g.nk, &lk, &hk);
+    double acc[D][D][D];
+    for (int a = 0; a < D; ++a) for (int b = 0; b < D; ++b) for (int c = 0; c < D; ++c) acc[a][b][c] = 0.0;
+    for (int k = lk; k <= hk; ++k)
+        for (int j = lj; j <= hj; ++j) {
+            long row = (long)g.ni * j + g.nn * k;
+            int a0 = start[row + li], b0 = start[row + hi_ + 1];
+            for (int p = a0; p < b0; ++p) {
+                const double x = px[p], y = py[p], z = pz[p];
+                double ex2[D], ey2[D], ez2[D];
+                for (int a = 0; a < D; ++a) {
+                    double e = xs[a] - x; ex2[a] = e * e;
+                    e = ys[a] - y; ey2[a] = e * e;
+                    e = zs[a] - z; ez2[a] = e * e;
+                }
+                for (int c = 0; c < D; ++c)
+                    for (int b = 0; b < D; ++b)
+                        for (int a = 0; a < D; ++a) {
+                            double s = ex2[a] + ey2[b] + ez2[c];
+                            if (s < W.r2) acc[c][b][a] += wyvill(W, s);
+                        }
+            }
+        }
+    for (int c = 0; c < nk_v; ++c)
+        for (int b = 0; b < nj_v; ++b)
+            for (int a = 0; a < ni_v; ++a) {
+                long v = (D * ci + a) + (long)vi * ((D * cj + b) + (long)vj * (D * ck + c));
+                phi[v] = acc[c][b][a];
+            }
+    (void)vk;
+}
+
+void launch_density(const Geo& g, const Particles& P, const CellIndex& ci, double* phi, int d, double radius,
+                    hipStream_t st) {
+    const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
+    if (d == 2) k_density<2><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
+    else if (d == 1) k_density<1><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
+    else if (d == 3) k_density<3><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
+    else k_density<4><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
+}
+
+// ---------------------------------------------------------------------------
+// Surface cells (directeddistancefield.cpp:171-244, 330-352) and cube case
+// (fluidmesher.cpp:397-417).  Coarse cells are ranked in the reference's loop
+// order (i slowest, k fastest); a coarse cell is a surface cell if its
+// (d+1)^3 vertices straddle iso (>= in / < out); its sub-cells, in (a,b,c)
+// order with c fastest, are kept if their 8 corners straddle iso.  Pass 1
+// counts per coarse cell, a prefix sum ranks them, pass 2 writes.
+__device__ inline long coarse_of_rank(const Geo& g, long t, int* i, int* j, int* k) {
+    *k = (int)(t % g.nk);
+    *j = (int)((t / g.nk) % g.nj);
+    *i = (int)(t / ((long)g.nk * g.nj));
+    return t;
+}
+
+template <bool WRITE>
+__global__ void k_surface(Geo g, int d, const double* __restrict__ phi, double iso, int32_t* counts,
+                          const int32_t* offs, int32_t* cells, uint8_t* cube, long cap) {
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= g.N) return;
+    int i, j, k;
+    coarse_of_rank(g, t, &i, &j, &k);
+    const long vi = d * g.ni + 1, vj = d * g.nj + 1;
+    auto PHI = [&](int a, int b, int c) { return phi[a + vi * (b + vj * c)]; };
+    const int s0 = i * d, s1 = j * d, s2 = k * d;
+    bool in = false, out = false;
+    for (int a = s0; a <= s0 + d; ++a)
+        for (int b = s1; b <= s1 + d; ++b)
+            for (int c = s2; c <= s2 + d; ++c) {
+                if (PHI(a, b, c) >= iso) in = true; else out = true;
+            }
+    int cnt = 0;
+    long pos = WRITE ? offs[t] : 0;
+    if (in && out) {
+        const long sni = (long)d * g.ni, snj = (long)d * g.nj;
+        for (int a = s0; a < s0 + d; ++a)
+            for (int b = s1; b < s1 + d; ++b)
+                for (int c = s2; c < s2 + d; ++c) {
+                    // vs[] corner order of fluidmesher.cpp:397-406
+                    double v0 = PHI(a, b, c), v1 = PHI(a + 1, b, c), v2 = PHI(a + 1, b, c + 1), v3 = PHI(a, b, c + 1);
+                    double v4 = PHI(a, b + 1, c), v5 = PHI(a + 1, b + 1, c), v6 = PHI(a + 1, b + 1, c + 1),
+                           v7 = PHI(a, b + 1, c + 1);
+                    bool ii = false, oo = false;
+                    double vv[8] = {v0, v1, v2, v3, v4, v5, v6, v7};
+                    for (int q = 0; q < 8; ++q) { if (vv[q] >= iso) ii = true; else oo = true; }
+                    if (!(ii && oo)) continue;
+                    if (WRITE && pos + cnt < cap) {
+                        int ci = 0;
+                        for (int q = 0; q < 8; ++q) if (vv[q] > iso) ci |= 1 << q;
+                        cells[pos + cnt] = (int32_t)(a + sni * (b + snj * c));
+                        cube[pos + cnt] = (uint8_t)ci;
+                    }
+                    ++cnt;
+                }
+    }
+    if (!WRITE) counts[t] = cnt;
+}
+
+int64_t surface_cells(const Geo& g, const double* phi, int d, double iso, int32_t* cells, uint8_t* cube,
+                      int64_t cap, int32_t* count_tmp, int32_t* start_tmp, int32_t* scan_tmp, int scan_tmp_n,
+                      hipStream_t st) {
+    k_surface<false><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, d, phi, iso, count_tmp, nullptr, nullptr, nullptr, 0);
+    exclusive_scan_i32(count_tmp, start_tmp, g.N, scan_tmp, scan_tmp_n, st);
+    int32_t total = 0;
+    hipMemcpyAsync(&total, start_tmp + g.N, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    if (cells && cap > 0)
+        k_surface<true><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, d, phi, iso, nullptr, start_tmp, cells, cube, cap);
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// max |v|^2 over particles (FluidSimulator.cpp:387-400): v*v = (vx*vx + vy*vy) + vz*vz
+__global__ void k_maxv2(long n, const double* vx, const double* vy, const double* vz, double* partial) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+        double c = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+        m = c > m ? c : m;
+    }
+    m = block_reduce<double, true>(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+void launch_maxv2(const Particles& P, double* partial, int nblocks, hipStream_t st) {
+    k_maxv2<<<nblocks, XF_BLOCK, 0, st>>>(P.n, P.vx, P.vy, P.vz, partial);
+}

@@ -1,0 +1,442 @@
+// xf_pcg.hip -- MIC(0)-preconditioned CG on the variational pressure matrix,
+// matrix-free over the 7-point stencil codes (pcg_solver.h:85-310,
+// sparse_matrix.h:266-276, blas_wrapper.h:12-51).
+//
+// The preconditioner reproduces the reference's lexicographic MIC(0): the
+// factorisation and both triangular sweeps are hyperplane recurrences (every
+// input of cell (i,j,k) lies on plane i+j+k-1, or i+j+k+1 for the backward
+// sweep), so any schedule that respects the planes reproduces the serial
+// arithmetic bit for bit.  Two schedules exist:
+//   * one launch per hyperplane (simple, launch-bound);
+//   * a pipelined persistent kernel over (j,k) line tiles (xf_sweep.hip).
+// Dot products accumulate in FP64 with a fixed blocked order (deterministic,
+// but not the reference's serial order: PCG iterates match to rounding).
+#include <vector>
+#include <cmath>
+#include <cstdio>
+#include "xf_internal.h"
+
+#define RED_BLOCKS 1024
+
+__device__ inline bool present(int cls) { return cls == 1 || cls == 2; }
+
+// ---------------------------------------------------------------------------
+// MIC(0) factorisation, pcg_solver.h:125-185 restated per cell (see the oracle
+// for the reduction of the pattern search): pivot d_c starts at A_cc and takes
+// one update from each present lower neighbour column k in ascending column
+// order (c-nn, c-ni, c-1):  d -= L*L;  d -= (tau*L)*missing, where missing is
+// the sum of the other present entries of column k in ascending row order.
+__global__ void k_mic_plane(Geo g, int t, int kmin, const uint16_t* __restrict__ code, double* __restrict__ inv,
+                            double tau, double sigma) {
+    long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int j = (int)(tid % g.nj), k = kmin + (int)(tid / g.nj);
+    int i = t - j - k;
+    if (k >= g.nk || i < 0 || i >= g.ni) return;
+    long c = cidx(g, i, j, k);
+    uint16_t cc = code[c];
+    double diag = code_diag(cc);
+    if (diag == 0) { inv[c] = 0.0; return; }
+    double d = diag;
+    int cl = code_dir(cc, D_ZM);
+    if (present(cl)) {  // column c-nn: other rows k+1, k+ni (both above c)
+        long kc = c - g.nn;
+        uint16_t ck = code[kc];
+        double ik = inv[kc];
+        double L = code_coef(cl) * ik;
+        double miss = 0;
+        int a = code_dir(ck, D_XP), b = code_dir(ck, D_YP);
+        if (present(a)) miss += code_coef(a) * ik;
+        if (present(b)) miss += code_coef(b) * ik;
+        d -= L * L;
+        d -= tau * L * miss;
+    }
+    cl = code_dir(cc, D_YM);
+    if (present(cl)) {  // column c-ni: rows k+1 (above c), k+nn (below c)
+        long kc = c - g.ni;
+        uint16_t ck = code[kc];
+        double ik = inv[kc];
+        double L = code_coef(cl) * ik;
+        double miss = 0;
+        int a = code_dir(ck, D_XP), b = code_dir(ck, D_ZP);
+        if (present(a)) miss += code_coef(a) * ik;
+        if (present(b)) miss += code_coef(b) * ik;
+        d -= L * L;
+        d -= tau * L * miss;
+    }
+    cl = code_dir(cc, D_XM);
+    if (present(cl)) {  // column c-1: rows k+ni, k+nn (both below c)
+        long kc = c - 1;
+        uint16_t ck = code[kc];
+        double ik = inv[kc];
+        double L = code_coef(cl) * ik;
+        double miss = 0;
+        int a = code_dir(ck, D_YP), b = code_dir(ck, D_ZP);
+        if (present(a)) miss += code_coef(a) * ik;
+        if (present(b)) miss += code_coef(b) * ik;
+        d -= L * L;
+        d -= tau * L * miss;
+    }
+    inv[c] = (d < sigma * diag) ? 1 / sqrt(diag) : 1 / sqrt(d);
+}
+
+static void plane_range(const Geo& g, int t, int* kmin, long* nthreads) {
+    int k0 = t - (g.ni - 1) - (g.nj - 1);
+    if (k0 < 0) k0 = 0;
+    int k1 = t < g.nk - 1 ? t : g.nk - 1;
+    *kmin = k0;
+    *nthreads = (long)g.nj * (k1 - k0 + 1);
+}
+
+void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, hipStream_t st) {
+    const int planes = g.ni + g.nj + g.nk - 2;
+    for (int t = 0; t < planes; ++t) {
+        int kmin;
+        long nt;
+        plane_range(g, t, &kmin, &nt);
+        k_mic_plane<<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, tau, sigma);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solve_lower (pcg_solver.h:192-204) from cell c's side:
+//   q_c = (((r_c - L_{c,c-nn} q_{c-nn}) - L_{c,c-ni} q_{c-ni}) - L_{c,c-1} q_{c-1}) * inv_c
+// with L_{c,k} = A_{c,k} * inv_k.
+template <class T>
+__global__ void k_fwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__ code, const T* __restrict__ inv,
+                            const T* __restrict__ r, T* __restrict__ q, const PcgScalars* S) {
+    if (S && S->done) return;
+    long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int j = (int)(tid % g.nj), k = kmin + (int)(tid / g.nj);
+    int i = t - j - k;
+    if (k >= g.nk || i < 0 || i >= g.ni) return;
+    long c = cidx(g, i, j, k);
+    uint16_t cc = code[c];
+    T v = r[c];
+    int cl = code_dir(cc, D_ZM);
+    if (present(cl)) v = v - ((T)code_coef(cl) * inv[c - g.nn]) * q[c - g.nn];
+    cl = code_dir(cc, D_YM);
+    if (present(cl)) v = v - ((T)code_coef(cl) * inv[c - g.ni]) * q[c - g.ni];
+    cl = code_dir(cc, D_XM);
+    if (present(cl)) v = v - ((T)code_coef(cl) * inv[c - 1]) * q[c - 1];
+    q[c] = v * inv[c];
+}
+
+// solve_lower_transpose_in_place (pcg_solver.h:207-220) from cell c's side:
+//   z_c = (((q_c - L_{c+1,c} z_{c+1}) - L_{c+ni,c} z_{c+ni}) - L_{c+nn,c} z_{c+nn}) * inv_c
+template <class T>
+__global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__ code, const T* __restrict__ inv,
+                            const T* __restrict__ q, T* __restrict__ z, const PcgScalars* S) {
+    if (S && S->done) return;
+    long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int j = (int)(tid % g.nj), k = kmin + (int)(tid / g.nj);
+    int i = t - j - k;
+    if (k >= g.nk || i < 0 || i >= g.ni) return;
+    long c = cidx(g, i, j, k);
+    uint16_t cc = code[c];
+    T ic = inv[c];
+    T v = q[c];
+    int cl = code_dir(cc, D_XP);
+    if (present(cl)) v = v - ((T)code_coef(cl) * ic) * z[c + 1];
+    cl = code_dir(cc, D_YP);
+    if (present(cl)) v = v - ((T)code_coef(cl) * ic) * z[c + g.ni];
+    cl = code_dir(cc, D_ZP);
+    if (present(cl)) v = v - ((T)code_coef(cl) * ic) * z[c + g.nn];
+    z[c] = v * ic;
+}
+
+// Pipelined sweep schedule (xf_sweep.hip).
+template <class T>
+bool sweep_pipelined(const Geo& g, bool forward, const uint16_t* code, const T* inv, const T* in, T* out,
+                     const PcgScalars* S, int* flags, hipStream_t st);
+
+template <class T>
+static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
+                      int sweep_kernel, int* flags, hipStream_t st) {
+    if (sweep_kernel == 2 && flags) {
+        if (sweep_pipelined<T>(g, true, code, inv, r, q, S, flags, st) &&
+            sweep_pipelined<T>(g, false, code, inv, q, z, S, flags, st))
+            return;
+    }
+    const int planes = g.ni + g.nj + g.nk - 2;
+    for (int t = 0; t < planes; ++t) {
+        int kmin;
+        long nt;
+        plane_range(g, t, &kmin, &nt);
+        k_fwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, r, q, S);
+    }
+    for (int t = planes - 1; t >= 0; --t) {
+        int kmin;
+        long nt;
+        plane_range(g, t, &kmin, &nt);
+        k_bwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, q, z, S);
+    }
+}
+
+void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
+                 int sweep_kernel, int* flags, hipStream_t st) {
+    precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, flags, st);
+}
+
+// ---------------------------------------------------------------------------
+// y = A x (sparse_matrix.h:266-276): present entries in column order, each
+// added as a*x to a sum starting at 0.  Optional FP64 partial dot(x, y).
+template <class T>
+__device__ inline T spmv_cell(const Geo& g, uint16_t cc, long c, const T* __restrict__ x) {
+    T y = 0;
+    int cl = code_dir(cc, D_ZM);
+    if (present(cl)) y += (T)code_coef(cl) * x[c - g.nn];
+    cl = code_dir(cc, D_YM);
+    if (present(cl)) y += (T)code_coef(cl) * x[c - g.ni];
+    cl = code_dir(cc, D_XM);
+    if (present(cl)) y += (T)code_coef(cl) * x[c - 1];
+    double dg = code_diag(cc);
+    if (dg != 0.0) y += (T)dg * x[c];
+    cl = code_dir(cc, D_XP);
+    if (present(cl)) y += (T)code_coef(cl) * x[c + 1];
+    cl = code_dir(cc, D_YP);
+    if (present(cl)) y += (T)code_coef(cl) * x[c + g.ni];
+    cl = code_dir(cc, D_ZP);
+    if (present(cl)) y += (T)code_coef(cl) * x[c + g.nn];
+    return y;
+}
+
+template <class T>
+__global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __restrict__ x, T* __restrict__ y,
+                           double* partial, const PcgScalars* S) {
+    __shared__ double sh[16];
+    if (S && S->done) return;
+    double acc = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
+        T v = spmv_cell<T>(g, code[c], c, x);
+        y[c] = v;
+        acc += (double)x[c] * (double)v;
+    }
+    acc = block_reduce<double, false>(acc, sh);
+    if (partial && threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st) {
+    k_spmv_dot<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, x, y, nullptr, nullptr);
+}
+
+// r = b - A x (FP64), partial max |r| -- the refinement residual.
+__global__ void k_residual(Geo g, const uint16_t* __restrict__ code, const double* __restrict__ b,
+                           const double* __restrict__ x, double* __restrict__ r, double* partial) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
+        double v = b[c] - spmv_cell<double>(g, code[c], c, x);
+        r[c] = v;
+        double a = fabs(v);
+        m = a > m ? a : m;
+    }
+    m = block_reduce<double, true>(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+// x += alpha s; r += (-alpha) z; partial max |r|   (blas_wrapper.h:41-51)
+template <class T>
+__global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ s,
+                           const T* __restrict__ z, double* partial, const PcgScalars* S) {
+    __shared__ double sh[16];
+    if (S->done) return;
+    const double alpha = S->alpha;
+    const T a = (T)alpha, na = (T)(-alpha);
+    double m = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        x[c] += a * s[c];
+        T rv = r[c] + na * z[c];
+        r[c] = rv;
+        double av = fabs((double)rv);
+        m = av > m ? av : m;
+    }
+    m = block_reduce<double, true>(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+template <class T>
+__global__ void k_dot(long n, const T* __restrict__ a, const T* __restrict__ b, double* partial, const PcgScalars* S) {
+    __shared__ double sh[16];
+    if (S && S->done) return;
+    double acc = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+        acc += (double)a[c] * (double)b[c];
+    acc = block_reduce<double, false>(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+template <class T>
+__global__ void k_absmax(long n, const T* __restrict__ a, double* partial) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        double v = fabs((double)a[c]);
+        m = v > m ? v : m;
+    }
+    m = block_reduce<double, true>(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+// s = z + beta s  (the reference computes z_i + beta*s_i, then swaps)
+template <class T>
+__global__ void k_update_s(long n, T* __restrict__ s, const T* __restrict__ z, const PcgScalars* S) {
+    if (S->done) return;
+    const T beta = (T)S->beta;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+        s[c] = z[c] + beta * s[c];
+}
+
+template <class D, class Sx>
+__global__ void k_copy(long n, D* __restrict__ d, const Sx* __restrict__ s, const PcgScalars* S) {
+    if (S && S->done) return;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+        d[c] = (D)s[c];
+}
+template <class T>
+__global__ void k_fill(long n, T* __restrict__ d, T v) {
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) d[c] = v;
+}
+__global__ void k_add_f2d(long n, double* __restrict__ x, const float* __restrict__ c) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] += (double)c[i];
+}
+
+// ---- final reductions (one block, fixed order) and scalar updates ----
+template <bool MAX>
+__device__ inline double reduce_partials(const double* p, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = MAX ? (p[i] > v ? p[i] : v) : v + p[i];
+    return block_reduce<double, MAX>(v, sh);
+}
+
+enum { FIN_RES0, FIN_RHO0, FIN_ALPHA, FIN_RES, FIN_BETA };
+__global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, double tol_factor, int it) {
+    __shared__ double sh[16];
+    if (what != FIN_RES0 && S->done) return;
+    const bool mx = (what == FIN_RES0 || what == FIN_RES);
+    double v = mx ? reduce_partials<true>(partial, n, sh) : reduce_partials<false>(partial, n, sh);
+    if (threadIdx.x != 0) return;
+    switch (what) {
+        case FIN_RES0:
+            S->done = 0; S->iters = 0; S->status = 0; S->it = 0;
+            S->res0 = v; S->res = v;
+            S->tol = tol_factor * v;
+            if (v == 0) S->done = 1;
+            break;
+        case FIN_RHO0:
+            S->rho = v;
+            if (v == 0 || v != v) { S->done = 1; S->status = 2; S->iters = 0; }
+            break;
+        case FIN_ALPHA:
+            S->sz = v;
+            S->alpha = S->rho / v;
+            break;
+        case FIN_RES:
+            S->res = v;
+            S->it = it + 1;
+            if (v <= S->tol) { S->done = 1; S->iters = it + 1; }
+            break;
+        case FIN_BETA:
+            S->rho_new = v;
+            S->beta = v / S->rho;
+            S->rho = v;
+            break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One PCG solve (pcg_solver.h:243-285) of A x = rhs with T-precision vectors.
+// x must be zeroed by the caller; rhs is consumed as the initial residual r.
+template <class T>
+static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* z, T* q, double* partial,
+                     PcgScalars* S, PcgScalars* S_host, double tol_factor, int max_iter, int sweep_kernel, int* flags,
+                     hipStream_t st) {
+    const long N = g.N;
+    k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
+    k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
+    precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, st);
+    k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
+    k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, RED_BLOCKS, S, tol_factor, 0);
+    k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
+    const int CH = 4;
+    int it = 0;
+    while (it < max_iter) {
+        int end = it + CH < max_iter ? it + CH : max_iter;
+        for (; it < end; ++it) {
+            k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, s, z, partial, S);
+            k_finish<<<1, 1024, 0, st>>>(FIN_ALPHA, partial, RED_BLOCKS, S, tol_factor, it);
+            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, S);
+            k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it);
+            precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, st);
+            k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
+            k_finish<<<1, 1024, 0, st>>>(FIN_BETA, partial, RED_BLOCKS, S, tol_factor, it);
+            k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
+        }
+        hipMemcpyAsync(S_host, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        if (S_host->done) break;
+    }
+    if (!S_host->done) {
+        hipMemcpyAsync(S_host, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        if (!S_host->done) { S_host->status = 1; S_host->iters = max_iter; }
+    }
+}
+
+PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,
+                    double sigma, int precision, double inner_tol, int max_outer, int sweep_kernel, hipStream_t st) {
+    const long N = g.N;
+    if (tol_factor < 1e-30) tol_factor = 1e-30;
+    PcgResult res{};
+    pcg_mic0(g, code, w.inv, tau, sigma, st);
+    if (precision == 0) {
+        k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
+        k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
+        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.z, w.q, w.partial, w.S, w.S_host, tol_factor, max_iter,
+                         sweep_kernel, w.sweep_flags, st);
+        res.status = w.S_host->status;
+        res.iterations = w.S_host->iters;
+        res.outer = 1;
+        res.residual = w.S_host->res;
+        res.rhs_norm = w.S_host->res0;
+        return res;
+    }
+    // Mixed precision: FP32 inner PCG (SpMV, sweeps, axpy in FP32; dots in
+    // FP64; pivots factored in FP64 then rounded) inside FP64 iterative
+    // refinement on the true residual r = b - A x.
+    PcgScalars* S_outer = w.S + 1;
+    PcgScalars* S_outer_h = w.S_host + 1;
+    k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.inv32, w.inv, nullptr);
+    k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
+    k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
+    k_absmax<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.partial);
+    k_finish<<<1, 1024, 0, st>>>(FIN_RES0, w.partial, RED_BLOCKS, S_outer, tol_factor, 0);
+    hipMemcpyAsync(S_outer_h, S_outer, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    res.rhs_norm = S_outer_h->res0;
+    res.residual = S_outer_h->res0;
+    const double tol = S_outer_h->tol;
+    int total = 0, outer = 0;
+    res.status = 0;
+    if (S_outer_h->res0 == 0) { res.iterations = 0; res.outer = 0; return res; }
+    res.status = 1;
+    while (outer < max_outer && total < max_iter * max_outer) {
+        k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
+        k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
+        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.z32, w.q32, w.partial, w.S, w.S_host, inner_tol,
+                        max_iter, sweep_kernel, w.sweep_flags, st);
+        total += w.S_host->iters;
+        ++outer;
+        k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
+        k_residual<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.b, w.x, w.r, w.partial);
+        k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0);
+        hipMemcpyAsync(S_outer_h, S_outer, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        res.residual = S_outer_h->res;
+        if (S_outer_h->res <= tol) { res.status = 0; break; }
+        if (w.S_host->status == 2) { res.status = 2; break; }
+    }
+    res.iterations = total;
+    res.outer = outer;
+    return res;
+}
