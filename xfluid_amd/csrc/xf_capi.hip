@@ -184,9 +184,23 @@ int st_solve(xf_sim* s) {
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s->st));
     launch_stencil_rhs(s->g, s->mat, s->u, s->v, s->w, s->code, s->pw.b, s->st);
+    s->last.reserved = 0;
     const xf_params& p = s->prm;
+    int sweep = p.sweep_kernel == 0 ? 1 : p.sweep_kernel;
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
-                            p.mixed_inner_tol, p.mixed_max_outer, p.sweep_kernel, s->st);
+                            p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
+    if (sweep == 2) {
+        int errw = 0;
+        HIPCHK(hipMemcpyAsync(&errw, s->pw.sweep_flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        if (errw) {  // a pipelined wait timed out: redo the solve with per-hyperplane launches
+            fprintf(stderr, "xfluid: pipelined sweep wait timed out; solving with per-plane sweeps\n");
+            HIPCHK(hipMemsetAsync(s->pw.sweep_flags, 0, sizeof(int), s->st));
+            r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
+                          p.mixed_inner_tol, p.mixed_max_outer, 1, s->st);
+            s->last.reserved = 1;
+        }
+    }
     HIPCHK(hipEventRecord(e1, s->st));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -306,10 +320,12 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
               dalloc(&s->pw.S, 2) == hipSuccess && dalloc(&s->red, RED_N) == hipSuccess &&
               dalloc(&s->surf_cnt, N + 1) == hipSuccess && dalloc(&s->surf_start, N + 2) == hipSuccess;
     if (!ok) return fail(set_err(-2, "device allocation failed"));
+    s->pw.nflags = sweep_flag_words(s->g);
+    if (dalloc(&s->pw.sweep_flags, s->pw.nflags) != hipSuccess) return fail(set_err(-2, "alloc flags"));
+    hipMemsetAsync(s->pw.sweep_flags, 0, s->pw.nflags * sizeof(int), s->st);
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
     if (dalloc(&s->ci.scan_tmp, s->ci.scan_tmp_n) != hipSuccess) return fail(set_err(-2, "alloc scan"));
     s->pw.npartial = RED_N;
-    s->pw.nflags = 0;
     if (hipHostMalloc((void**)&s->pw.S_host, 2 * sizeof(PcgScalars)) != hipSuccess ||
         hipHostMalloc((void**)&s->h_red, RED_N * sizeof(double)) != hipSuccess)
         return fail(set_err(-2, "pinned alloc"));

@@ -49,6 +49,8 @@ void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, doubl
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
                  int sweep_kernel, int* flags, hipStream_t st);
 void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st);
+// ints needed for the pipelined sweep's error word + progress counters
+int sweep_flag_words(const Geo& g);
 
 // ---------------- particle kernels (xf_particles.hip) ----------------
 struct Particles {
