@@ -22,6 +22,8 @@ CASES = {
     "c1_small": (24, 16, 12, 0.125, (0.35, 0.6, 1.0), 0),
     "c1_quarter": (32, 16, 16, 0.125, (0.35, 0.6, 1.0), 0),
     "odd": (19, 13, 11, 0.1, (0.5, 0.55, 1.0), 7),
+    # several 16x16 (j,k) tiles: exercises the tile-to-tile hand-off of the pipelined sweeps
+    "tiles": (21, 40, 35, 0.05, (0.6, 0.7, 1.0), 9),
 }
 
 
@@ -204,12 +206,12 @@ def test_frame_step_runs(xf):
     _close(Pg, Po, 1e-8, "positions after one frame")
 
 
-@pytest.mark.parametrize("case", ["c1_quarter", "odd"])
-def test_pipelined_sweeps_bitexact(xf, case):
+@pytest.mark.parametrize("case,kind", [("odd", 2), ("c1_quarter", 3)] + [("tiles", k) for k in (2, 3, 4, 5, 6)])
+def test_pipelined_sweeps_bitexact(xf, case, kind):
     """The pipelined line-tile sweeps give the same bits as the serial sweeps,
     and a whole solve with them equals the per-hyperplane solve exactly."""
     ni, nj, nk, dx, frac, seed = CASES[case]
-    say(f"[pipelined] case={case}")
+    say(f"[pipelined] case={case} schedule={kind}")
     O = oracle()
     o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
     o.time_step(1.0 / 60)
@@ -233,7 +235,7 @@ def test_pipelined_sweeps_bitexact(xf, case):
     q, z = np.empty(N), np.empty(N)
     O.xo_sweep_fwd(ni, nj, nk, A7, inv, x, q)
     O.xo_sweep_bwd(ni, nj, nk, A7, inv, q, z)
-    p.sweep_kernel = 2
+    p.sweep_kernel = kind
     g.set_params(p)
     _exact(g.apply_precond(x), z, "precond (pipelined sweeps)")
     copy_state(o, g)

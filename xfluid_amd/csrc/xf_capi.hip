@@ -189,7 +189,7 @@ int st_solve(xf_sim* s) {
     int sweep = p.sweep_kernel == 0 ? 1 : p.sweep_kernel;
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
-    if (sweep == 2) {
+    if (sweep >= 2) {
         int errw = 0;
         HIPCHK(hipMemcpyAsync(&errw, s->pw.sweep_flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
@@ -321,6 +321,7 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
               dalloc(&s->surf_cnt, N + 1) == hipSuccess && dalloc(&s->surf_start, N + 2) == hipSuccess;
     if (!ok) return fail(set_err(-2, "device allocation failed"));
     s->pw.nflags = sweep_flag_words(s->g);
+    s->pw.epoch = 1;
     if (dalloc(&s->pw.sweep_flags, s->pw.nflags) != hipSuccess) return fail(set_err(-2, "alloc flags"));
     hipMemsetAsync(s->pw.sweep_flags, 0, s->pw.nflags * sizeof(int), s->st);
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
@@ -708,7 +709,7 @@ int xf_sim_apply_precond(xf_sim* s, const double* r, double* z) {
     if (check(s) || !r || !z) return set_err(-1, "null argument");
     const long N = s->g.N;
     HIPCHK(hipMemcpyAsync(s->pw.s, r, N * 8, hipMemcpyHostToDevice, s->st));
-    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, s->prm.sweep_kernel, s->pw.sweep_flags, s->st);
+    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, s->prm.sweep_kernel, s->pw.sweep_flags, &s->pw.epoch, s->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(z, s->pw.z, N * 8, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));

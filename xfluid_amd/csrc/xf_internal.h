@@ -34,8 +34,9 @@ struct PcgWork {
     int npartial;
     PcgScalars* S;       // device scalars
     PcgScalars* S_host;  // pinned mirror
-    int* sweep_flags;    // progress counters of the pipelined sweep
+    int* sweep_flags;    // error word, progress counters, tickets of the pipelined sweep
     int nflags;
+    unsigned long long epoch;  // sweep launch counter (tags progress counters)
 };
 struct PcgResult {
     int status, iterations, outer;
@@ -47,7 +48,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
 // Individual pieces, for parity checks.
 void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, hipStream_t st);
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
-                 int sweep_kernel, int* flags, hipStream_t st);
+                 int sweep_kernel, int* flags, unsigned long long* epoch, hipStream_t st);
 void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st);
 // ints needed for the pipelined sweep's error word + progress counters
 int sweep_flag_words(const Geo& g);

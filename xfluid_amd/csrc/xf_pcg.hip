@@ -146,15 +146,15 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
 
 // Pipelined sweep schedule (xf_sweep.hip).
 template <class T>
-bool sweep_pipelined(const Geo& g, bool forward, const uint16_t* code, const T* inv, const T* in, T* out,
-                     const PcgScalars* S, int* flags, hipStream_t st);
+bool sweep_pipelined(const Geo& g, int kind, bool forward, const uint16_t* code, const T* inv, const T* in, T* out,
+                     const PcgScalars* S, int* flags, unsigned long long* epoch, hipStream_t st);
 
 template <class T>
 static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
-                      int sweep_kernel, int* flags, hipStream_t st) {
-    if (sweep_kernel == 2 && flags) {
-        if (sweep_pipelined<T>(g, true, code, inv, r, q, S, flags, st) &&
-            sweep_pipelined<T>(g, false, code, inv, q, z, S, flags, st))
+                      int sweep_kernel, int* flags, unsigned long long* epoch, hipStream_t st) {
+    if (sweep_kernel >= 2 && flags) {
+        if (sweep_pipelined<T>(g, sweep_kernel, true, code, inv, r, q, S, flags, epoch, st) &&
+            sweep_pipelined<T>(g, sweep_kernel, false, code, inv, q, z, S, flags, epoch, st))
             return;
     }
     const int planes = g.ni + g.nj + g.nk - 2;
@@ -173,8 +173,8 @@ static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
 }
 
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
-                 int sweep_kernel, int* flags, hipStream_t st) {
-    precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, flags, st);
+                 int sweep_kernel, int* flags, unsigned long long* epoch, hipStream_t st) {
+    precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, flags, epoch, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -350,11 +350,11 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
 template <class T>
 static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* z, T* q, double* partial,
                      PcgScalars* S, PcgScalars* S_host, double tol_factor, int max_iter, int sweep_kernel, int* flags,
-                     hipStream_t st) {
+                     unsigned long long* epoch, hipStream_t st) {
     const long N = g.N;
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
-    precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, st);
+    precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, epoch, st);
     k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
     k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, RED_BLOCKS, S, tol_factor, 0);
     k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
@@ -367,7 +367,7 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             k_finish<<<1, 1024, 0, st>>>(FIN_ALPHA, partial, RED_BLOCKS, S, tol_factor, it);
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it);
-            precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, st);
+            precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, epoch, st);
             k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_BETA, partial, RED_BLOCKS, S, tol_factor, it);
             k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
@@ -393,7 +393,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
         pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.z, w.q, w.partial, w.S, w.S_host, tol_factor, max_iter,
-                         sweep_kernel, w.sweep_flags, st);
+                         sweep_kernel, w.sweep_flags, &w.epoch, st);
         res.status = w.S_host->status;
         res.iterations = w.S_host->iters;
         res.outer = 1;
@@ -424,7 +424,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
         pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.z32, w.q32, w.partial, w.S, w.S_host, inner_tol,
-                        max_iter, sweep_kernel, w.sweep_flags, st);
+                        max_iter, sweep_kernel, w.sweep_flags, &w.epoch, st);
         total += w.S_host->iters;
         ++outer;
         k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
