@@ -54,11 +54,14 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0):
     s.set_timing(True)
     stage_ms = [0.0] * 9
     solve_ms, iters, outer, worst = 0.0, 0, 0, 0.0
+    sweep_ms, sweeps = 0.0, 0
     t0 = time.perf_counter()
     for _ in range(steps):
         rc = s.timeStep(dt)
         st = s.last_solve()
         solve_ms += st.ms
+        sweep_ms += st.sweep_ms
+        sweeps += st.sweeps
         iters += st.iterations
         outer += st.outer
         worst = max(worst, st.residual / max(st.rhs_norm, 1e-300))
@@ -72,7 +75,7 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0):
     n = s.num_particles()
     s.close()
     return dict(elapsed=elapsed, solve_ms=solve_ms, iters=iters, outer=outer, worst_rel_res=worst,
-                stage_ms=stage_ms, particles=n)
+                stage_ms=stage_ms, particles=n, sweep_ms=sweep_ms, sweeps=sweeps)
 
 
 def cpu_baseline(cfg, dt):
@@ -123,8 +126,15 @@ def main():
         log(f"[bench] fp64: {fp64['elapsed']:.3f} s, solve {fp64['solve_ms']:.1f} ms, iters {fp64['iters']}")
     K = args.steps
     value = K / mixed["elapsed"]
-    # roofline of the pressure solve (algorithmic bytes per PCG iteration: FP64
-    # 131 B/cell, FP32 bulk 67 B/cell over the full N-cell vectors)
+    # Roofline of the dominant kernel, the pipelined MIC(0) sweep (k_sweep_pipe):
+    # algorithmic bytes per launch over the N-cell grid = read input + pivots +
+    # 2-byte stencil code, write output: FP32 (mixed run) 4+4+2+4 = 14 B/cell,
+    # FP64 8+8+2+8 = 26 B/cell.  Average launch duration from device events
+    # around the sweep launches of the timed substeps.
+    sweep_us = 1e3 * mixed["sweep_ms"] / max(mixed["sweeps"], 1)
+    ach_sweep = 14.0 * N / (sweep_us * 1e-6) / 1e9 if mixed["sweeps"] else 0.0
+    # whole-solve rate (algorithmic bytes per PCG iteration: FP64 131 B/cell,
+    # FP32 bulk 67 B/cell over the full N-cell vectors)
     it_m = max(mixed["iters"], 1)
     ach_m = 67.0 * N * it_m / (mixed["solve_ms"] * 1e-3) / 1e9
     out = {
@@ -148,9 +158,12 @@ def main():
         "worst_rel_residual_mixed": mixed["worst_rel_res"],
         "particles_per_s": round(mixed["particles"] * value, 1),
         "stage_ms_per_step_mixed": [round(x / K, 3) for x in mixed["stage_ms"]],
-        "roofline": {"bound": "hbm", "kernel": "pcg_solve (mixed)", "achieved": round(ach_m, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_m / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "k_sweep_pipe (MIC(0) sweep, FP32 in the mixed solve)",
+                     "achieved": round(ach_sweep, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_sweep / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_launch": 14 * N, "avg_launch_us": round(sweep_us, 2),
+                     "launches": mixed["sweeps"]},
+        "pcg_mixed_achieved_GBs": round(ach_m, 2),
     }
     if fp64:
         it_f = max(fp64["iters"], 1)
@@ -159,6 +172,8 @@ def main():
         out["pcg_iters_per_solve_fp64"] = round(fp64["iters"] / K, 2)
         out["mixed_vs_fp64_solve_rate"] = round(fp64["solve_ms"] / max(mixed["solve_ms"], 1e-9), 3)
         out["pcg_fp64_achieved_GBs"] = round(131.0 * N * it_f / (fp64["solve_ms"] * 1e-3) / 1e9, 2)
+        if fp64["sweeps"]:
+            out["sweep_fp64_avg_launch_us"] = round(1e3 * fp64["sweep_ms"] / fp64["sweeps"], 2)
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 substep)")
         out["cpu_baseline"] = cpu_baseline(args.config, dt)

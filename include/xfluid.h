@@ -54,7 +54,8 @@ typedef struct xf_params {
     int    p2g_kernel;        /* XF_P2G_WYVILL (reference) or XF_P2G_TRILINEAR      */
     int    tail_skip;         /* 1: the last n%6 particles are not advected (reference chunking) */
     int    density_every_substep; /* 1: density each substep (reference); 0: only when a mesh is output */
-    int    sweep_kernel;      /* MIC(0) sweep schedule: 0 auto, 1 per-hyperplane launches, 2 pipelined tiles */
+    int    sweep_kernel;      /* MIC(0) sweep schedule: 0 auto (= 3), 1 per-hyperplane launches,
+                                 2..5 pipelined line tiles, chunks of 8, 4, 2, 1 steps          */
     double mixed_inner_tol;   /* mixed mode: relative reduction asked of each FP32 inner solve */
     int    mixed_max_outer;   /* mixed mode: cap on FP64 refinement steps           */
     int    reserved[5];
@@ -105,6 +106,10 @@ typedef struct xf_solve_stats {
     double residual;          /* final inf-norm residual (FP64)                    */
     double rhs_norm;          /* inf-norm of the right-hand side                   */
     double ms;                /* wall time of the solve on the device, ms          */
+    double sweep_ms;          /* timing on, pipelined sweeps: summed device time of
+                                 the MIC(0) sweep launches, ms                     */
+    int    sweeps;            /* number of sweep launches summed in sweep_ms       */
+    int    reserved2;
 } xf_solve_stats;
 
 /* Per-substep stage timings (device events, ms), filled when timing is on. */

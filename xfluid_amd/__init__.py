@@ -38,7 +38,8 @@ class XfParams(C.Structure):
 
 class XfSolveStats(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("outer", C.c_int), ("reserved", C.c_int),
-                ("residual", C.c_double), ("rhs_norm", C.c_double), ("ms", C.c_double)]
+                ("residual", C.c_double), ("rhs_norm", C.c_double), ("ms", C.c_double),
+                ("sweep_ms", C.c_double), ("sweeps", C.c_int), ("reserved2", C.c_int)]
 
 
 class XfStepTiming(C.Structure):

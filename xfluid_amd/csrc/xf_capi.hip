@@ -186,7 +186,15 @@ int st_solve(xf_sim* s) {
     launch_stencil_rhs(s->g, s->mat, s->u, s->v, s->w, s->code, s->pw.b, s->st);
     s->last.reserved = 0;
     const xf_params& p = s->prm;
-    int sweep = p.sweep_kernel == 0 ? 1 : p.sweep_kernel;
+    SweepTimer& tm = s->pw.timer;
+    tm.on = s->timing;
+    tm.used = 0;
+    if (tm.on && !tm.ev) {
+        tm.cap = 4096;
+        tm.ev = new hipEvent_t[tm.cap];
+        for (int i = 0; i < tm.cap; ++i) HIPCHK(hipEventCreate(&tm.ev[i]));
+    }
+    int sweep = p.sweep_kernel == 0 ? 3 : p.sweep_kernel;
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
     if (sweep >= 2) {
@@ -196,6 +204,7 @@ int st_solve(xf_sim* s) {
         if (errw) {  // a pipelined wait timed out: redo the solve with per-hyperplane launches
             fprintf(stderr, "xfluid: pipelined sweep wait timed out; solving with per-plane sweeps\n");
             HIPCHK(hipMemsetAsync(s->pw.sweep_flags, 0, sizeof(int), s->st));
+            tm.used = 0;
             r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                           p.mixed_inner_tol, p.mixed_max_outer, 1, s->st);
             s->last.reserved = 1;
@@ -214,6 +223,16 @@ int st_solve(xf_sim* s) {
     s->last.residual = r.residual;
     s->last.rhs_norm = r.rhs_norm;
     s->last.ms = ms;
+    s->last.sweep_ms = 0.0;
+    s->last.sweeps = 0;
+    for (int i = 0; i + 1 < tm.used; i += 2) {
+        float d = 0;
+        if (hipEventElapsedTime(&d, tm.ev[i], tm.ev[i + 1]) == hipSuccess) {
+            s->last.sweep_ms += d;
+            s->last.sweeps += 2;
+        }
+    }
+    tm.used = 0;
     return r.status;
 }
 int st_apply(xf_sim* s) {
@@ -323,6 +342,17 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     s->pw.nflags = sweep_flag_words(s->g);
     s->pw.epoch = 1;
     if (dalloc(&s->pw.sweep_flags, s->pw.nflags) != hipSuccess) return fail(set_err(-2, "alloc flags"));
+    s->pw.nskew = sweep_skewed_size(s->g);
+    {
+        const long M = s->pw.nskew;
+        SweepSkew<double>& a = s->pw.sk64;
+        SweepSkew<float>& b = s->pw.sk32;
+        if (dalloc(&a.code, M) != hipSuccess || dalloc(&a.inv, M) != hipSuccess || dalloc(&a.r, M) != hipSuccess ||
+            dalloc(&a.q, M) != hipSuccess || dalloc(&b.inv, M) != hipSuccess || dalloc(&b.r, M) != hipSuccess ||
+            dalloc(&b.q, M) != hipSuccess)
+            return fail(set_err(-2, "alloc sweep buffers"));
+        b.code = a.code;
+    }
     hipMemsetAsync(s->pw.sweep_flags, 0, s->pw.nflags * sizeof(int), s->st);
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
     if (dalloc(&s->ci.scan_tmp, s->ci.scan_tmp_n) != hipSuccess) return fail(set_err(-2, "alloc scan"));
@@ -363,6 +393,12 @@ void xf_sim_destroy(xf_sim* s) {
                       &s->P.key, &s->P.perm, &s->surf_cnt, &s->surf_start};
     for (auto* p : is) dfree(*p);
     dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
+    dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q);
+    dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q);
+    if (s->pw.timer.ev) {
+        for (int i = 0; i < s->pw.timer.cap; ++i) hipEventDestroy(s->pw.timer.ev[i]);
+        delete[] s->pw.timer.ev;
+    }
     if (s->pw.S_host) hipHostFree(s->pw.S_host);
     if (s->h_red) hipHostFree(s->h_red);
     for (auto& e : s->ev) if (e) hipEventDestroy(e);
@@ -709,7 +745,7 @@ int xf_sim_apply_precond(xf_sim* s, const double* r, double* z) {
     if (check(s) || !r || !z) return set_err(-1, "null argument");
     const long N = s->g.N;
     HIPCHK(hipMemcpyAsync(s->pw.s, r, N * 8, hipMemcpyHostToDevice, s->st));
-    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, s->prm.sweep_kernel, s->pw.sweep_flags, &s->pw.epoch, s->st);
+    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, s->prm.sweep_kernel, s->pw, s->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(z, s->pw.z, N * 8, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));

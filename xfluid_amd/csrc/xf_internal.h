@@ -25,6 +25,20 @@ void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* mar
                         hipStream_t st);
 
 // ---------------- pressure solve kernels (xf_pcg.hip) ----------------
+// Inputs/outputs of the pipelined sweeps in their skewed tile layout
+// (xf_sweep.hip): stencil codes, pivots, the residual and the forward result.
+template <class T>
+struct SweepSkew {
+    uint16_t* code;
+    T *inv, *r, *q;
+};
+// Optional device timing of the sweep launches (pairs of events around the
+// forward + backward launches of each preconditioner application).
+struct SweepTimer {
+    hipEvent_t* ev;
+    int cap, used;
+    bool on;
+};
 struct PcgWork {
     // FP64 vectors
     double *x, *r, *s, *z, *q, *b, *inv;
@@ -34,8 +48,12 @@ struct PcgWork {
     int npartial;
     PcgScalars* S;       // device scalars
     PcgScalars* S_host;  // pinned mirror
-    int* sweep_flags;    // error word, progress counters, tickets of the pipelined sweep
+    int* sweep_flags;    // error word + progress counters of the pipelined sweep
     int nflags;
+    SweepSkew<double> sk64;  // skewed copies for the pipelined sweeps (FP64 solve)
+    SweepSkew<float> sk32;   // (FP32 inner solve; shares sk64.code)
+    long nskew;
+    SweepTimer timer;
     unsigned long long epoch;  // sweep launch counter (tags progress counters)
 };
 struct PcgResult {
@@ -48,10 +66,18 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
 // Individual pieces, for parity checks.
 void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, hipStream_t st);
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
-                 int sweep_kernel, int* flags, unsigned long long* epoch, hipStream_t st);
+                 int sweep_kernel, PcgWork& w, hipStream_t st);
 void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st);
 // ints needed for the pipelined sweep's error word + progress counters
 int sweep_flag_words(const Geo& g);
+// elements of one skewed-layout array
+long sweep_skewed_size(const Geo& g);
+// skewed copies of the stencil codes (if code != null) and pivots
+template <class T>
+void sweep_prepare(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
+template <class T>
+bool sweep_precond_pipelined(const Geo& g, int kind, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S,
+                             int* flags, unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
 
 // ---------------- particle kernels (xf_particles.hip) ----------------
 struct Particles {

@@ -144,18 +144,18 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     z[c] = v * ic;
 }
 
-// Pipelined sweep schedule (xf_sweep.hip).
-template <class T>
-bool sweep_pipelined(const Geo& g, int kind, bool forward, const uint16_t* code, const T* inv, const T* in, T* out,
-                     const PcgScalars* S, int* flags, unsigned long long* epoch, hipStream_t st);
-
 template <class T>
 static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
-                      int sweep_kernel, int* flags, unsigned long long* epoch, hipStream_t st) {
-    if (sweep_kernel >= 2 && flags) {
-        if (sweep_pipelined<T>(g, sweep_kernel, true, code, inv, r, q, S, flags, epoch, st) &&
-            sweep_pipelined<T>(g, sweep_kernel, false, code, inv, q, z, S, flags, epoch, st))
+                      int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
+                      SweepTimer* tm, hipStream_t st) {
+    // pipelined schedule (xf_sweep.hip) on the skewed copies prepared by the caller
+    if (sweep_kernel >= 2 && sk) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (tm && tm->on && tm->used + 2 <= tm->cap) { a = tm->ev[tm->used]; b = tm->ev[tm->used + 1]; }
+        if (sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st)) {
+            if (a) tm->used += 2;
             return;
+        }
     }
     const int planes = g.ni + g.nj + g.nk - 2;
     for (int t = 0; t < planes; ++t) {
@@ -173,8 +173,9 @@ static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
 }
 
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
-                 int sweep_kernel, int* flags, unsigned long long* epoch, hipStream_t st) {
-    precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, flags, epoch, st);
+                 int sweep_kernel, PcgWork& w, hipStream_t st) {
+    if (sweep_kernel >= 2) sweep_prepare<double>(g, code, inv, w.sk64, st);
+    precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, nullptr, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,12 +350,14 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
 // x must be zeroed by the caller; rhs is consumed as the initial residual r.
 template <class T>
 static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* z, T* q, double* partial,
-                     PcgScalars* S, PcgScalars* S_host, double tol_factor, int max_iter, int sweep_kernel, int* flags,
-                     unsigned long long* epoch, hipStream_t st) {
+                     PcgScalars* S, PcgScalars* S_host, double tol_factor, int max_iter, int sweep_kernel,
+                     const SweepSkew<T>* sk, int* flags, unsigned long long* epoch, SweepTimer* tm,
+                     hipStream_t st) {
     const long N = g.N;
+    const int tm_start = tm ? tm->used : 0;
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
-    precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, epoch, st);
+    precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st);
     k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
     k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, RED_BLOCKS, S, tol_factor, 0);
     k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
@@ -367,7 +370,7 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             k_finish<<<1, 1024, 0, st>>>(FIN_ALPHA, partial, RED_BLOCKS, S, tol_factor, it);
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it);
-            precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, flags, epoch, st);
+            precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st);
             k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_BETA, partial, RED_BLOCKS, S, tol_factor, it);
             k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
@@ -381,6 +384,12 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
         hipStreamSynchronize(st);
         if (!S_host->done) { S_host->status = 1; S_host->iters = max_iter; }
     }
+    if (tm && tm->used > tm_start) {
+        // keep the timings of the applications that ran: the initial one and one per
+        // iteration before the converged one (later launches returned at once)
+        const int apps = S_host->res0 == 0 ? 0 : S_host->status == 1 ? 1 + max_iter : S_host->iters;
+        if (tm->used > tm_start + 2 * apps) tm->used = tm_start + 2 * apps;
+    }
 }
 
 PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,
@@ -389,11 +398,12 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (tol_factor < 1e-30) tol_factor = 1e-30;
     PcgResult res{};
     pcg_mic0(g, code, w.inv, tau, sigma, st);
+    if (sweep_kernel >= 2) sweep_prepare<double>(g, code, w.inv, w.sk64, st);
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
         pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.z, w.q, w.partial, w.S, w.S_host, tol_factor, max_iter,
-                         sweep_kernel, w.sweep_flags, &w.epoch, st);
+                         sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, &w.timer, st);
         res.status = w.S_host->status;
         res.iterations = w.S_host->iters;
         res.outer = 1;
@@ -407,6 +417,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     PcgScalars* S_outer = w.S + 1;
     PcgScalars* S_outer_h = w.S_host + 1;
     k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.inv32, w.inv, nullptr);
+    if (sweep_kernel >= 2) sweep_prepare<float>(g, nullptr, w.inv32, w.sk32, st);
     k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
     k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
     k_absmax<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.partial);
@@ -424,7 +435,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
         pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.z32, w.q32, w.partial, w.S, w.S_host, inner_tol,
-                        max_iter, sweep_kernel, w.sweep_flags, &w.epoch, st);
+                        max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
         total += w.S_host->iters;
         ++outer;
         k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
