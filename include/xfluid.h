@@ -175,6 +175,25 @@ int  xf_sim_frame_count(const xf_sim* s);
  *      `cap` entries are written. */
 int64_t xf_sim_surface_cells(xf_sim* s, int32_t* cells, uint8_t* cube_index, int64_t cap);
 
+/* ---- surface mesh: FluidMesher::generateMesh (fluidmesher.cpp:364-392,
+ *      _generateSurfaceCellMesh :395-545, vertexInterp :4-38) on the current
+ *      density field, with its smoothing (_smoothMesh :547-556 ->
+ *      TriangleMesh::smooth, trianglemesh.cpp:736-810) and vertex normals
+ *      (trianglemesh.cpp:332-360).  Vertices and triangles come in the
+ *      reference's order.  smooth = 0 skips the smoothing pass. */
+int  xf_sim_generate_mesh(xf_sim* s, int smooth, int64_t* nverts, int64_t* ntris);
+/* Copies the last mesh: verts 3*nverts doubles, tris 3*ntris ints, normals
+ * 3*nverts doubles; any pointer may be NULL. */
+int  xf_sim_get_mesh(xf_sim* s, double* verts, int32_t* tris, double* normals);
+/* TriangleMesh::writeMeshToPLY (trianglemesh.cpp:95-252): binary little-endian
+ * PLY of the last mesh (float x,y,z; uchar 3 + int indices per face). */
+int  xf_sim_write_mesh_ply(xf_sim* s, const char* path);
+/* Frame output of FluidSimulator::timeStep (FluidSimulator.cpp:290-305):
+ * with a prefix set, each xf_sim_time_step(..., output_mesh=1) (the last
+ * substep of xf_sim_frame_step) builds the mesh and writes
+ * <prefix><frame>.ply.  NULL disables writing (the mesh is still built). */
+int  xf_sim_set_mesh_output(xf_sim* s, const char* prefix);
+
 /* ---- operator diagnostics on the last assembled system (parity hooks):
  *      y = A x (FixedSparseMatrix multiply, sparse_matrix.h:266-276) and
  *      z = (L L^T)^-1 r (PCGSolver::apply_preconditioner, pcg_solver.h:305-309).

@@ -18,7 +18,10 @@
 #include <cstdio>
 #include <vector>
 #include <algorithm>
+#include <climits>
+#include <string>
 #include "../include/xfluid.h"
+#include "../xfluid_amd/csrc/xf_mc_tables.h"
 
 namespace {
 
@@ -426,6 +429,108 @@ struct Sim {
                 for (int t = 0; t < 8; ++t) if (PHI(a + cv[t][0], b + cv[t][1], c + cv[t][2]) > thr) ci |= 1 << t;
                 cube.push_back((uint8_t)ci);
             }
+        }
+    }
+
+    // ---- a19 mesh: FluidMesher::generateMesh / _generateSurfaceCellMesh /
+    // vertexInterp (fluidmesher.cpp:364-545, 4-38), smoothing and normals
+    // (fluidmesher.cpp:547-556; trianglemesh.cpp:332-360, 609-630, 736-810) ----
+    std::vector<double> mverts;    // xyz per vertex
+    std::vector<int32_t> mtris;    // 3 vertex indices per triangle
+    std::vector<double> mnormals;  // xyz per vertex
+    void mesh(bool smooth) {
+        static constexpr McTables T = mc_make_tables();
+        const int dd = d();
+        const long VI = dd * ni + 1, VJ = dd * nj + 1, VK = dd * nk + 1, NV = VI * VJ * VK;
+        const long sni = (long)dd * ni, snj = (long)dd * nj;
+        const double sdx = dx / (double)dd;
+        const double thr = prm.iso;
+        std::vector<int32_t> edge(3 * NV, -1);  // edgeU/V/W keyed by the lower endpoint
+        mverts.clear(); mtris.clear();
+        for (size_t q = 0; q < surf.size(); ++q) {
+            const long a = surf[q] % sni, b = (surf[q] / sni) % snj, c = surf[q] / (sni * snj);
+            const int ci = cube[q];
+            const uint16_t cut = T.cut[ci];
+            if (cut == 0) continue;
+            int vl[12];
+            for (int e = 0; e < 12; ++e) {
+                if (!(cut & (1 << e))) continue;
+                const int* l = kMcCorner[kMcEdge[e][0]];
+                const int* r = kMcCorner[kMcEdge[e][1]];
+                const long li = a + l[0], lj = b + l[1], lk = c + l[2];
+                const long key = kMcEdgeDir[e] * NV + li + VI * (lj + VJ * lk);
+                if (edge[key] == -1) {
+                    const long ri = a + r[0], rj = b + r[1], rk = c + r[2];
+                    const double v1 = phi[li + VI * (lj + VJ * lk)], v2 = phi[ri + VI * (rj + VJ * rk)];
+                    const double p1[3] = {(li + 0.5) * sdx, (lj + 0.5) * sdx, (lk + 0.5) * sdx};
+                    const double p2[3] = {(ri + 0.5) * sdx, (rj + 0.5) * sdx, (rk + 0.5) * sdx};
+                    double t = v1 - v2;
+                    if (-1e-7 < t && 1e-7 > t) t = 0.5;
+                    else t = (v1 - thr) / t;
+                    for (int x = 0; x < 3; ++x) mverts.push_back(p1[x] + (p2[x] - p1[x]) * t);
+                    edge[key] = (int32_t)(mverts.size() / 3 - 1);
+                }
+                vl[e] = edge[key];
+            }
+            for (int t = 0; t < 3 * T.ntri[ci]; ++t) mtris.push_back(vl[T.tri[ci][t]]);
+        }
+        const long nv = (long)mverts.size() / 3, nt = (long)mtris.size() / 3;
+        // _updateVertexTriangles: per vertex, its triangles in ascending order
+        std::vector<std::vector<int32_t>> vt(nv);
+        for (long t = 0; t < nt; ++t)
+            for (int x = 0; x < 3; ++x) vt[mtris[3 * t + x]].push_back((int32_t)t);
+        normals(vt);  // generateMesh: updateVertexNormals() before smoothing
+        if (!smooth) return;
+        // smooth(0.8, 4, all vertices): Jacobi umbrella steps, neighbours with multiplicity
+        for (int it = 0; it < 4; ++it) {
+            std::vector<double> nvx(3 * nv);
+            for (long i = 0; i < nv; ++i) {
+                double avg[3] = {0.0, 0.0, 0.0};
+                int count = 0;
+                for (int32_t t : vt[i])
+                    for (int x = 0; x < 3; ++x) {
+                        const int32_t o = mtris[3 * t + x];
+                        if (o == i) continue;
+                        for (int y = 0; y < 3; ++y) avg[y] = avg[y] + mverts[3 * o + y];
+                        ++count;
+                    }
+                for (int y = 0; y < 3; ++y) {
+                    const double m = avg[y] / (double)count, v0 = mverts[3 * i + y];
+                    nvx[3 * i + y] = v0 + (m - v0) * 0.8;
+                }
+            }
+            mverts.swap(nvx);
+        }
+        normals(vt);
+    }
+    // trianglemesh.cpp:332-360 (the sum starts from zero: the reference starts it
+    // from a default-constructed, uninitialised Vec3d at :351)
+    void normals(const std::vector<std::vector<int32_t>>& vt) {
+        const long nv = (long)mverts.size() / 3, nt = (long)mtris.size() / 3;
+        std::vector<double> fn(3 * nt);
+        auto nrm = [](double* x) {
+            const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+            x[0] = x[0] / l; x[1] = x[1] / l; x[2] = x[2] / l;
+        };
+        for (long t = 0; t < nt; ++t) {
+            const double* p0 = &mverts[3 * mtris[3 * t]];
+            const double* p1 = &mverts[3 * mtris[3 * t + 1]];
+            const double* p2 = &mverts[3 * mtris[3 * t + 2]];
+            const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+            nrm(c);
+            for (int x = 0; x < 3; ++x) fn[3 * t + x] = c[x];
+        }
+        mnormals.assign(3 * nv, 0.0);
+        for (long i = 0; i < nv; ++i) {
+            double n[3] = {0.0, 0.0, 0.0};
+            for (int32_t t : vt[i])
+                for (int x = 0; x < 3; ++x) n[x] = n[x] + fn[3 * t + x];
+            const double cnt = (double)(float)vt[i].size();
+            for (int x = 0; x < 3; ++x) n[x] = n[x] / cnt;
+            nrm(n);
+            for (int x = 0; x < 3; ++x) mnormals[3 * i + x] = n[x];
         }
     }
 
@@ -901,5 +1006,20 @@ int64_t xo_surface_cells(void* h, int32_t* cells, uint8_t* cube, int64_t cap) {
     return n;
 }
 double xo_particle_radius(void* h) { return ((Sim*)h)->particle_radius(); }
+
+// mesh of the current density field and surface list (stage DENSITY first)
+int64_t xo_mesh(void* h, int smooth, int64_t* ntris) {
+    Sim* s = (Sim*)h;
+    s->surface();
+    s->mesh(smooth != 0);
+    *ntris = (int64_t)s->mtris.size() / 3;
+    return (int64_t)s->mverts.size() / 3;
+}
+void xo_get_mesh(void* h, double* verts, int32_t* tris, double* normals) {
+    Sim* s = (Sim*)h;
+    if (verts) std::memcpy(verts, s->mverts.data(), s->mverts.size() * sizeof(double));
+    if (tris) std::memcpy(tris, s->mtris.data(), s->mtris.size() * sizeof(int32_t));
+    if (normals) std::memcpy(normals, s->mnormals.data(), s->mnormals.size() * sizeof(double));
+}
 
 }  // extern "C"

@@ -60,6 +60,8 @@ struct Ref {
     double res = 0;
     int status = 0;
     double blob = 0;
+    std::vector<Vec3d> mverts;  // last mesh (copied out of a TriangleMesh)
+    std::vector<Triangle> mtris;
     Ref(int ni, int nj, int nk) : sim(ni, nj, nk) {}
 };
 
@@ -72,6 +74,10 @@ void to_host(Array3d<T>& a, void* dst) {
 }  // namespace
 
 extern "C" {
+
+// Directory prefix of the reference's frame meshes (config.h:4-7, used at
+// FluidSimulator.cpp:298-301); the harness points it at a scratch directory.
+void xr_set_output(const char* prefix) { Config::outputFild = prefix; }
 
 void* xr_create(int ni, int nj, int nk, double dx) {
     Ref* r = new Ref(ni, nj, nk);
@@ -242,6 +248,51 @@ int64_t xr_surface_cells(void* h, int32_t* cells, int64_t cap) {
     for (int64_t i = 0; i < n && i < cap; ++i)
         cells[i] = (int32_t)(sc[i].i + (int64_t)(d * s._ni) * (sc[i].j + (int64_t)(d * s._nj) * sc[i].k));
     return n;
+}
+
+// FluidMesher::generateMesh (fluidmesher.cpp:364-392) on the current density
+// field and surface list.  smooth = 0 runs the same per-cell loop without the
+// final smoothing pass (for checking vertex creation and triangles alone).
+int64_t xr_mesh(void* h, int smooth, int64_t* ntris) {
+    Ref* r = (Ref*)h;
+    FluidSimulator& s = r->sim;
+    Quiet q;
+    TriangleMesh mesh;
+    if (smooth) {
+        s._mesher.generateMesh(s._dis_field, mesh);
+    } else {
+        FluidMesher& m = s._mesher;
+        m._cur_field = &s._dis_field;
+        Vec3i dim = s._dis_field.subSize();
+        Array3d<int> eu(dim.x + 1, dim.y + 1, dim.z + 1), ev(dim.x + 1, dim.y + 1, dim.z + 1),
+            ew(dim.x + 1, dim.y + 1, dim.z + 1);
+        eu.fill(-1); ev.fill(-1); ew.fill(-1);
+        const std::vector<GridIndex3d>& sc = s._dis_field.getSubSurfaceCell();
+        for (size_t i = 0; i < sc.size(); ++i) m._generateSurfaceCellMesh(sc[i], mesh, eu, ev, ew);
+        m._cur_field = nullptr;
+    }
+    r->mverts = mesh.vertices;
+    r->mtris = mesh.triangles;
+    *ntris = (int64_t)r->mtris.size();
+    return (int64_t)r->mverts.size();
+}
+void xr_get_mesh(void* h, double* verts, int32_t* tris) {
+    Ref* r = (Ref*)h;
+    for (size_t i = 0; i < r->mverts.size(); ++i) {
+        verts[3 * i] = r->mverts[i].x;
+        verts[3 * i + 1] = r->mverts[i].y;
+        verts[3 * i + 2] = r->mverts[i].z;
+    }
+    for (size_t t = 0; t < r->mtris.size(); ++t)
+        for (int x = 0; x < 3; ++x) tris[3 * t + x] = r->mtris[t].tri[x];
+}
+// TriangleMesh::writeMeshToPLY (trianglemesh.cpp:95-252) of the last mesh
+void xr_write_ply(void* h, const char* path) {
+    Ref* r = (Ref*)h;
+    TriangleMesh mesh;
+    mesh.vertices = r->mverts;
+    mesh.triangles = r->mtris;
+    mesh.writeMeshToPLY(path);
 }
 
 // Standalone solver pieces on an assembled matrix, for pinning the restated

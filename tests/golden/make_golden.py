@@ -45,7 +45,28 @@ def pressure():
                         w=r.get(FIELD["W"]), A7=A, b=b, inv=inv, r=x, z=z)
 
 
+def mesh():
+    """FluidMesher::generateMesh + TriangleMesh::writeMeshToPLY on a dam-break
+    density field: phi, the smoothed vertices, triangles and the PLY bytes."""
+    import tempfile
+    dims, dx, frac, seed = (14, 12, 10), 0.1, (0.5, 0.6, 1.0), 6
+    o = oracle_dam_break(*dims, dx, frac, seed, small_params())
+    r = ref_like(o)
+    for _ in range(3):
+        r.time_step(1.0 / 60)
+    r.stage(0, 0.0)
+    verts, tris = r.mesh(True)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "mesh.ply")
+        r.write_ply(path)
+        with open(path, "rb") as f:
+            ply = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "dam14x12x10_mesh.npz"), dims=np.array(dims), dx=dx,
+                        phi=r.get(FIELD["PHI"]), verts=verts, tris=tris, ply=ply)
+
+
 if __name__ == "__main__":
     substeps()
     pressure()
+    mesh()
     print("golden fixtures written")

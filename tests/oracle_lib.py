@@ -5,8 +5,11 @@ imports this module.
 Both expose the same handle shape as the HIP library so a test can drive the
 reference, the oracle and the GPU through one scenario object (`Scene`).
 """
+import atexit
 import ctypes as C
 import os
+import shutil
+import tempfile
 
 import numpy as np
 
@@ -35,8 +38,10 @@ class XfParams(C.Structure):
 
 
 class XfSolveStats(C.Structure):
+    # layout of xf_solve_stats (include/xfluid.h)
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("outer", C.c_int), ("reserved", C.c_int),
-                ("residual", C.c_double), ("rhs_norm", C.c_double), ("ms", C.c_double)]
+                ("residual", C.c_double), ("rhs_norm", C.c_double), ("ms", C.c_double),
+                ("sweep_ms", C.c_double), ("sweeps", C.c_int), ("reserved2", C.c_int)]
 
 
 def _load_oracle():
@@ -64,6 +69,9 @@ def _load_oracle():
     lib.xo_last_solve.argtypes = [C.c_void_p, C.POINTER(XfSolveStats)]
     lib.xo_surface_cells.restype = C.c_int64
     lib.xo_surface_cells.argtypes = [C.c_void_p, _i32, _u8, C.c_int64]
+    lib.xo_mesh.restype = C.c_int64
+    lib.xo_mesh.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+    lib.xo_get_mesh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.xo_particle_radius.restype = C.c_double
     lib.xo_particle_radius.argtypes = [C.c_void_p]
     g = [C.c_int] * 3
@@ -103,6 +111,16 @@ def _load_ref():
     lib.xr_surface_cells.restype = C.c_int64
     lib.xr_surface_cells.argtypes = [C.c_void_p, _i32, C.c_int64]
     lib.xr_pressure_pieces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    lib.xr_mesh.restype = C.c_int64
+    lib.xr_mesh.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+    lib.xr_get_mesh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.xr_write_ply.argtypes = [C.c_void_p, C.c_char_p]
+    lib.xr_set_output.argtypes = [C.c_char_p]
+    # the reference writes each frame's mesh to <Config::outputFild><frame>.ply:
+    # send those files to a scratch directory removed at exit
+    out = tempfile.mkdtemp(prefix="xfref_frames_")
+    atexit.register(shutil.rmtree, out, True)
+    lib.xr_set_output(os.fsencode(out + os.sep))
     return lib
 
 
@@ -208,6 +226,14 @@ class OracleSim:
         self.lib.xo_last_solve(self.h, C.byref(st))
         return st
 
+    def mesh(self, smooth=True):
+        """Mesh of the current density field (surface list recomputed from phi)."""
+        nt = C.c_int64()
+        nv = self.lib.xo_mesh(self.h, int(smooth), C.byref(nt))
+        verts = np.empty((nv, 3)); tris = np.empty((nt.value, 3), np.int32); normals = np.empty((nv, 3))
+        self.lib.xo_get_mesh(self.h, verts.ctypes.data, tris.ctypes.data, normals.ctypes.data)
+        return verts, tris, normals
+
     def surface_cells(self):
         n = self.lib.xo_surface_cells(self.h, np.empty(1, np.int32), np.empty(1, np.uint8), 0)
         cells = np.empty(max(n, 1), np.int32)
@@ -272,6 +298,16 @@ class RefSim:
         it, res, st = C.c_int(), C.c_double(), C.c_int()
         self.lib.xr_last_solve(self.h, C.byref(it), C.byref(res), C.byref(st))
         return it.value, res.value, st.value
+
+    def mesh(self, smooth=True):
+        nt = C.c_int64()
+        nv = self.lib.xr_mesh(self.h, int(smooth), C.byref(nt))
+        verts = np.empty((nv, 3)); tris = np.empty((nt.value, 3), np.int32)
+        self.lib.xr_get_mesh(self.h, verts.ctypes.data, tris.ctypes.data)
+        return verts, tris
+
+    def write_ply(self, path):
+        self.lib.xr_write_ply(self.h, os.fsencode(path))
 
     def surface_cells(self):
         n = self.lib.xr_surface_cells(self.h, np.empty(1, np.int32), 0)

@@ -156,6 +156,67 @@ def test_surface_classification_exact(xf):
     _exact(qg, qo, "cube index")
 
 
+@pytest.mark.parametrize("smooth", [False, True])
+def test_mesh_exact(xf, smooth):
+    """Same density field -> the same mesh as the oracle: vertices in creation
+    order, triangles, smoothed positions and normals, bit for bit."""
+    ni, nj, nk, dx, frac, seed = CASES["c1_quarter"]
+    say(f"[mesh] c1_quarter smooth={smooth}")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    _advance(o, 2, 1.0 / 60)
+    o.stage(STAGE["DENSITY"])
+    g = gpu_like(o)
+    g.set_field(FIELD["PHI"], o.get(FIELD["PHI"]))
+    vo, to, no = o.mesh(smooth)
+    vg, tg, ng = g.generate_mesh(smooth)
+    assert len(to) > 1000
+    _exact(tg, to, "triangles")
+    _exact(vg, vo, "vertices")
+    _exact(ng, no, "normals")
+
+
+def test_mesh_golden_ply(xf, tmp_path):
+    """The reference's own mesh and PLY bytes (golden fixture) from its density field."""
+    import os
+    g0 = np.load(os.path.join(os.path.dirname(__file__), "golden", "dam14x12x10_mesh.npz"))
+    ni, nj, nk = (int(x) for x in g0["dims"])
+    say("[mesh] golden 14x12x10")
+    p = xf.default_params(**{k: getattr(small_params(), k) for k in ("cfl", "pcg_tol", "pcg_max_iter")})
+    g = xf.FluidSimulator(ni, nj, nk, p)
+    g.setCellWidth(float(g0["dx"]))
+    g.set_field(FIELD["PHI"], g0["phi"])
+    v, t, _ = g.generate_mesh(True)
+    _exact(t, g0["tris"], "triangles")
+    _exact(v, g0["verts"], "vertices")
+    path = tmp_path / "m.ply"
+    g.write_mesh_ply(str(path))
+    assert path.read_bytes() == g0["ply"].tobytes()
+
+
+def test_mesh_frame_output(xf, tmp_path):
+    """frameStep writes <prefix><frame>.ply on its last substep (FluidSimulator.cpp:290-305)."""
+    ni, nj, nk, dx, frac, seed = CASES["c1_small"]
+    say("[mesh] frame output c1_small")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    g = gpu_like(o)
+    prefix = str(tmp_path) + "/frame_"
+    g.set_mesh_output(prefix)
+    assert g.frameStep(1.0 / 30) == 0
+    assert g.frameStep(1.0 / 30) == 0
+    for f in (0, 1):
+        data = (tmp_path / f"frame_{f}.ply").read_bytes()
+        head = data[:data.index(b"end_header\n") + 11].decode()
+        nv = int(head.split("element vertex ")[1].split()[0])
+        nt = int(head.split("element face ")[1].split()[0])
+        assert nv > 100 and nt > 100
+        assert len(data) == len(head) + 12 * nv + 13 * nt
+    # the second file is the mesh of the density field of frame 1's last substep,
+    # which is still the current one
+    g.generate_mesh(True)
+    g.write_mesh_ply(str(tmp_path / "again.ply"))
+    assert (tmp_path / "again.ply").read_bytes() == (tmp_path / "frame_1.ply").read_bytes()
+
+
 def test_trajectory_tolerance(xf):
     """Five full substeps run independently on GPU and oracle stay within tolerance."""
     ni, nj, nk, dx, frac, seed = CASES["c1_small"]

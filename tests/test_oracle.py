@@ -141,6 +141,76 @@ def test_pressure_pieces_match_reference():
     _same(z, z_r, "preconditioner")
 
 
+@needs_ref
+@pytest.mark.parametrize("smooth", [False, True])
+def test_mesh_matches_reference(smooth, tmp_path):
+    """Marching-cubes vertices (creation order), triangles and the smoothed
+    positions equal FluidMesher::generateMesh's on the same density field."""
+    p = small_params()
+    o = oracle_dam_break(14, 12, 10, 0.1, (0.5, 0.6, 1.0), 5, p)
+    for _ in range(2):
+        o.time_step(1.0 / 60)
+    r = ref_like(o)
+    o.stage(STAGE["DENSITY"])
+    r.stage(STAGE["DENSITY"])
+    _same(o.get(FIELD["PHI"]), r.get(FIELD["PHI"]), "phi")
+    vo, to, _ = o.mesh(smooth)
+    vr, tr = r.mesh(smooth)
+    assert len(to) > 100
+    _same(to, tr, "triangles")
+    _same(vo, vr, "vertices")
+    if smooth:
+        # the reference's PLY bytes = header + float32 xyz + (uchar 3, int32 x3) per face
+        path = tmp_path / "ref.ply"
+        r.write_ply(str(path))
+        data = path.read_bytes()
+        assert data == ply_bytes(vo, to)
+
+
+def ply_bytes(verts, tris):
+    """Binary little-endian PLY in TriangleMesh::writeMeshToPLY's layout."""
+    nv, nt = len(verts), len(tris)
+    h = "ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % nv
+    h += "property float x\nproperty float y\nproperty float z\n"
+    if nv == 0:
+        h += "property uchar red\nproperty uchar green\nproperty uchar blue\n"
+    h += "element face %d\nproperty list uchar int vertex_index\nend_header\n" % nt
+    faces = np.zeros(nt, dtype=[("n", "u1"), ("i", "<i4", 3)])
+    faces["n"] = 3
+    faces["i"] = tris
+    return h.encode() + np.asarray(verts, "<f4").tobytes() + faces.tobytes()
+
+
+def test_mesh_closed_forms():
+    """Mesh on a hand-made field: one fluid blob that is a single sub-lattice
+    vertex above iso -> an octahedron-like closed surface of 8 triangles whose
+    vertices sit at the linear-interpolation points of the 6 cut edges."""
+    p = small_params(iso=0.5)
+    o = oracle_dam_break(6, 6, 6, 0.1, (0.0, 0.0, 0.0), 0, p)
+    phi = np.zeros_like(o.get(FIELD["PHI"]))  # (k, j, i) lattice of 13^3
+    phi[6, 6, 6] = 0.8
+    o.set(FIELD["PHI"], phi)
+    v, t, n = o.mesh(False)
+    assert t.shape == (8, 3) and v.shape == (6, 3)
+    sdx = 0.05
+    c = (6 + 0.5) * sdx
+    frac = (0.8 - 0.5) / 0.8  # vertexInterp from the inside vertex: t = (v1 - iso) / (v1 - v2)
+    # every vertex lies on an axis through the centre at distance frac*sdx
+    d = v - c
+    assert np.all(np.sort(np.abs(d), axis=1)[:, :2] == 0.0)
+    np.testing.assert_allclose(np.abs(d).max(axis=1), frac * sdx, rtol=1e-14)
+    # each vertex in 4 triangles; normals point outwards (closed, consistently oriented)
+    assert np.all(np.bincount(t.ravel(), minlength=6) == 4)
+    np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1.0, rtol=1e-14)
+    assert np.all((n * d).sum(axis=1) < 0) or np.all((n * d).sum(axis=1) > 0)
+    # smoothing: the 4 neighbours of a vertex average to the centre, so each of
+    # the 4 steps v += 0.8 (avg - v) scales the radius by 0.2
+    vs, ts, _ = o.mesh(True)
+    assert np.array_equal(ts, t)
+    np.testing.assert_allclose(vs.mean(axis=0), [c, c, c], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(np.linalg.norm(vs - c, axis=1), frac * sdx * 0.2 ** 4, rtol=0, atol=1e-15)
+
+
 # ---------------------------------------------------------------------------
 # 2. golden fixtures written by the reference build
 def _golden(name):
@@ -186,6 +256,18 @@ def test_golden_pressure():
 
 # ---------------------------------------------------------------------------
 # 3. closed forms and invariants
+def test_golden_mesh():
+    """Oracle mesh of the reference's density field = the reference's mesh and PLY bytes."""
+    g = _golden("dam14x12x10_mesh.npz")
+    dims = tuple(int(x) for x in g["dims"])
+    o = oracle_dam_break(*dims, float(g["dx"]), (0.0, 0.0, 0.0), 0, small_params())
+    o.set(FIELD["PHI"], g["phi"])
+    v, t, _ = o.mesh(True)
+    _same(t, g["tris"], "triangles")
+    _same(v, g["verts"], "vertices")
+    assert ply_bytes(v, t) == g["ply"].tobytes()
+
+
 def _dense(A7, ni, nj, nk):
     N = ni * nj * nk
     nn = ni * nj

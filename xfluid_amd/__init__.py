@@ -94,6 +94,10 @@ def lib():
     L.xf_sim_frame_count.argtypes = [V]
     L.xf_sim_surface_cells.restype = C.c_int64
     L.xf_sim_surface_cells.argtypes = [V, V, V, C.c_int64]
+    L.xf_sim_generate_mesh.argtypes = [V, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.xf_sim_get_mesh.argtypes = [V, V, V, V]
+    L.xf_sim_write_mesh_ply.argtypes = [V, C.c_char_p]
+    L.xf_sim_set_mesh_output.argtypes = [V, C.c_char_p]
     L.xf_sim_apply_matrix.argtypes = [V, _dp, _dp]
     L.xf_sim_apply_precond.argtypes = [V, _dp, _dp]
     L.xf_project.argtypes = [C.c_int, C.c_int, C.c_int, _u8, _dp, _dp, _dp, V, C.POINTER(XfParams),
@@ -250,6 +254,25 @@ class FluidSimulator:
         if n:
             _chk(self.L.xf_sim_surface_cells(self.h, cells.ctypes.data, cube.ctypes.data, n), "surface_cells")
         return cells[:n], cube[:n]
+
+    def generate_mesh(self, smooth=True):
+        """FluidMesher::generateMesh on the current density field -> (verts, tris, normals)."""
+        nv, nt = C.c_int64(), C.c_int64()
+        _chk(self.L.xf_sim_generate_mesh(self.h, int(smooth), C.byref(nv), C.byref(nt)), "generate_mesh")
+        verts = np.empty((nv.value, 3), np.float64)
+        tris = np.empty((nt.value, 3), np.int32)
+        normals = np.empty((nv.value, 3), np.float64)
+        _chk(self.L.xf_sim_get_mesh(self.h, verts.ctypes.data, tris.ctypes.data, normals.ctypes.data), "get_mesh")
+        return verts, tris, normals
+
+    def write_mesh_ply(self, path):
+        """TriangleMesh::writeMeshToPLY of the last generated mesh."""
+        _chk(self.L.xf_sim_write_mesh_ply(self.h, os.fsencode(path)), "write_mesh_ply")
+
+    def set_mesh_output(self, prefix):
+        """Config::outputFild: the last substep of each frame writes <prefix><frame>.ply."""
+        _chk(self.L.xf_sim_set_mesh_output(self.h, None if prefix is None else os.fsencode(prefix)),
+             "set_mesh_output")
 
     def apply_matrix(self, x):
         x = np.ascontiguousarray(x, np.float64).ravel()

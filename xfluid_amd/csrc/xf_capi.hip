@@ -77,6 +77,9 @@ struct xf_sim {
     xf_step_timing tlast{};
     hipEvent_t ev[XF_STAGE_COUNT + 1]{};
     bool inited = false;
+    MeshWork mesh{};
+    bool mesh_out = false;
+    std::string mesh_prefix;
 
     long nu() const { return (long)(g.ni + 1) * g.nj * g.nk; }
     long nv() const { return (long)g.ni * (g.nj + 1) * g.nk; }
@@ -395,6 +398,7 @@ void xf_sim_destroy(xf_sim* s) {
     dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
     dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q);
     dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q);
+    mesh_free(s->mesh);
     if (s->pw.timer.ev) {
         for (int i = 0; i < s->pw.timer.cap; ++i) hipEventDestroy(s->pw.timer.ev[i]);
         delete[] s->pw.timer.ev;
@@ -547,6 +551,75 @@ int xf_sim_stage(xf_sim* s, int stage, double dt) {
     return rc;
 }
 
+// PLY writer: TriangleMesh::writeMeshToPLY (trianglemesh.cpp:95-252).  With no
+// vertices the reference's colour test (vertices.size() == vertexcolors.size())
+// holds and it writes the header with colour properties; reproduced.
+static int write_ply(const char* path, const std::vector<double>& verts, const std::vector<int32_t>& tris) {
+    const size_t nv = verts.size() / 3, nt = tris.size() / 3;
+    std::string h = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(nv) +
+                    "\nproperty float x\nproperty float y\nproperty float z\n";
+    if (nv == 0) h += "property uchar red\nproperty uchar green\nproperty uchar blue\n";
+    h += "element face " + std::to_string(nt) + "\nproperty list uchar int vertex_index\nend_header\n";
+    std::vector<char> bin(h.begin(), h.end());
+    const size_t off = bin.size();
+    bin.resize(off + nv * 12 + nt * 13);
+    char* q = bin.data() + off;
+    for (size_t i = 0; i < 3 * nv; ++i) {
+        const float f = (float)verts[i];
+        std::memcpy(q, &f, 4);
+        q += 4;
+    }
+    for (size_t t = 0; t < nt; ++t) {
+        *q++ = 3;
+        std::memcpy(q, &tris[3 * t], 12);
+        q += 12;
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_err(-1, "cannot open %s", path);
+    const size_t w = std::fwrite(bin.data(), 1, bin.size(), f);
+    std::fclose(f);
+    return w == bin.size() ? 0 : set_err(-2, "short write to %s", path);
+}
+
+static int mesh_build(xf_sim* s, int smooth) {
+    if (ensure_phi(s)) return -2;
+    int rc = build_mesh(s->g, s->phi, s->prm.sub_division, s->prm.iso, smooth ? 4 : 0, 0.8, s->surf_cnt,
+                        s->surf_start, s->ci.scan_tmp, s->ci.scan_tmp_n, s->mesh, s->st);
+    if (rc == -1) return set_err(-1, "density lattice too large for the mesh edge table");
+    if (rc < 0) return set_err(-2, "mesh allocation failed");
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+static int mesh_download(xf_sim* s, std::vector<double>* verts, std::vector<int32_t>* tris, std::vector<double>* normals) {
+    const MeshWork& m = s->mesh;
+    if (verts) {
+        verts->resize(3 * m.nverts);
+        if (m.nverts) HIPCHK(hipMemcpyAsync(verts->data(), m.verts, 24 * m.nverts, hipMemcpyDeviceToHost, s->st));
+    }
+    if (tris) {
+        tris->resize(3 * m.ntris);
+        if (m.ntris) HIPCHK(hipMemcpyAsync(tris->data(), m.tris, 12 * m.ntris, hipMemcpyDeviceToHost, s->st));
+    }
+    if (normals) {
+        normals->resize(3 * m.nverts);
+        if (m.nverts) HIPCHK(hipMemcpyAsync(normals->data(), m.normals, 24 * m.nverts, hipMemcpyDeviceToHost, s->st));
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+static int mesh_stage(xf_sim* s) {
+    int rc = mesh_build(s, 1);
+    if (rc < 0 || !s->mesh_out) return rc;
+    std::vector<double> v;
+    std::vector<int32_t> t;
+    if ((rc = mesh_download(s, &v, &t, nullptr)) < 0) return rc;
+    const std::string path = s->mesh_prefix + std::to_string(s->frame) + ".ply";
+    // like the reference (an unopenable file writes nothing), a failed frame
+    // output does not stop the simulation; it is reported on stderr
+    if (write_ply(path.c_str(), v, t) < 0) std::fprintf(stderr, "xfluid: %s\n", xf_last_error());
+    return 0;
+}
+
 // FluidSimulator::timeStep (FluidSimulator.cpp:265-335)
 int xf_sim_time_step(xf_sim* s, double dt, int output_mesh) {
     if (check(s)) return -1;
@@ -564,6 +637,10 @@ int xf_sim_time_step(xf_sim* s, double dt, int output_mesh) {
         int rc = run_stage(s, stg, dt);
         if (rc < 0) return rc;
         if (rc > 0) status = rc;
+        if (stg == XF_STAGE_SAVE && output_mesh) {  // FluidSimulator.cpp:290-305
+            rc = mesh_stage(s);
+            if (rc < 0) return rc;
+        }
         if (s->timing) {
             hipEventRecord(s->ev[stg], s->st);
             marks.push_back({stg, s->ev[stg]});
@@ -730,6 +807,38 @@ int64_t xf_sim_surface_cells(xf_sim* s, int32_t* cells, uint8_t* cube_index, int
     }
     HIPCHK(hipGetLastError());
     return total;
+}
+
+int xf_sim_generate_mesh(xf_sim* s, int smooth, int64_t* nverts, int64_t* ntris) {
+    if (check(s)) return -1;
+    int rc = mesh_build(s, smooth);
+    if (rc < 0) return rc;
+    if (nverts) *nverts = s->mesh.nverts;
+    if (ntris) *ntris = s->mesh.ntris;
+    return 0;
+}
+int xf_sim_get_mesh(xf_sim* s, double* verts, int32_t* tris, double* normals) {
+    if (check(s)) return -1;
+    const MeshWork& m = s->mesh;
+    if (verts && m.nverts) HIPCHK(hipMemcpyAsync(verts, m.verts, 24 * m.nverts, hipMemcpyDeviceToHost, s->st));
+    if (tris && m.ntris) HIPCHK(hipMemcpyAsync(tris, m.tris, 12 * m.ntris, hipMemcpyDeviceToHost, s->st));
+    if (normals && m.nverts) HIPCHK(hipMemcpyAsync(normals, m.normals, 24 * m.nverts, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_write_mesh_ply(xf_sim* s, const char* path) {
+    if (check(s) || !path) return set_err(-1, "null argument");
+    std::vector<double> v;
+    std::vector<int32_t> t;
+    int rc = mesh_download(s, &v, &t, nullptr);
+    if (rc < 0) return rc;
+    return write_ply(path, v, t);
+}
+int xf_sim_set_mesh_output(xf_sim* s, const char* prefix) {
+    if (check(s)) return -1;
+    s->mesh_out = prefix != nullptr;
+    s->mesh_prefix = prefix ? prefix : "";
+    return 0;
 }
 
 int xf_sim_apply_matrix(xf_sim* s, const double* x, double* y) {

@@ -109,3 +109,24 @@ int64_t surface_cells(const Geo& g, const double* phi, int d, double iso, int32_
 void launch_maxv2(const Particles& P, double* partial, int nblocks, hipStream_t st);
 void exclusive_scan_i32(const int32_t* in, int32_t* out, long n, int32_t* tmp, int tmp_n, hipStream_t st);
 long scan_tmp_size(long n);
+
+// ---------------- surface mesh (xf_mesh.hip) ----------------
+// Device buffers of the last marching-cubes mesh (grown on demand).
+struct MeshWork {
+    int32_t* owner;  // per edge of the density lattice: claiming cell, then vertex index
+    long nowner;
+    int32_t* cells;
+    uint8_t* cube;
+    int32_t *flag, *vpos, *tcnt, *tpos, *scan_tmp;
+    double *verts, *verts2, *normals, *fnorm;
+    int32_t* tris;
+    int32_t *vt_cnt, *vt_start, *vt_cursor, *vt_list;
+    long cap_cells, cap_cube, cap_flag, cap_vpos, cap_tcnt, cap_tpos, cap_scan, cap_verts, cap_verts2, cap_normals,
+        cap_fnorm, cap_tris, cap_vt_cnt, cap_vt_start, cap_vt_cursor, cap_vt_list;
+    long ncells, nverts, ntris;
+};
+// Builds the mesh of phi: 0 ok, -1 lattice too large, -2 allocation failure.
+int build_mesh(const Geo& g, const double* phi, int d, double iso, int smooth_iters, double smooth_value,
+               int32_t* count_tmp, int32_t* start_tmp, int32_t* scan_tmp, int scan_tmp_n, MeshWork& w,
+               hipStream_t st);
+void mesh_free(MeshWork& w);

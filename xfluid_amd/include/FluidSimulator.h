@@ -63,11 +63,13 @@ public:
     void init() { report(xf_sim_init(_sim)); }
     // frameStep: CFL substeps; the last one writes <Config::outputFild><frame>.ply
     void frameStep(double dt) {
+        report(xf_sim_set_mesh_output(_sim, Config::outputFild.c_str()));
         int rc = xf_sim_frame_step(_sim, dt);
         report(rc);
         std::printf("Frame   :   %d\n", xf_sim_frame_count(_sim) - 1);
     }
     void timeStep(double dt, bool outputMesh) {
+        report(xf_sim_set_mesh_output(_sim, Config::outputFild.c_str()));
         int rc = xf_sim_time_step(_sim, dt, outputMesh ? 1 : 0);
         report(rc);
     }
