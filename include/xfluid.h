@@ -194,6 +194,17 @@ int  xf_sim_write_mesh_ply(xf_sim* s, const char* path);
  * <prefix><frame>.ply.  NULL disables writing (the mesh is still built). */
 int  xf_sim_set_mesh_output(xf_sim* s, const char* prefix);
 
+/* ---- differentiation (no counterpart in the reference, which has no
+ *      gradients; for gradient-based callers such as torch.autograd) ------
+ *      Both work on the sim's current material and overwrite its velocity
+ *      (u,v,w) and pressure fields.  Arrays are the three MAC face arrays;
+ *      device_ptrs != 0 means they are device (HBM) pointers, else host. */
+/* u' = u - G A^-1 B u in place (VariationalPressureSolver::solve + applyPressure). */
+int  xf_sim_project_fields(xf_sim* s, double* u, double* v, double* w, int device_ptrs);
+/* Vector-Jacobian product of the projection: g := g + B^T A^-1 (-G^T g), in
+ * place (A symmetric: the transposed solve is a PCG solve with A). */
+int  xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs);
+
 /* ---- operator diagnostics on the last assembled system (parity hooks):
  *      y = A x (FixedSparseMatrix multiply, sparse_matrix.h:266-276) and
  *      z = (L L^T)^-1 r (PCGSolver::apply_preconditioner, pcg_solver.h:305-309).

@@ -267,6 +267,74 @@ void apply_pressure(int ni, int nj, int nk, const uint8_t* mat, const double* p,
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Adjoint of the projection u' = u - G A^-1 B u for fixed material (no
+// reference counterpart: the reference has no gradients).  B is the RHS
+// assembly above (b = B u), G the pressure application; A is symmetric, so
+// the transposed solve is a solve with A.  Given g = dL/du', returns
+// dL/du = g + B^T A^-1 (-G^T g), computed in FP64 with the pcg() above.
+
+// pbar = -G^T g (transpose of apply_pressure; outside the grid counts as solid)
+void apply_pressure_adjoint(int ni, int nj, int nk, const uint8_t* mat, const double* gu, const double* gv,
+                            const double* gw, double* pbar) {
+    const long N = (long)ni * nj * nk, nn = (long)ni * nj;
+    std::fill(pbar, pbar + N, 0.0);
+    auto M = [&](int i, int j, int k) -> uint8_t {
+        if (i < 0 || j < 0 || k < 0 || i >= ni || j >= nj || k >= nk) return SOLID;
+        return mat[i + (long)ni * j + nn * k];
+    };
+    auto add = [&](int i, int j, int k, double x) {
+        if (i < 0 || j < 0 || k < 0 || i >= ni || j >= nj || k >= nk) return;
+        pbar[i + (long)ni * j + nn * k] += x;
+    };
+    for (int axis = 0; axis < 3; ++axis) {
+        int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
+        int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
+        const double* g = axis == 0 ? gu : axis == 1 ? gv : gw;
+        for (int k = 0; k < sk; ++k)
+            for (int j = 0; j < sj; ++j)
+                for (int i = 0; i < si; ++i) {
+                    uint8_t ml = M(i + di, j + dj, k + dk), mr = M(i, j, k);
+                    if (ml != FLUID && mr != FLUID) continue;
+                    const double x = g[i + (long)si * (j + (long)sj * k)];
+                    // f' = f - (p2 - p1); air ghost p = -p(fluid)
+                    if (ml == FLUID && mr == AIR) add(i + di, j + dj, k + dk, 2.0 * x);
+                    else if (mr == FLUID && ml == AIR) add(i, j, k, -2.0 * x);
+                    else { add(i, j, k, -x); add(i + di, j + dj, k + dk, x); }
+                }
+    }
+}
+
+// g += B^T bbar (transpose of the RHS part of assemble())
+void rhs_adjoint(int ni, int nj, int nk, const uint8_t* mat, const double* bbar, double* gu, double* gv,
+                 double* gw) {
+    const long nn = (long)ni * nj;
+    const uint8_t OUT = 3;
+    auto M = [&](int i, int j, int k) -> uint8_t {
+        if (i < 0 || j < 0 || k < 0 || i >= ni || j >= nj || k >= nk) return OUT;
+        return mat[i + (long)ni * j + nn * k];
+    };
+    auto B = [&](int i, int j, int k) { return bbar[i + (long)ni * j + nn * k]; };
+    for (int axis = 0; axis < 3; ++axis) {
+        int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
+        int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
+        double* g = axis == 0 ? gu : axis == 1 ? gv : gw;
+        for (int k = 0; k < sk; ++k)
+            for (int j = 0; j < sj; ++j)
+                for (int i = 0; i < si; ++i) {
+                    const uint8_t ml = M(i + di, j + dj, k + dk), mr = M(i, j, k);
+                    double acc = 0.0;
+                    // the face is the + face (vsign -1) of the left cell, the - face (+1) of the right
+                    if (ml == FLUID) acc += -1.0 * ((mr == FLUID || mr == AIR) ? 1.0 : 0.5) * B(i + di, j + dj, k + dk);
+                    if (mr == FLUID) acc += ((ml == FLUID || ml == AIR) ? 1.0 : 0.5) * B(i, j, k);
+                    // rows of fluid-adjacent solid cells: b[m] += -vsign * face * 0.5
+                    if (ml == FLUID && mr == SOLID) acc += 0.5 * B(i, j, k);
+                    if (mr == FLUID && ml == SOLID) acc += -0.5 * B(i + di, j + dj, k + dk);
+                    g[i + (long)si * (j + (long)sj * k)] += acc;
+                }
+    }
+}
+
 struct Sim {
     int ni, nj, nk;
     double dx = 0.0001;
@@ -1006,6 +1074,32 @@ int64_t xo_surface_cells(void* h, int32_t* cells, uint8_t* cube, int64_t cap) {
     return n;
 }
 double xo_particle_radius(void* h) { return ((Sim*)h)->particle_radius(); }
+
+// Projection u' = u - G A^-1 B u on explicit arrays (in place) and its adjoint
+// g -> g + B^T A^-1 (-G^T g) (in place).  Return the PCG status.
+int xo_project(int ni, int nj, int nk, const uint8_t* mat, double* u, double* v, double* w, double tol,
+               int max_iter, double tau, double sigma) {
+    const long N = (long)ni * nj * nk;
+    std::vector<double> A(7 * N), b(N), p(N);
+    assemble(ni, nj, nk, mat, u, v, w, A.data(), b.data());
+    int it = 0;
+    double res = 0;
+    int st = pcg(ni, nj, nk, A.data(), b.data(), p.data(), tol, max_iter, tau, sigma, &it, &res, nullptr);
+    apply_pressure(ni, nj, nk, mat, p.data(), u, v, w);
+    return st;
+}
+int xo_project_adjoint(int ni, int nj, int nk, const uint8_t* mat, double* gu, double* gv, double* gw,
+                       double tol, int max_iter, double tau, double sigma) {
+    const long N = (long)ni * nj * nk;
+    std::vector<double> A(7 * N), b(N), pbar(N), bbar(N);
+    assemble(ni, nj, nk, mat, gu, gv, gw, A.data(), b.data());  // A only
+    apply_pressure_adjoint(ni, nj, nk, mat, gu, gv, gw, pbar.data());
+    int it = 0;
+    double res = 0;
+    int st = pcg(ni, nj, nk, A.data(), pbar.data(), bbar.data(), tol, max_iter, tau, sigma, &it, &res, nullptr);
+    rhs_adjoint(ni, nj, nk, mat, bbar.data(), gu, gv, gw);
+    return st;
+}
 
 // mesh of the current density field and surface list (stage DENSITY first)
 int64_t xo_mesh(void* h, int smooth, int64_t* ntris) {

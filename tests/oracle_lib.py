@@ -83,6 +83,8 @@ def _load_oracle():
     lib.xo_pcg.argtypes = g + [_dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double,
                                C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.xo_apply_pressure.argtypes = g + [_u8, _dp, _dp, _dp, _dp]
+    lib.xo_project.argtypes = g + [_u8, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double]
+    lib.xo_project_adjoint.argtypes = g + [_u8, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double]
     return lib
 
 

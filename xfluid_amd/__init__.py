@@ -53,6 +53,20 @@ class XfError(RuntimeError):
 _LIB = None
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm ships its own libamdhip64.so with the same soname as the
+    system one.  Two HIP runtimes in one process do not see the same devices
+    (the second one finds no GPU), so when torch is installed it is imported
+    first and the library binds to torch's runtime; torch device tensors can
+    then be passed to the *_device entry points.  XFLUID_NO_TORCH=1 skips this."""
+    if os.environ.get("XFLUID_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libxfluid_hip.so (raises if it was not built)."""
     global _LIB
@@ -60,6 +74,7 @@ def lib():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise XfError(f"{LIB_PATH} is missing: build it with `make -C xfluid_amd` (hipcc, gfx950)")
+    _share_torch_hip_runtime()
     L = C.CDLL(LIB_PATH)
     V = C.c_void_p
     L.xf_last_error.restype = C.c_char_p
@@ -94,6 +109,8 @@ def lib():
     L.xf_sim_frame_count.argtypes = [V]
     L.xf_sim_surface_cells.restype = C.c_int64
     L.xf_sim_surface_cells.argtypes = [V, V, V, C.c_int64]
+    L.xf_sim_project_fields.argtypes = [V, V, V, V, C.c_int]
+    L.xf_sim_project_adjoint.argtypes = [V, V, V, V, C.c_int]
     L.xf_sim_generate_mesh.argtypes = [V, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xf_sim_get_mesh.argtypes = [V, V, V, V]
     L.xf_sim_write_mesh_ply.argtypes = [V, C.c_char_p]
@@ -254,6 +271,28 @@ class FluidSimulator:
         if n:
             _chk(self.L.xf_sim_surface_cells(self.h, cells.ctypes.data, cube.ctypes.data, n), "surface_cells")
         return cells[:n], cube[:n]
+
+    def _faces_call(self, fn, u, v, w, what):
+        """Runs fn on three face arrays in place: numpy float64 (host) or
+        contiguous float64 device tensors (anything with data_ptr() and is_cuda)."""
+        arrs = (u, v, w)
+        if all(getattr(a, "is_cuda", False) for a in arrs):
+            for a in arrs:
+                if not a.is_contiguous() or str(a.dtype) != "torch.float64":
+                    raise ValueError(f"{what}: device arrays must be contiguous float64")
+            return _chk(fn(self.h, arrs[0].data_ptr(), arrs[1].data_ptr(), arrs[2].data_ptr(), 1), what)
+        for a in arrs:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+                raise ValueError(f"{what}: host arrays must be C-contiguous float64 numpy arrays")
+        return _chk(fn(self.h, u.ctypes.data, v.ctypes.data, w.ctypes.data, 0), what)
+
+    def project_fields(self, u, v, w):
+        """u' = u - G A^-1 B u in place on the sim's material; returns the PCG status."""
+        return self._faces_call(self.L.xf_sim_project_fields, u, v, w, "project_fields")
+
+    def project_adjoint(self, gu, gv, gw):
+        """Vector-Jacobian product of project_fields, in place; returns the PCG status."""
+        return self._faces_call(self.L.xf_sim_project_adjoint, gu, gv, gw, "project_adjoint")
 
     def generate_mesh(self, smooth=True):
         """FluidMesher::generateMesh on the current density field -> (verts, tris, normals)."""

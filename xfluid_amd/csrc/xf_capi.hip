@@ -78,6 +78,7 @@ struct xf_sim {
     hipEvent_t ev[XF_STAGE_COUNT + 1]{};
     bool inited = false;
     MeshWork mesh{};
+    double *ga_u = nullptr, *ga_v = nullptr, *ga_w = nullptr, *ga_p = nullptr;  // adjoint scratch
     bool mesh_out = false;
     std::string mesh_prefix;
 
@@ -181,12 +182,17 @@ int st_forces(xf_sim* s, double dt) {  // FluidSimulator.cpp:429-499
     HIPCHK(hipGetLastError());
     return 0;
 }
-int st_solve(xf_sim* s) {
+// Pressure solve of the substep: stencil codes and RHS from the face arrays
+// (fu, fv, fw), then PCG into pw.x.  With rhs_override the RHS is replaced by
+// rhs_override (device, N values) after the stencil codes are built.
+int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* fw, const double* rhs_override) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s->st));
-    launch_stencil_rhs(s->g, s->mat, s->u, s->v, s->w, s->code, s->pw.b, s->st);
+    launch_stencil_rhs(s->g, s->mat, fu, fv, fw, s->code, s->pw.b, s->st);
+    if (rhs_override)
+        HIPCHK(hipMemcpyAsync(s->pw.b, rhs_override, s->g.N * 8, hipMemcpyDeviceToDevice, s->st));
     s->last.reserved = 0;
     const xf_params& p = s->prm;
     SweepTimer& tm = s->pw.timer;
@@ -238,6 +244,7 @@ int st_solve(xf_sim* s) {
     tm.used = 0;
     return r.status;
 }
+int st_solve(xf_sim* s) { return st_solve_from(s, s->u, s->v, s->w, nullptr); }
 int st_apply(xf_sim* s) {
     double* fa[3] = {s->u, s->v, s->w};
     for (int a = 0; a < 3; ++a) launch_apply_pressure(s->g, s->mat, s->pw.x, fa[a], a, s->st);
@@ -399,6 +406,7 @@ void xf_sim_destroy(xf_sim* s) {
     dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q);
     dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q);
     mesh_free(s->mesh);
+    dfree(s->ga_u); dfree(s->ga_v); dfree(s->ga_w); dfree(s->ga_p);
     if (s->pw.timer.ev) {
         for (int i = 0; i < s->pw.timer.cap; ++i) hipEventDestroy(s->pw.timer.ev[i]);
         delete[] s->pw.timer.ev;
@@ -839,6 +847,53 @@ int xf_sim_set_mesh_output(xf_sim* s, const char* prefix) {
     s->mesh_out = prefix != nullptr;
     s->mesh_prefix = prefix ? prefix : "";
     return 0;
+}
+
+// ---- projection and its adjoint on caller arrays ----
+static int copy_faces(xf_sim* s, double* dst[3], double* const src[3], bool to_dev, int device_ptrs) {
+    const long n[3] = {s->nu(), s->nv(), s->nw()};
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice
+                                           : (to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+    for (int a = 0; a < 3; ++a) HIPCHK(hipMemcpyAsync(dst[a], src[a], n[a] * 8, kind, s->st));
+    return 0;
+}
+int xf_sim_project_fields(xf_sim* s, double* u, double* v, double* w, int device_ptrs) {
+    if (check(s) || !u || !v || !w) return set_err(-1, "null argument");
+    double* dev[3] = {s->u, s->v, s->w};
+    double* const io[3] = {u, v, w};
+    int rc = copy_faces(s, dev, io, true, device_ptrs);
+    if (rc < 0) return rc;
+    int status = st_solve(s);
+    if (status < 0) return status;
+    if ((rc = st_apply(s)) < 0) return rc;
+    double* out[3] = {u, v, w};
+    double* const from[3] = {s->u, s->v, s->w};
+    if ((rc = copy_faces(s, out, from, false, device_ptrs)) < 0) return rc;
+    HIPCHK(hipStreamSynchronize(s->st));
+    return status;
+}
+int xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs) {
+    if (check(s) || !gu || !gv || !gw) return set_err(-1, "null argument");
+    if (!s->ga_u) {
+        if (dalloc(&s->ga_u, s->nu()) != hipSuccess || dalloc(&s->ga_v, s->nv()) != hipSuccess ||
+            dalloc(&s->ga_w, s->nw()) != hipSuccess || dalloc(&s->ga_p, s->g.N) != hipSuccess)
+            return set_err(-2, "adjoint allocation failed");
+    }
+    double* dev[3] = {s->ga_u, s->ga_v, s->ga_w};
+    double* const io[3] = {gu, gv, gw};
+    int rc = copy_faces(s, dev, io, true, device_ptrs);
+    if (rc < 0) return rc;
+    // pbar = -G^T g, solve A bbar = pbar, g += B^T bbar
+    launch_apply_adjoint(s->g, s->mat, s->ga_u, s->ga_v, s->ga_w, s->ga_p, s->st);
+    int status = st_solve_from(s, s->ga_u, s->ga_v, s->ga_w, s->ga_p);
+    if (status < 0) return status;
+    for (int a = 0; a < 3; ++a) launch_rhs_adjoint(s->g, s->mat, s->pw.x, dev[a], a, s->st);
+    HIPCHK(hipGetLastError());
+    double* out[3] = {gu, gv, gw};
+    double* const from[3] = {s->ga_u, s->ga_v, s->ga_w};
+    if ((rc = copy_faces(s, out, from, false, device_ptrs)) < 0) return rc;
+    HIPCHK(hipStreamSynchronize(s->st));
+    return status;
 }
 
 int xf_sim_apply_matrix(xf_sim* s, const double* x, double* y) {

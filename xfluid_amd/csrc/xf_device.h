@@ -111,3 +111,8 @@ __device__ inline T block_reduce(T v, T* sh) {
     __syncthreads();
     return v;  // valid in thread 0
 }
+
+// material of cell (i,j,k), `outside` for cells beyond the grid
+__device__ inline uint8_t mat_at(const Geo& g, const uint8_t* mat, int i, int j, int k, uint8_t outside) {
+    return cin(g, i, j, k) ? mat[cidx(g, i, j, k)] : outside;
+}

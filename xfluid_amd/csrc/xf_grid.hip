@@ -19,10 +19,6 @@ void launch_material(const Geo& g, uint8_t* mat, const int32_t* cell_count, hipS
     k_material<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, mat, cell_count);
 }
 
-__device__ inline uint8_t mat_at(const Geo& g, const uint8_t* mat, int i, int j, int k, uint8_t outside) {
-    return cin(g, i, j, k) ? mat[cidx(g, i, j, k)] : outside;
-}
-
 // ---------------------------------------------------------------------------
 // _applyForces (FluidSimulator.cpp:429-499): faces bordering a fluid cell get
 // += a*dt along the axis.

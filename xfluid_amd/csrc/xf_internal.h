@@ -130,3 +130,10 @@ int build_mesh(const Geo& g, const double* phi, int d, double iso, int smooth_it
                int32_t* count_tmp, int32_t* start_tmp, int32_t* scan_tmp, int scan_tmp_n, MeshWork& w,
                hipStream_t st);
 void mesh_free(MeshWork& w);
+
+// ---------------- adjoint kernels (xf_adjoint.hip) ----------------
+// pbar = -G^T g (transpose of the pressure application over all three face arrays)
+void launch_apply_adjoint(const Geo& g, const uint8_t* mat, const double* gu, const double* gv, const double* gw,
+                          double* pbar, hipStream_t st);
+// f += B^T bbar for one face array (transpose of the right-hand-side assembly)
+void launch_rhs_adjoint(const Geo& g, const uint8_t* mat, const double* bbar, double* f, int axis, hipStream_t st);
