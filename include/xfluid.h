@@ -205,6 +205,23 @@ int  xf_sim_project_fields(xf_sim* s, double* u, double* v, double* w, int devic
  * place (A symmetric: the transposed solve is a PCG solve with A). */
 int  xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs);
 
+/* Particle transfers, particles as n x 3 arrays (x,y,z interleaved), on the
+ * sim's current grid fields; grid gradients gu/gv/gw are ADDED to.
+ * VJP of G2P sampling v_p = S(u; p_p) (_getVelocity, FluidSimulator.cpp:
+ * 814-894): gvel -> gu/gv/gw and gpos (added).  saved != 0 samples the saved
+ * velocities (the FLIP "old" field). */
+int  xf_sim_sample_adjoint(xf_sim* s, int64_t n, const double* pos, const double* gvel, double* gpos,
+                           double* gu, double* gv, double* gw, int saved, int device_ptrs);
+/* VJP of the RK4/RK2 position update (FluidSimulator.cpp:896-907) without the
+ * collision branch: gnew (gradient on the new positions) -> gpos (written),
+ * gu/gv/gw (added). */
+int  xf_sim_advect_adjoint(xf_sim* s, int64_t n, const double* pos, double dt, const double* gnew,
+                           double* gpos, double* gu, double* gv, double* gw, int device_ptrs);
+/* VJP of P2G (FluidSimulator.cpp:909-1085) for the given particles: gu/gv/gw
+ * (gradient on the face arrays) -> gvel (written) and gpos (added). */
+int  xf_sim_p2g_adjoint(xf_sim* s, int64_t n, const double* pos, const double* vel, const double* gu,
+                        const double* gv, const double* gw, double* gvel, double* gpos, int device_ptrs);
+
 /* ---- operator diagnostics on the last assembled system (parity hooks):
  *      y = A x (FixedSparseMatrix multiply, sparse_matrix.h:266-276) and
  *      z = (L L^T)^-1 r (PCGSolver::apply_preconditioner, pcg_solver.h:305-309).

@@ -910,6 +910,175 @@ struct Sim {
     }
     void p2g() { p2g_axis(0); p2g_axis(1); p2g_axis(2); }
 
+    // ---- adjoints of the particle transfers (no reference counterpart; the
+    // reference has no gradients).  Particles are passed as n x 3 arrays. ----
+
+    // P2G of one axis for given particles: weight sums wt and values out (as p2g_axis)
+    void p2g_points(int axis, long n, const double* pos, const double* vel, double* out, double* wt) const {
+        const int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
+        const long nf = (long)si * sj * sk;
+        std::vector<double> val(nf, 0.0);
+        std::fill(wt, wt + nf, 0.0);
+        for (long p = 0; p < n; ++p)
+            p2g_visit(axis, pos + 3 * p, [&](long f, double cw, const double*) {
+                wt[f] += cw;
+                val[f] += vel[3 * p + axis] * cw;
+            });
+        for (long f = 0; f < nf; ++f) out[f] = wt[f] > 0.0000001 ? val[f] / wt[f] : 0.0;
+    }
+    // visits the faces of one axis in a particle's support with the weight and
+    // its gradient with respect to the particle position
+    template <class F>
+    void p2g_visit(int axis, const double* pp, F&& fn) const {
+        const int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
+        const bool tri = prm.p2g_kernel == XF_P2G_TRILINEAR;
+        const double r = tri ? dx : 1.5 * dx;
+        const double r2 = r * r;
+        const double c0 = (4.0 / 9.0) * (1.0 / (r2 * r2 * r2));
+        const double c1 = (17.0 / 9.0) * (1.0 / (r2 * r2));
+        const double c2 = (22.0 / 9.0) * (1.0 / (r2));
+        double q[3] = {pp[0] + (axis == 0 ? 0.5 * dx : 0.0), pp[1] + (axis == 1 ? 0.5 * dx : 0.0),
+                       pp[2] + (axis == 2 ? 0.5 * dx : 0.0)};
+        int g0[3], g1[3];
+        for (int a = 0; a < 3; ++a) { g0[a] = pos2idx(-r + q[a], dx); g1[a] = pos2idx(q[a] + r, dx); }
+        for (int i = g0[0]; i <= g1[0]; ++i)
+            for (int j = g0[1]; j <= g1[1]; ++j)
+                for (int k = g0[2]; k <= g1[2]; ++k) {
+                    if (i < 0 || j < 0 || k < 0 || i >= si || j >= sj || k >= sk) continue;
+                    double e[3] = {q[0] - idx2pos(i, dx), q[1] - idx2pos(j, dx), q[2] - idx2pos(k, dx)};
+                    double cw = 0.0, gw[3] = {0.0, 0.0, 0.0};
+                    if (tri) {
+                        double a[3] = {std::fabs(e[0]) / dx, std::fabs(e[1]) / dx, std::fabs(e[2]) / dx};
+                        if (a[0] < 1.0 && a[1] < 1.0 && a[2] < 1.0) {
+                            cw = (1.0 - a[0]) * (1.0 - a[1]) * (1.0 - a[2]);
+                            for (int t = 0; t < 3; ++t) {
+                                double sg = e[t] > 0 ? 1.0 : e[t] < 0 ? -1.0 : 0.0;
+                                gw[t] = -sg / dx * (1.0 - a[(t + 1) % 3]) * (1.0 - a[(t + 2) % 3]);
+                            }
+                        }
+                    } else {
+                        double rs = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                        if (rs < r2) {
+                            cw = 1.0 - c0 * rs * rs * rs + c1 * rs * rs - c2 * rs;
+                            const double dw = -3.0 * c0 * rs * rs + 2.0 * c1 * rs - c2;
+                            for (int t = 0; t < 3; ++t) gw[t] = dw * 2.0 * e[t];
+                        }
+                    }
+                    fn(i + (long)si * (j + (long)sj * k), cw, gw);
+                }
+    }
+    // VJP of P2G: g on the face arrays -> gvel (n x 3, overwritten), gpos (n x 3, accumulated)
+    void p2g_adjoint(long n, const double* pos, const double* vel, const double* const g[3], double* gvel,
+                     double* gpos) const {
+        std::fill(gvel, gvel + 3 * n, 0.0);
+        for (int axis = 0; axis < 3; ++axis) {
+            const long nf = (long)(ni + (axis == 0)) * (nj + (axis == 1)) * (nk + (axis == 2));
+            std::vector<double> uf(nf), wt(nf), a(nf);
+            p2g_points(axis, n, pos, vel, uf.data(), wt.data());
+            for (long f = 0; f < nf; ++f) a[f] = wt[f] > 0.0000001 ? g[axis][f] / wt[f] : 0.0;
+            for (long p = 0; p < n; ++p) {
+                const double vp = vel[3 * p + axis];
+                p2g_visit(axis, pos + 3 * p, [&](long f, double cw, const double* gw) {
+                    if (a[f] == 0.0) return;
+                    gvel[3 * p + axis] += a[f] * cw;
+                    for (int t = 0; t < 3; ++t) gpos[3 * p + t] += a[f] * (vp - uf[f]) * gw[t];
+                });
+            }
+        }
+    }
+
+    // VJP of trilerp (xfluidutil.cpp:53-65): corner weights and d/d(x,y,z)
+    static void trilerp_adj(const double* t, double x, double y, double z, double go, double* gt, double* gx) {
+        const double wx[2] = {1.0 - x, x}, wy[2] = {1.0 - y, y}, wz[2] = {1.0 - z, z};
+        gx[0] = gx[1] = gx[2] = 0.0;
+        for (int c = 0; c < 2; ++c)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) {
+                    const int q = a + 2 * b + 4 * c;
+                    gt[q] = go * wx[a] * wy[b] * wz[c];
+                    gx[0] += t[q] * (a ? 1.0 : -1.0) * wy[b] * wz[c];
+                    gx[1] += t[q] * wx[a] * (b ? 1.0 : -1.0) * wz[c];
+                    gx[2] += t[q] * wx[a] * wy[b] * (c ? 1.0 : -1.0);
+                }
+        gx[0] *= go; gx[1] *= go; gx[2] *= go;
+    }
+    // VJP of get_velocity at p: gU/gV/gW and gp accumulated
+    void sample_adjoint(const double* p, const double* go, const double* const F[3], double* const gF[3],
+                        double* gp) const {
+        if (!((p[0] >= 0.0) && (p[1] >= 0.0) && (p[2] >= 0.0) && (p[0] <= dx * ni) && (p[1] <= dx * nj) &&
+              (p[2] <= dx * nk)))
+            return;
+        const double invdx = 1.0 / dx;
+        for (int comp = 0; comp < 3; ++comp) {
+            double x, y, z;
+            int i, j, k;
+            long si, sj;
+            if (comp == 0) {
+                x = p[0] * invdx; y = (p[1] - 0.5 * dx) * invdx; z = (p[2] - 0.5 * dx) * invdx;
+                i = (int)x;
+                j = y < 0.0 ? 0 : (int)y; j = j > (nj - 2) ? (nj - 2) : j;
+                k = z < 0.0 ? 0 : (int)z; k = k > (nk - 2) ? (nk - 2) : k;
+                si = ni + 1; sj = nj;
+            } else if (comp == 1) {
+                x = (p[0] - 0.5 * dx) * invdx; y = p[1] * invdx; z = (p[2] - 0.5 * dx) * invdx;
+                j = (int)y;
+                i = x < 0.0 ? 0 : (int)x; i = i > (ni - 2) ? (ni - 2) : i;
+                k = z < 0.0 ? 0 : (int)z; k = k > (nk - 2) ? (nk - 2) : k;
+                si = ni; sj = nj + 1;
+            } else {
+                z = p[2] * invdx; y = (p[1] - 0.5 * dx) * invdx; x = (p[0] - 0.5 * dx) * invdx;
+                k = (int)z;
+                j = y < 0.0 ? 0 : (int)y; j = j > (nj - 2) ? (nj - 2) : j;
+                i = x < 0.0 ? 0 : (int)x; i = i > (ni - 2) ? (ni - 2) : i;
+                si = ni; sj = nj;
+            }
+            x -= i; y -= j; z -= k;
+            long idx[8];
+            double t[8], gt[8], gx[3];
+            for (int q = 0; q < 8; ++q) {
+                idx[q] = (i + (q & 1)) + si * ((j + ((q >> 1) & 1)) + sj * (k + (q >> 2)));
+                t[q] = F[comp][idx[q]];
+            }
+            trilerp_adj(t, x, y, z, go[comp], gt, gx);
+            for (int q = 0; q < 8; ++q) gF[comp][idx[q]] += gt[q];
+            for (int a = 0; a < 3; ++a) gp[a] += gx[a] * invdx;
+        }
+    }
+    // VJP of advect_point (RK4 / RK2, collision-free): gnew -> gp (overwritten), gF accumulated
+    void advect_adjoint(const double* p, double dt, const double* gnew, double* const gF[3], double* gp) const {
+        const double* F[3] = {u.data(), v.data(), w.data()};
+        double k1[3], k2[3], k3[3], q2[3], q3[3], q4[3], g[3];
+        for (int a = 0; a < 3; ++a) gp[a] = gnew[a];
+        get_velocity(p, k1, F[0], F[1], F[2]);
+        if (prm.integrator == XF_INTEGRATOR_RK2) {
+            for (int a = 0; a < 3; ++a) q2[a] = p[a] + k1[a] * (0.5 * dt);
+            double gk2[3] = {gnew[0] * dt, gnew[1] * dt, gnew[2] * dt}, gq2[3] = {0, 0, 0};
+            sample_adjoint(q2, gk2, F, gF, gq2);
+            double gk1[3];
+            for (int a = 0; a < 3; ++a) { gp[a] += gq2[a]; gk1[a] = gq2[a] * (0.5 * dt); }
+            sample_adjoint(p, gk1, F, gF, gp);
+            return;
+        }
+        for (int a = 0; a < 3; ++a) q2[a] = p[a] + k1[a] * (0.5 * dt);
+        get_velocity(q2, k2, F[0], F[1], F[2]);
+        for (int a = 0; a < 3; ++a) q3[a] = p[a] + k2[a] * (0.5 * dt);
+        get_velocity(q3, k3, F[0], F[1], F[2]);
+        for (int a = 0; a < 3; ++a) q4[a] = p[a] + k3[a] * dt;
+        double gk1[3], gk2[3], gk3[3], gk4[3];
+        for (int a = 0; a < 3; ++a) {
+            g[a] = gnew[a] * (dt / 6.0);
+            gk1[a] = g[a]; gk2[a] = g[a] * 2.0; gk3[a] = g[a] * 2.0; gk4[a] = g[a];
+        }
+        double gq[3] = {0, 0, 0};
+        sample_adjoint(q4, gk4, F, gF, gq);
+        for (int a = 0; a < 3; ++a) { gp[a] += gq[a]; gk3[a] += gq[a] * dt; gq[a] = 0; }
+        sample_adjoint(q3, gk3, F, gF, gq);
+        for (int a = 0; a < 3; ++a) { gp[a] += gq[a]; gk2[a] += gq[a] * (0.5 * dt); gq[a] = 0; }
+        sample_adjoint(q2, gk2, F, gF, gq);
+        for (int a = 0; a < 3; ++a) { gp[a] += gq[a]; gk1[a] += gq[a] * (0.5 * dt); }
+        sample_adjoint(p, gk1, F, gF, gp);
+    }
+
     int stage(int s, double dt) {
         switch (s) {
             case XF_STAGE_DENSITY: density(); surface(); return 0;
@@ -1099,6 +1268,47 @@ int xo_project_adjoint(int ni, int nj, int nk, const uint8_t* mat, double* gu, d
     int st = pcg(ni, nj, nk, A.data(), pbar.data(), bbar.data(), tol, max_iter, tau, sigma, &it, &res, nullptr);
     rhs_adjoint(ni, nj, nk, mat, bbar.data(), gu, gv, gw);
     return st;
+}
+
+// Particle transfers and their adjoints on explicit particle arrays (n x 3),
+// using the handle's grid fields and parameters.
+void xo_sample_points(void* h, int64_t n, const double* pos, double* out, int saved) {
+    Sim* s = (Sim*)h;
+    const double* F[3] = {saved ? s->us.data() : s->u.data(), saved ? s->vs.data() : s->v.data(),
+                          saved ? s->ws.data() : s->w.data()};
+    for (int64_t p = 0; p < n; ++p) s->get_velocity(pos + 3 * p, out + 3 * p, F[0], F[1], F[2]);
+}
+void xo_sample_adjoint(void* h, int64_t n, const double* pos, const double* gvel, double* gpos, double* gu,
+                       double* gv, double* gw, int saved) {
+    Sim* s = (Sim*)h;
+    const double* F[3] = {saved ? s->us.data() : s->u.data(), saved ? s->vs.data() : s->v.data(),
+                          saved ? s->ws.data() : s->w.data()};
+    double* gF[3] = {gu, gv, gw};
+    for (int64_t p = 0; p < n; ++p) s->sample_adjoint(pos + 3 * p, gvel + 3 * p, F, gF, gpos + 3 * p);
+}
+void xo_advect_points(void* h, int64_t n, const double* pos, double dt, double* out) {
+    Sim* s = (Sim*)h;
+    for (int64_t p = 0; p < n; ++p) s->advect_point(pos + 3 * p, dt, out + 3 * p);
+}
+void xo_advect_adjoint(void* h, int64_t n, const double* pos, double dt, const double* gnew, double* gpos,
+                       double* gu, double* gv, double* gw) {
+    Sim* s = (Sim*)h;
+    double* gF[3] = {gu, gv, gw};
+    for (int64_t p = 0; p < n; ++p) s->advect_adjoint(pos + 3 * p, dt, gnew + 3 * p, gF, gpos + 3 * p);
+}
+void xo_p2g_points(void* h, int64_t n, const double* pos, const double* vel, double* u, double* v, double* w) {
+    Sim* s = (Sim*)h;
+    double* out[3] = {u, v, w};
+    for (int axis = 0; axis < 3; ++axis) {
+        std::vector<double> wt((size_t)(s->ni + (axis == 0)) * (s->nj + (axis == 1)) * (s->nk + (axis == 2)));
+        s->p2g_points(axis, n, pos, vel, out[axis], wt.data());
+    }
+}
+void xo_p2g_adjoint(void* h, int64_t n, const double* pos, const double* vel, const double* gu, const double* gv,
+                    const double* gw, double* gvel, double* gpos) {
+    Sim* s = (Sim*)h;
+    const double* g[3] = {gu, gv, gw};
+    s->p2g_adjoint(n, pos, vel, g, gvel, gpos);
 }
 
 // mesh of the current density field and surface list (stage DENSITY first)

@@ -101,3 +101,115 @@ def test_projection_adjoint_solid_and_air_faces():
     lhs = dot(project(mat, *x), y)
     rhs = dot(x, project_adjoint(mat, *y))
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+# ---------------------------------------------------------------------------
+# particle transfers: G2P sampling, RK advection, P2G
+def particle_scene(integrator=0, kernel=0):
+    p = small_params(integrator=integrator, p2g_kernel=kernel)
+    o = oracle_dam_break(12, 10, 9, 0.1, (0.5, 0.6, 1.0), 7, p)
+    for _ in range(2):
+        o.time_step(1.0 / 60)
+    P, V = o.particles()
+    return o, np.ascontiguousarray(P), np.ascontiguousarray(V)
+
+
+def faces_of(o):
+    return [o.get(FIELD[f]).copy() for f in ("U", "V", "W")]
+
+
+def _fd_check(loss, x0, grad, rng, eps, n_dirs=6, rtol=1e-5):
+    for _ in range(n_dirs):
+        d = rng.standard_normal(x0.shape)
+        fd = (loss(x0 + eps * d) - loss(x0 - eps * d)) / (2 * eps)
+        an = float(np.sum(grad * d))
+        assert abs(fd - an) <= rtol * max(abs(an), 1e-12), (fd, an)
+
+
+def test_sample_adjoint():
+    o, P, _ = particle_scene()
+    L = oracle()
+    n = len(P)
+    rng = np.random.default_rng(21)
+    c = rng.standard_normal((n, 3))
+    F = faces_of(o)
+
+    def loss_pos(pos):
+        out = np.empty((n, 3))
+        L.xo_sample_points(o.h, n, np.ascontiguousarray(pos), out, 0)
+        return float(np.sum(c * out))
+
+    gpos = np.zeros((n, 3))
+    gF = [np.zeros_like(a) for a in F]
+    L.xo_sample_adjoint(o.h, n, P, c, gpos, *gF, 0)
+    _fd_check(loss_pos, P, gpos, rng, 1e-8)
+
+    def loss_u(fields):
+        for name, a in zip(("U", "V", "W"), fields):
+            o.set(FIELD[name], a)
+        return loss_pos(P)
+
+    for a in range(3):  # linear in the grid: dot identity per face array
+        d = rng.standard_normal(F[a].shape)
+        fields = [x.copy() for x in F]
+        fields[a] = F[a] + d
+        fd = loss_u(fields) - loss_u(F)
+        assert abs(fd - float(np.sum(gF[a] * d))) <= 1e-10 * max(abs(fd), 1.0)
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_advect_adjoint(integrator):
+    o, P, _ = particle_scene(integrator=integrator)
+    L = oracle()
+    n = len(P)
+    dt = 1.0 / 60
+    rng = np.random.default_rng(22)
+    c = rng.standard_normal((n, 3))
+
+    def loss_pos(pos):
+        out = np.empty((n, 3))
+        L.xo_advect_points(o.h, n, np.ascontiguousarray(pos), dt, out)
+        return float(np.sum(c * out))
+
+    F = faces_of(o)
+    gpos = np.zeros((n, 3))
+    gF = [np.zeros_like(a) for a in F]
+    L.xo_advect_adjoint(o.h, n, P, dt, c, gpos, *gF)
+    _fd_check(loss_pos, P, gpos, rng, 1e-8)
+
+    def loss_f(x):
+        fields = np.split(x, np.cumsum([a.size for a in F])[:-1])
+        for name, a, ref in zip(("U", "V", "W"), fields, F):
+            o.set(FIELD[name], a.reshape(ref.shape))
+        return loss_pos(P)
+
+    x0 = np.concatenate([a.ravel() for a in F])
+    g0 = np.concatenate([a.ravel() for a in gF])
+    _fd_check(loss_f, x0, g0, rng, 1e-7)
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_p2g_adjoint(kernel):
+    o, P, V = particle_scene(kernel=kernel)
+    L = oracle()
+    n = len(P)
+    rng = np.random.default_rng(23)
+    F = faces_of(o)
+    c = [rng.standard_normal(a.shape) for a in F]
+
+    def loss(pos, vel):
+        out = [np.empty_like(a) for a in F]
+        L.xo_p2g_points(o.h, n, np.ascontiguousarray(pos), np.ascontiguousarray(vel), *out)
+        return sum(float(np.sum(a * b)) for a, b in zip(c, out))
+
+    gvel = np.zeros((n, 3))
+    gpos = np.zeros((n, 3))
+    L.xo_p2g_adjoint(o.h, n, P, V, *c, gvel, gpos)
+    _fd_check(lambda v: loss(P, v), V, gvel, rng, 1e-3, rtol=1e-9)   # linear in the velocities
+    _fd_check(lambda x: loss(x, V), P, gpos, rng, 1e-8)
+    # P2G of the oracle's particles equals the substep's own P2G stage
+    out = [np.empty_like(a) for a in F]
+    L.xo_p2g_points(o.h, n, P, V, *out)
+    o.stage(8)
+    for name, a in zip(("U", "V", "W"), out):
+        assert np.array_equal(a, o.get(FIELD[name]))

@@ -107,3 +107,47 @@ def test_torch_autograd_and_checkpointing(setup):
             fd = (lp - lm) / (2 * eps)
             an = dot(grad, [t.cpu().numpy() for t in d])
             assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+@pytest.mark.parametrize("integrator,kernel", [(0, 0), (1, 1)])
+def test_particle_adjoints_match_oracle(integrator, kernel):
+    """G2P sampling, RK advection and P2G adjoints equal the FP64 CPU adjoints
+    (which are pinned by finite differences) up to the atomic-add summation order."""
+    import xfluid_amd as xf
+    from test_adjoint import faces_of, particle_scene
+    say(f"[adjoint] particle transfers integrator={integrator} kernel={kernel}")
+    o, P, V = particle_scene(integrator=integrator, kernel=kernel)
+    L = oracle()
+    n = len(P)
+    p = xf.default_params(integrator=integrator, p2g_kernel=kernel)
+    s = xf.FluidSimulator(o.ni, o.nj, o.nk, p)
+    s.setCellWidth(o.dx)
+    F = faces_of(o)
+    for name, a in zip(("U", "V", "W"), F):
+        s.set_field(FIELD[name], a)
+    rng = np.random.default_rng(31)
+    c = rng.standard_normal((n, 3))
+    dt = 1.0 / 60
+
+    def zeros():
+        return [np.zeros_like(a) for a in F]
+
+    # sampling
+    go, gg = np.zeros((n, 3)), zeros()
+    L.xo_sample_adjoint(o.h, n, P, c, go, *gg, 0)
+    gd, gdg = np.zeros((n, 3)), zeros()
+    s.sample_adjoint(P, c, gd, *gdg)
+    assert _rel([gd], [go]) <= 1e-12 and _rel(gdg, gg) <= 1e-12
+    # advection
+    go, gg = np.zeros((n, 3)), zeros()
+    L.xo_advect_adjoint(o.h, n, P, dt, c, go, *gg)
+    gd, gdg = np.zeros((n, 3)), zeros()
+    s.advect_adjoint(P, dt, c, gd, *gdg)
+    assert _rel([gd], [go]) <= 1e-12 and _rel(gdg, gg) <= 1e-12
+    # P2G
+    cf = [rng.standard_normal(a.shape) for a in F]
+    gvo, gpo = np.zeros((n, 3)), np.zeros((n, 3))
+    L.xo_p2g_adjoint(o.h, n, P, V, *cf, gvo, gpo)
+    gvd, gpd = np.zeros((n, 3)), np.zeros((n, 3))
+    s.p2g_adjoint(P, V, *cf, gvd, gpd)
+    assert _rel([gvd], [gvo]) <= 1e-12 and _rel([gpd], [gpo]) <= 1e-11

@@ -111,6 +111,9 @@ def lib():
     L.xf_sim_surface_cells.argtypes = [V, V, V, C.c_int64]
     L.xf_sim_project_fields.argtypes = [V, V, V, V, C.c_int]
     L.xf_sim_project_adjoint.argtypes = [V, V, V, V, C.c_int]
+    L.xf_sim_sample_adjoint.argtypes = [V, C.c_int64, V, V, V, V, V, V, C.c_int, C.c_int]
+    L.xf_sim_advect_adjoint.argtypes = [V, C.c_int64, V, C.c_double, V, V, V, V, V, C.c_int]
+    L.xf_sim_p2g_adjoint.argtypes = [V, C.c_int64, V, V, V, V, V, V, V, C.c_int]
     L.xf_sim_generate_mesh.argtypes = [V, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xf_sim_get_mesh.argtypes = [V, V, V, V]
     L.xf_sim_write_mesh_ply.argtypes = [V, C.c_char_p]
@@ -293,6 +296,34 @@ class FluidSimulator:
     def project_adjoint(self, gu, gv, gw):
         """Vector-Jacobian product of project_fields, in place; returns the PCG status."""
         return self._faces_call(self.L.xf_sim_project_adjoint, gu, gv, gw, "project_adjoint")
+
+    @staticmethod
+    def _ptrs(arrays, what):
+        """data pointers of host float64 arrays or device tensors, and the device flag"""
+        if all(getattr(a, "is_cuda", False) for a in arrays):
+            for a in arrays:
+                if not a.is_contiguous() or str(a.dtype) != "torch.float64":
+                    raise ValueError(f"{what}: device arrays must be contiguous float64")
+            return [a.data_ptr() for a in arrays], 1
+        for a in arrays:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+                raise ValueError(f"{what}: host arrays must be C-contiguous float64 numpy arrays")
+        return [a.ctypes.data for a in arrays], 0
+
+    def sample_adjoint(self, pos, gvel, gpos, gu, gv, gw, saved=False):
+        """VJP of G2P sampling; gpos, gu, gv, gw are added to in place."""
+        p, dev = self._ptrs((pos, gvel, gpos, gu, gv, gw), "sample_adjoint")
+        return _chk(self.L.xf_sim_sample_adjoint(self.h, len(pos), *p, int(saved), dev), "sample_adjoint")
+
+    def advect_adjoint(self, pos, dt, gnew, gpos, gu, gv, gw):
+        """VJP of the RK position update; gpos is written, gu, gv, gw are added to."""
+        p, dev = self._ptrs((pos, gnew, gpos, gu, gv, gw), "advect_adjoint")
+        return _chk(self.L.xf_sim_advect_adjoint(self.h, len(pos), p[0], dt, *p[1:], dev), "advect_adjoint")
+
+    def p2g_adjoint(self, pos, vel, gu, gv, gw, gvel, gpos):
+        """VJP of P2G; gvel is written, gpos is added to."""
+        p, dev = self._ptrs((pos, vel, gu, gv, gw, gvel, gpos), "p2g_adjoint")
+        return _chk(self.L.xf_sim_p2g_adjoint(self.h, len(pos), *p, dev), "p2g_adjoint")
 
     def generate_mesh(self, smooth=True):
         """FluidMesher::generateMesh on the current density field -> (verts, tris, normals)."""

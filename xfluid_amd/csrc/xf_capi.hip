@@ -896,6 +896,108 @@ int xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int de
     return status;
 }
 
+// Caller arrays for the particle adjoints: device pointers are used as they
+// are; host arrays are staged through device buffers (copied in when `in`,
+// copied back on sync() when `out`).
+namespace {
+struct Staged {
+    xf_sim* s;
+    double* host;
+    double* dev = nullptr;
+    long n;
+    bool out, owned = false;
+    Staged(xf_sim* s_, const double* h, long n_, bool in, bool out_, int device_ptrs, hipError_t* err)
+        : s(s_), host(const_cast<double*>(h)), n(n_), out(out_) {
+        if (device_ptrs || n == 0) { dev = host; return; }
+        owned = true;
+        if ((*err = hipMalloc((void**)&dev, n * 8)) != hipSuccess) { dev = nullptr; return; }
+        if (in) *err = hipMemcpyAsync(dev, host, n * 8, hipMemcpyHostToDevice, s->st);
+        else *err = hipMemsetAsync(dev, 0, n * 8, s->st);
+    }
+    hipError_t sync() {
+        if (owned && out && dev) return hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, s->st);
+        return hipSuccess;
+    }
+    ~Staged() {
+        if (owned && dev) {
+            (void)hipStreamSynchronize(s->st);
+            (void)hipFree(dev);
+        }
+    }
+};
+}  // namespace
+
+#define STAGE_OR_FAIL(v, ...)                                          \
+    Staged v(s, __VA_ARGS__, device_ptrs, &herr);                      \
+    if (herr != hipSuccess) return set_err(-2, "%s", hipGetErrorString(herr))
+
+int xf_sim_sample_adjoint(xf_sim* s, int64_t n, const double* pos, const double* gvel, double* gpos, double* gu,
+                          double* gv, double* gw, int saved, int device_ptrs) {
+    if (check(s) || n < 0 || (n > 0 && (!pos || !gvel || !gpos)) || !gu || !gv || !gw)
+        return set_err(-1, "null argument");
+    hipError_t herr = hipSuccess;
+    STAGE_OR_FAIL(P, pos, 3 * n, true, false);
+    STAGE_OR_FAIL(GV, gvel, 3 * n, true, false);
+    STAGE_OR_FAIL(GP, gpos, 3 * n, true, true);
+    STAGE_OR_FAIL(GU, gu, s->nu(), true, true);
+    STAGE_OR_FAIL(GVV, gv, s->nv(), true, true);
+    STAGE_OR_FAIL(GW, gw, s->nw(), true, true);
+    const double* F[3] = {saved ? s->us : s->u, saved ? s->vs : s->v, saved ? s->ws : s->w};
+    double* G[3] = {GU.dev, GVV.dev, GW.dev};
+    launch_sample_adjoint(s->g, F, G, n, P.dev, GV.dev, GP.dev, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(GP.sync()); HIPCHK(GU.sync()); HIPCHK(GVV.sync()); HIPCHK(GW.sync());
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_advect_adjoint(xf_sim* s, int64_t n, const double* pos, double dt, const double* gnew, double* gpos,
+                          double* gu, double* gv, double* gw, int device_ptrs) {
+    if (check(s) || n < 0 || (n > 0 && (!pos || !gnew || !gpos)) || !gu || !gv || !gw)
+        return set_err(-1, "null argument");
+    hipError_t herr = hipSuccess;
+    STAGE_OR_FAIL(P, pos, 3 * n, true, false);
+    STAGE_OR_FAIL(GN, gnew, 3 * n, true, false);
+    STAGE_OR_FAIL(GP, gpos, 3 * n, false, true);
+    STAGE_OR_FAIL(GU, gu, s->nu(), true, true);
+    STAGE_OR_FAIL(GVV, gv, s->nv(), true, true);
+    STAGE_OR_FAIL(GW, gw, s->nw(), true, true);
+    const double* F[3] = {s->u, s->v, s->w};
+    double* G[3] = {GU.dev, GVV.dev, GW.dev};
+    launch_advect_adjoint(s->g, F, G, n, P.dev, dt, s->prm.integrator, GN.dev, GP.dev, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(GP.sync()); HIPCHK(GU.sync()); HIPCHK(GVV.sync()); HIPCHK(GW.sync());
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_p2g_adjoint(xf_sim* s, int64_t n, const double* pos, const double* vel, const double* gu,
+                       const double* gv, const double* gw, double* gvel, double* gpos, int device_ptrs) {
+    if (check(s) || n < 0 || (n > 0 && (!pos || !vel || !gvel || !gpos)) || !gu || !gv || !gw)
+        return set_err(-1, "null argument");
+    hipError_t herr = hipSuccess;
+    STAGE_OR_FAIL(P, pos, 3 * n, true, false);
+    STAGE_OR_FAIL(V, vel, 3 * n, true, false);
+    STAGE_OR_FAIL(GVEL, gvel, 3 * n, false, true);
+    STAGE_OR_FAIL(GP, gpos, 3 * n, true, true);
+    STAGE_OR_FAIL(GU, gu, s->nu(), true, false);
+    STAGE_OR_FAIL(GVV, gv, s->nv(), true, false);
+    STAGE_OR_FAIL(GW, gw, s->nw(), true, false);
+    const long mf = s->maxface();
+    double *wt = nullptr, *val = nullptr;
+    if (dalloc(&wt, mf) != hipSuccess || dalloc(&val, mf) != hipSuccess) {
+        dfree(wt); dfree(val);
+        return set_err(-2, "p2g adjoint scratch");
+    }
+    const double* gf[3] = {GU.dev, GVV.dev, GW.dev};
+    launch_p2g_adjoint(s->g, s->prm.p2g_kernel, n, P.dev, V.dev, gf, GVEL.dev, GP.dev, wt, val, s->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = GVEL.sync();
+    if (e == hipSuccess) e = GP.sync();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    dfree(wt); dfree(val);
+    if (e != hipSuccess) return set_err(-2, "p2g adjoint: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int xf_sim_apply_matrix(xf_sim* s, const double* x, double* y) {
     if (check(s) || !x || !y) return set_err(-1, "null argument");
     const long N = s->g.N;

@@ -116,3 +116,57 @@ __device__ inline T block_reduce(T v, T* sh) {
 __device__ inline uint8_t mat_at(const Geo& g, const uint8_t* mat, int i, int j, int k, uint8_t outside) {
     return cin(g, i, j, k) ? mat[cidx(g, i, j, k)] : outside;
 }
+
+// ---------------------------------------------------------------------------
+// _getVelocity (FluidSimulator.cpp:814-894): staggered trilinear sample with
+// the reference's clamps; outside the box the velocity is 0.
+struct Faces {
+    const double *u, *v, *w;
+};
+__device__ inline void sample(const Geo& g, const Faces& F, const double* p, double* out) {
+    const double dx = g.dx;
+    if (!((p[0] >= 0.0) && (p[1] >= 0.0) && (p[2] >= 0.0) && (p[0] <= dx * g.ni) && (p[1] <= dx * g.nj) &&
+          (p[2] <= dx * g.nk))) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double invdx = 1.0 / dx;
+    double t[8], x, y, z;
+    int i, j, k;
+    x = p[0] * invdx; y = (p[1] - 0.5 * dx) * invdx; z = (p[2] - 0.5 * dx) * invdx;
+    i = (int)x;
+    j = y < 0.0 ? 0 : (int)y; j = j > (g.nj - 2) ? (g.nj - 2) : j;
+    k = z < 0.0 ? 0 : (int)z; k = k > (g.nk - 2) ? (g.nk - 2) : k;
+    x -= i; y -= j; z -= k;
+    {
+        const long si = g.ni + 1, sij = si * g.nj;
+        const double* b = F.u + i + si * j + sij * k;
+        t[0] = b[0]; t[1] = b[1]; t[2] = b[si]; t[3] = b[si + 1];
+        t[4] = b[sij]; t[5] = b[sij + 1]; t[6] = b[sij + si]; t[7] = b[sij + si + 1];
+    }
+    out[0] = trilerp8(t, x, y, z);
+    x = (p[0] - 0.5 * dx) * invdx; y = p[1] * invdx; z = (p[2] - 0.5 * dx) * invdx;
+    j = (int)y;
+    i = x < 0.0 ? 0 : (int)x; i = i > (g.ni - 2) ? (g.ni - 2) : i;
+    k = z < 0.0 ? 0 : (int)z; k = k > (g.nk - 2) ? (g.nk - 2) : k;
+    x -= i; y -= j; z -= k;
+    {
+        const long si = g.ni, sij = si * (g.nj + 1);
+        const double* b = F.v + i + si * j + sij * k;
+        t[0] = b[0]; t[1] = b[1]; t[2] = b[si]; t[3] = b[si + 1];
+        t[4] = b[sij]; t[5] = b[sij + 1]; t[6] = b[sij + si]; t[7] = b[sij + si + 1];
+    }
+    out[1] = trilerp8(t, x, y, z);
+    z = p[2] * invdx; y = (p[1] - 0.5 * dx) * invdx; x = (p[0] - 0.5 * dx) * invdx;
+    k = (int)z;
+    j = y < 0.0 ? 0 : (int)y; j = j > (g.nj - 2) ? (g.nj - 2) : j;
+    i = x < 0.0 ? 0 : (int)x; i = i > (g.ni - 2) ? (g.ni - 2) : i;
+    x -= i; y -= j; z -= k;
+    {
+        const long si = g.ni, sij = si * g.nj;
+        const double* b = F.w + i + si * j + sij * k;
+        t[0] = b[0]; t[1] = b[1]; t[2] = b[si]; t[3] = b[si + 1];
+        t[4] = b[sij]; t[5] = b[sij + 1]; t[6] = b[sij + si]; t[7] = b[sij + si + 1];
+    }
+    out[2] = trilerp8(t, x, y, z);
+}

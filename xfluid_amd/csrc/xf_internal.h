@@ -137,3 +137,11 @@ void launch_apply_adjoint(const Geo& g, const uint8_t* mat, const double* gu, co
                           double* pbar, hipStream_t st);
 // f += B^T bbar for one face array (transpose of the right-hand-side assembly)
 void launch_rhs_adjoint(const Geo& g, const uint8_t* mat, const double* bbar, double* f, int axis, hipStream_t st);
+// particle transfers (particles as n x 3 arrays); grid gradients accumulate
+void launch_sample_adjoint(const Geo& g, const double* const F[3], double* const G[3], long n, const double* pos,
+                           const double* gvel, double* gpos, hipStream_t st);
+void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const G[3], long n, const double* pos,
+                           double dt, int integrator, const double* gnew, double* gpos, hipStream_t st);
+void launch_p2g_adjoint(const Geo& g, int kernel, long n, const double* pos, const double* vel,
+                        const double* const gf[3], double* gvel, double* gpos, double* s_wt, double* s_val,
+                        hipStream_t st);
