@@ -267,7 +267,7 @@ def test_frame_step_runs(xf):
     _close(Pg, Po, 1e-8, "positions after one frame")
 
 
-@pytest.mark.parametrize("case,kind", [("odd", 2), ("c1_quarter", 3)] + [("tiles", k) for k in (2, 3, 4, 5)])
+@pytest.mark.parametrize("case,kind", [("odd", 2), ("c1_quarter", 3)] + [("tiles", k) for k in (2, 3, 4, 5, 6, 7)] + [("odd", 6), ("c1_quarter", 7)])
 def test_pipelined_sweeps_bitexact(xf, case, kind):
     """The pipelined line-tile sweeps give the same bits as the serial sweeps,
     and a whole solve with them equals the per-hyperplane solve exactly."""

@@ -203,7 +203,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
         tm.ev = new hipEvent_t[tm.cap];
         for (int i = 0; i < tm.cap; ++i) HIPCHK(hipEventCreate(&tm.ev[i]));
     }
-    int sweep = p.sweep_kernel == 0 ? 3 : p.sweep_kernel;
+    int sweep = p.sweep_kernel == 0 ? 6 : p.sweep_kernel;
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
     if (sweep >= 2) {
@@ -358,8 +358,9 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
         SweepSkew<double>& a = s->pw.sk64;
         SweepSkew<float>& b = s->pw.sk32;
         if (dalloc(&a.code, M) != hipSuccess || dalloc(&a.inv, M) != hipSuccess || dalloc(&a.r, M) != hipSuccess ||
-            dalloc(&a.q, M) != hipSuccess || dalloc(&b.inv, M) != hipSuccess || dalloc(&b.r, M) != hipSuccess ||
-            dalloc(&b.q, M) != hipSuccess)
+            dalloc(&a.q, M) != hipSuccess || dalloc(&a.z, M) != hipSuccess || dalloc(&b.inv, M) != hipSuccess ||
+            dalloc(&b.r, M) != hipSuccess || dalloc(&b.q, M) != hipSuccess || dalloc(&b.z, M) != hipSuccess ||
+            dalloc(&a.eb, M / 4 + 64) != hipSuccess || dalloc(&b.eb, M / 4 + 64) != hipSuccess)
             return fail(set_err(-2, "alloc sweep buffers"));
         b.code = a.code;
     }
@@ -403,8 +404,9 @@ void xf_sim_destroy(xf_sim* s) {
                       &s->P.key, &s->P.perm, &s->surf_cnt, &s->surf_start};
     for (auto* p : is) dfree(*p);
     dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
-    dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q);
-    dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q);
+    dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q); dfree(s->pw.sk64.z);
+    dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q); dfree(s->pw.sk32.z);
+    dfree(s->pw.sk64.eb); dfree(s->pw.sk32.eb);
     mesh_free(s->mesh);
     dfree(s->ga_u); dfree(s->ga_v); dfree(s->ga_w); dfree(s->ga_p);
     if (s->pw.timer.ev) {

@@ -30,7 +30,8 @@ void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* mar
 template <class T>
 struct SweepSkew {
     uint16_t* code;
-    T *inv, *r, *q;
+    T *inv, *r, *q, *z;
+    T* eb;  // tile-edge hand-off buffer of the memory-wave schedule
 };
 // Optional device timing of the sweep launches (pairs of events around the
 // forward + backward launches of each preconditioner application).
