@@ -267,7 +267,8 @@ def test_frame_step_runs(xf):
     _close(Pg, Po, 1e-8, "positions after one frame")
 
 
-@pytest.mark.parametrize("case,kind", [("odd", 2), ("c1_quarter", 3)] + [("tiles", k) for k in (2, 3, 4, 5, 6, 7)] + [("odd", 6), ("c1_quarter", 7)])
+@pytest.mark.parametrize("case,kind", [("odd", 2), ("c1_quarter", 3)] + [("tiles", k) for k in (2, 3, 4, 5, 6, 7, 8, 9)]
+                         + [("odd", 6), ("c1_quarter", 7), ("odd", 8), ("c1_quarter", 8), ("odd", 9), ("c1_quarter", 9)])
 def test_pipelined_sweeps_bitexact(xf, case, kind):
     """The pipelined line-tile sweeps give the same bits as the serial sweeps,
     and a whole solve with them equals the per-hyperplane solve exactly."""
@@ -305,3 +306,29 @@ def test_pipelined_sweeps_bitexact(xf, case, kind):
     assert st.reserved == 0, "pipelined wait timed out"
     assert st.iterations == it1
     _exact(g.get_field(FIELD["PRESSURE"]), p1, "pressure, pipelined vs per-plane sweeps")
+
+
+@pytest.mark.parametrize("case,kind", [("tiles", 6), ("tiles", 8), ("odd", 8), ("tiles", 9), ("odd", 9)])
+def test_pipelined_sweeps_fp32_bitexact(xf, case, kind):
+    """FP32 sweeps (mixed-precision solve): a pipelined schedule gives the same
+    bits as the per-hyperplane FP32 sweeps, so the whole mixed solve (iteration
+    counts, outer refinements, pressure) is identical."""
+    ni, nj, nk, dx, frac, seed = CASES[case]
+    say(f"[pipelined fp32] case={case} schedule={kind}")
+    o = oracle_dam_break(ni, nj, nk, dx, frac, seed, small_params())
+    o.time_step(1.0 / 60)
+    o.stage(STAGE["MATERIAL"]); o.stage(STAGE["SAVE"]); o.stage(STAGE["FORCES"], 1.0 / 60)
+    g = gpu_like(o)
+    res = []
+    for k in (1, kind):
+        copy_state(o, g)
+        p = g.params
+        p.precision = xf.PREC_MIXED
+        p.sweep_kernel = k
+        g.set_params(p)
+        assert g.stage(STAGE["SOLVE"]) == 0
+        st = g.last_solve()
+        assert st.reserved == 0, "pipelined wait timed out"
+        res.append((st.iterations, st.outer, g.get_field(FIELD["PRESSURE"])))
+    assert res[0][:2] == res[1][:2]
+    _exact(res[1][2], res[0][2], "mixed pressure, pipelined vs per-plane FP32 sweeps")

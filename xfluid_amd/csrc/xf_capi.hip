@@ -363,6 +363,24 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
             dalloc(&a.eb, M / 4 + 64) != hipSuccess || dalloc(&b.eb, M / 4 + 64) != hipSuccess)
             return fail(set_err(-2, "alloc sweep buffers"));
         b.code = a.code;
+        // single-wave schedule buffers
+        const long W = sweep_wave_size(s->g), E = sweep_wave_edge_words(s->g);
+        if (dalloc(&a.wL, 7 * W) != hipSuccess || dalloc(&a.wm, 6 * W) != hipSuccess || dalloc(&a.wr, W) != hipSuccess ||
+            dalloc(&a.wq, W) != hipSuccess || dalloc(&a.wz, W) != hipSuccess || dalloc(&a.wedge, E) != hipSuccess ||
+            dalloc(&b.wL, 7 * W) != hipSuccess || dalloc(&b.wm, 6 * W) != hipSuccess || dalloc(&b.wr, W) != hipSuccess ||
+            dalloc(&b.wq, W) != hipSuccess || dalloc(&b.wz, W) != hipSuccess || dalloc(&b.wedge, E) != hipSuccess)
+            return fail(set_err(-2, "alloc sweep buffers"));
+        const long W8 = sweep_w8_size(s->g), E8 = sweep_w8_edge_words(s->g);
+        if (dalloc(&a.oRec, 8 * W8) != hipSuccess || dalloc(&a.oR, W8) != hipSuccess || dalloc(&a.oQ, W8) != hipSuccess ||
+            dalloc(&a.oZ, W8) != hipSuccess || dalloc(&a.oEdge, E8) != hipSuccess || dalloc(&b.oRec, 8 * W8) != hipSuccess ||
+            dalloc(&b.oR, W8) != hipSuccess || dalloc(&b.oQ, W8) != hipSuccess || dalloc(&b.oZ, W8) != hipSuccess ||
+            dalloc(&b.oEdge, E8) != hipSuccess)
+            return fail(set_err(-2, "alloc sweep buffers"));
+        hipMemsetAsync(a.oEdge, 0, E8 * sizeof(uint32_t), s->st);
+        hipMemsetAsync(b.oEdge, 0, E8 * sizeof(uint32_t), s->st);
+        // tags are never zero: zeroed edge words read as not yet written
+        hipMemsetAsync(a.wedge, 0, E * sizeof(uint32_t), s->st);
+        hipMemsetAsync(b.wedge, 0, E * sizeof(uint32_t), s->st);
     }
     hipMemsetAsync(s->pw.sweep_flags, 0, s->pw.nflags * sizeof(int), s->st);
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
@@ -407,6 +425,11 @@ void xf_sim_destroy(xf_sim* s) {
     dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q); dfree(s->pw.sk64.z);
     dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q); dfree(s->pw.sk32.z);
     dfree(s->pw.sk64.eb); dfree(s->pw.sk32.eb);
+    dfree(s->pw.sk64.wL); dfree(s->pw.sk64.wm); dfree(s->pw.sk64.wr); dfree(s->pw.sk64.wq); dfree(s->pw.sk64.wz);
+    dfree(s->pw.sk64.wedge); dfree(s->pw.sk32.wL); dfree(s->pw.sk32.wm); dfree(s->pw.sk32.wr); dfree(s->pw.sk32.wq);
+    dfree(s->pw.sk32.wz); dfree(s->pw.sk32.wedge);
+    dfree(s->pw.sk64.oRec); dfree(s->pw.sk64.oR); dfree(s->pw.sk64.oQ); dfree(s->pw.sk64.oZ); dfree(s->pw.sk64.oEdge);
+    dfree(s->pw.sk32.oRec); dfree(s->pw.sk32.oR); dfree(s->pw.sk32.oQ); dfree(s->pw.sk32.oZ); dfree(s->pw.sk32.oEdge);
     mesh_free(s->mesh);
     dfree(s->ga_u); dfree(s->ga_v); dfree(s->ga_w); dfree(s->ga_p);
     if (s->pw.timer.ev) {

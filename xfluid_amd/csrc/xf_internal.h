@@ -32,6 +32,18 @@ struct SweepSkew {
     uint16_t* code;
     T *inv, *r, *q, *z;
     T* eb;  // tile-edge hand-off buffer of the memory-wave schedule
+    // single-wave schedule (xf_sweep_wave.hip), quad tile layout: 7 coefficient
+    // planes, 6 presence-mask planes, input / forward / backward vectors, and
+    // the tagged edge words
+    T* wL;
+    uint32_t* wm;
+    T *wr, *wq, *wz;
+    uint32_t* wedge;
+    // 8x8-tile schedule (xf_sweep_wave.hip), oct layout: forward and backward
+    // records, input / forward / backward vectors, tagged edge words
+    T* oRec;
+    T *oR, *oQ, *oZ;
+    uint32_t* oEdge;
 };
 // Optional device timing of the sweep launches (pairs of events around the
 // forward + backward launches of each preconditioner application).
@@ -76,6 +88,21 @@ long sweep_skewed_size(const Geo& g);
 // skewed copies of the stencil codes (if code != null) and pivots
 template <class T>
 void sweep_prepare(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
+// single-wave schedule: elements per quad-layout plane, edge-buffer words
+long sweep_wave_size(const Geo& g);
+long sweep_wave_edge_words(const Geo& g);
+template <class T>
+void sweep_prepare_wave(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
+template <class T>
+bool sweep_precond_wave(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
+                        unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
+long sweep_w8_size(const Geo& g);
+long sweep_w8_edge_words(const Geo& g);
+template <class T>
+void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
+template <class T>
+bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
+                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
 template <class T>
 bool sweep_precond_pipelined(const Geo& g, int kind, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S,
                              int* flags, unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);

@@ -152,7 +152,10 @@ static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
     if (sweep_kernel >= 2 && sk) {
         hipEvent_t a = nullptr, b = nullptr;
         if (tm && tm->on && tm->used + 2 <= tm->cap) { a = tm->ev[tm->used]; b = tm->ev[tm->used + 1]; }
-        if (sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st)) {
+        const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
+                         : sweep_kernel >= 8 ? sweep_precond_wave<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
+                                           : sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st);
+        if (ran) {
             if (a) tm->used += 2;
             return;
         }
@@ -174,7 +177,9 @@ static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
 
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
                  int sweep_kernel, PcgWork& w, hipStream_t st) {
-    if (sweep_kernel >= 2) sweep_prepare<double>(g, code, inv, w.sk64, st);
+    if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, inv, w.sk64, st);
+    else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, inv, w.sk64, st);
+    else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, inv, w.sk64, st);
     precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, nullptr, st);
 }
 
@@ -398,7 +403,9 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (tol_factor < 1e-30) tol_factor = 1e-30;
     PcgResult res{};
     pcg_mic0(g, code, w.inv, tau, sigma, st);
-    if (sweep_kernel >= 2) sweep_prepare<double>(g, code, w.inv, w.sk64, st);
+    if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
+    else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, w.inv, w.sk64, st);
+    else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, w.inv, w.sk64, st);
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
@@ -417,7 +424,9 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     PcgScalars* S_outer = w.S + 1;
     PcgScalars* S_outer_h = w.S_host + 1;
     k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.inv32, w.inv, nullptr);
-    if (sweep_kernel >= 2) sweep_prepare<float>(g, nullptr, w.inv32, w.sk32, st);
+    if (sweep_kernel == 9) sweep_prepare_w8<float>(g, code, w.inv32, w.sk32, st);
+    else if (sweep_kernel >= 8) sweep_prepare_wave<float>(g, code, w.inv32, w.sk32, st);
+    else if (sweep_kernel >= 2) sweep_prepare<float>(g, nullptr, w.inv32, w.sk32, st);
     k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
     k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
     k_absmax<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.partial);
