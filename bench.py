@@ -126,7 +126,7 @@ def main():
         log(f"[bench] fp64: {fp64['elapsed']:.3f} s, solve {fp64['solve_ms']:.1f} ms, iters {fp64['iters']}")
     K = args.steps
     value = K / mixed["elapsed"]
-    # Roofline of the dominant kernel, the pipelined MIC(0) sweep (k_sweep_pipe):
+    # Roofline of the dominant kernel, the pipelined MIC(0) sweep (k_sweep_w8):
     # algorithmic bytes per launch over the N-cell grid = read input + pivots +
     # 2-byte stencil code, write output: FP32 (mixed run) 4+4+2+4 = 14 B/cell,
     # FP64 8+8+2+8 = 26 B/cell.  Average launch duration from device events
@@ -158,7 +158,7 @@ def main():
         "worst_rel_residual_mixed": mixed["worst_rel_res"],
         "particles_per_s": round(mixed["particles"] * value, 1),
         "stage_ms_per_step_mixed": [round(x / K, 3) for x in mixed["stage_ms"]],
-        "roofline": {"bound": "hbm", "kernel": "k_sweep_pipe (MIC(0) sweep, FP32 in the mixed solve)",
+        "roofline": {"bound": "hbm", "kernel": "k_sweep_w8 (MIC(0) sweep, FP32 in the mixed solve)",
                      "achieved": round(ach_sweep, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_sweep / HBM_PEAK_GBS, 5), "traffic": None,
                      "bytes_per_launch": 14 * N, "avg_launch_us": round(sweep_us, 2),

@@ -54,9 +54,10 @@ typedef struct xf_params {
     int    p2g_kernel;        /* XF_P2G_WYVILL (reference) or XF_P2G_TRILINEAR      */
     int    tail_skip;         /* 1: the last n%6 particles are not advected (reference chunking) */
     int    density_every_substep; /* 1: density each substep (reference); 0: only when a mesh is output */
-    int    sweep_kernel;      /* MIC(0) sweep schedule: 0 auto (= 6), 1 per-hyperplane launches,
-                                 2..5 pipelined line tiles, chunks of 8, 4, 2, 1 steps,
-                                 6, 7 pipelined line tiles with load/store waves              */
+    int    sweep_kernel;      /* MIC(0) sweep schedule: 0 auto (= 9), 1 per-hyperplane launches,
+                                 2..5 pipelined 16x16 line tiles, chunks of 8, 4, 2, 1 steps,
+                                 6, 7 pipelined 16x16 line tiles with load/store waves,
+                                 8 one wavefront per 16x16 tile, 9 one wavefront per 8x8 tile */
     double mixed_inner_tol;   /* mixed mode: relative reduction asked of each FP32 inner solve */
     int    mixed_max_outer;   /* mixed mode: cap on FP64 refinement steps           */
     int    reserved[5];

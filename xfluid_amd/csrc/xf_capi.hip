@@ -203,7 +203,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
         tm.ev = new hipEvent_t[tm.cap];
         for (int i = 0; i < tm.cap; ++i) HIPCHK(hipEventCreate(&tm.ev[i]));
     }
-    int sweep = p.sweep_kernel == 0 ? 6 : p.sweep_kernel;
+    int sweep = p.sweep_kernel == 0 ? 9 : p.sweep_kernel;
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
     if (sweep >= 2) {
