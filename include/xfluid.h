@@ -58,7 +58,8 @@ typedef struct xf_params {
                                  2..5 pipelined 16x16 line tiles, chunks of 8, 4, 2, 1 steps,
                                  6, 7 pipelined 16x16 line tiles with load/store waves,
                                  8 one wavefront per 16x16 tile, 9 one wavefront per 8x8 tile */
-    double mixed_inner_tol;   /* mixed mode: relative reduction asked of each FP32 inner solve */
+    double mixed_inner_tol;   /* mixed mode: relative reduction asked of an FP32 inner solve; later
+                                 inner solves stop at half of what is left to the outer tolerance */
     int    mixed_max_outer;   /* mixed mode: cap on FP64 refinement steps           */
     int    reserved[5];
 } xf_params;

@@ -441,9 +441,14 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (S_outer_h->res0 == 0) { res.iterations = 0; res.outer = 0; return res; }
     res.status = 1;
     while (outer < max_outer && total < max_iter * max_outer) {
+        // The inner solve reduces its residual by inner_tol, or only by what is
+        // still missing to the outer tolerance (with a factor 2 of margin for the
+        // FP32 recurrence drifting from the true residual), whichever is less.
+        const double need = 0.5 * tol / res.residual;
+        const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
-        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.z32, w.q32, w.partial, w.S, w.S_host, inner_tol,
+        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.z32, w.q32, w.partial, w.S, w.S_host, itol,
                         max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
         total += w.S_host->iters;
         ++outer;

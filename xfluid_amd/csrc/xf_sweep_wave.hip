@@ -528,9 +528,15 @@ template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, con
 // by kl, 8-15 the edge row by jl, whose corner cell is not repeated there
 // (readers take it from the column word).
 #define W8_PAD 14
+#ifndef W8_R
 #define W8_R 16     // LDS ring depth in steps
+#endif
+#ifndef W8_LEAD
 #define W8_LEAD 12  // steps of record loads in flight
-#define W8_E 4      // steps of edge loads in flight
+#endif
+#ifndef W8_E
+#define W8_E 1      // steps of edge loads in flight (each step of lead adds to every tile-to-tile hop)
+#endif
 
 static inline void w8_tiles(const Geo& g, int* ntj, int* ntk) {
     *ntj = (g.nj + 7) / 8;
