@@ -96,6 +96,19 @@ def cpu_baseline(cfg, dt):
                        f"density and G2P), {el:.1f} s")
 
 
+def copy_calibration(mib):
+    """Plain device-to-device copies of a known size: the reference point for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +118,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--sweep", type=int, default=0)
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the substep on one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md for how the path would shard).", file=sys.stderr)

@@ -82,6 +82,16 @@ __host__ __device__ inline double code_diag(uint16_t c) {
     return s;
 }
 
+// Workgroups go to the 8 XCDs of the MI355X round-robin (block b to XCD b % 8),
+// and each XCD has its own L2.  For gathers whose neighbouring blocks read the
+// same data, give each XCD a contiguous run of logical blocks instead of every
+// 8th one (a bijection on [0, nb): the last nb % 8 blocks keep their index).
+__device__ inline long xcd_block(long b, long nb) {
+    const long full = nb & ~7l;
+    if (b >= full) return b;
+    return (b & 7) * (full >> 3) + (b >> 3);
+}
+
 template <class T>
 __device__ inline T warp_sum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

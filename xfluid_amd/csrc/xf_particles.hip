@@ -211,7 +211,7 @@ __device__ bool ray_box(const double* p0, const double* din, const double* bmin,
 __global__ void k_g2p_advect(Geo g, long n, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                              const int32_t* pid, const uint8_t* mat, Faces Fn, Faces Fo, double dt, double flip,
                              int integrator, long limit_id) {
-    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     if (pid[p] >= limit_id) return;
     double pos[3] = {px[p], py[p], pz[p]};
@@ -281,7 +281,7 @@ __global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, 
                       const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
                       double* __restrict__ out) {
     const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     long nf = (long)si * sj * sk;
     if (t >= nf) return;
     int fi = (int)(t % si), fj = (int)((t / si) % sj), fk = (int)(t / ((long)si * sj));
@@ -348,7 +348,7 @@ __global__ void k_density(Geo g, const double* __restrict__ px, const double* __
                           const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi,
                           double R) {
     const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n1) return;
     const int ci = (int)(t % (g.ni + 1)), cj = (int)((t / (g.ni + 1)) % (g.nj + 1)),
               ck = (int)(t / ((long)(g.ni + 1) * (g.nj + 1)));
