@@ -78,6 +78,18 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0):
                 stage_ms=stage_ms, particles=n, sweep_ms=sweep_ms, sweeps=sweeps)
 
 
+def measured_traffic(cfg):
+    """HBM bytes per sweep launch from the committed counter summary (FETCH_SIZE
+    and WRITE_SIZE of the 128^3 run, scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "sweep_traffic_dam128.md")
+    if cfg != "dam128" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("sweep_traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(cfg, dt):
     """The oracle (FP64 C++ restatement, reference-faithful threading) on one substep."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -178,7 +190,8 @@ def main():
         "stage_ms_per_step_mixed": [round(x / K, 3) for x in mixed["stage_ms"]],
         "roofline": {"bound": "hbm", "kernel": "k_sweep_w8 (MIC(0) sweep, FP32 in the mixed solve)",
                      "achieved": round(ach_sweep, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_sweep / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(ach_sweep / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(args.config) if args.sweep in (0, 9) else None,
                      "bytes_per_launch": 14 * N, "avg_launch_us": round(sweep_us, 2),
                      "launches": mixed["sweeps"]},
         "pcg_mixed_achieved_GBs": round(ach_m, 2),

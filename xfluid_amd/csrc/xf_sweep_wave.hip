@@ -516,8 +516,11 @@ template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, con
 // bound by its bytes rather than its arithmetic.  Tiles of 8x8 lines (one cell
 // per lane) put four times as many CUs on the same grid with a quarter of the
 // bytes per step each.  A step's records arrive W8_LEAD steps ahead of the
-// compute wave, the upstream edge words (usually written just before they are
-// needed) W8_E steps ahead, each through a load wave of its own.
+// compute wave through a record wave; an edge wave follows the upstream edge
+// words one step ahead (every step of lead would be paid again on each
+// tile-to-tile hop).  The two are separate waves because a wave's vector-memory
+// counter completes in issue order: waiting for an edge would also wait for
+// every record load issued before it.
 //
 // Oct layout: element (slot, lane) of tile t = J*ntk + K at (t*SL + slot)*64 +
 // lane, lane = jl + 8*kl, SL = ni + 14; cell (i, 8J + jl, 8K + kl) with
@@ -526,7 +529,9 @@ template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, con
 // and a present one as A_cm * inv (never positive: its sign bit marks the
 // entry present).  Edge rows: 16 words per tile and step, 0-7 the edge column
 // by kl, 8-15 the edge row by jl, whose corner cell is not repeated there
-// (readers take it from the column word).
+// (readers take it from the column word).  An edge word's tag is the sweep's
+// tag base with the producing step in its low 8 bits, so a ring slot's copy of
+// an older step never passes for the current one.
 #define W8_PAD 14
 #ifndef W8_R
 #define W8_R 16     // LDS ring depth in steps
@@ -534,9 +539,7 @@ template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, con
 #ifndef W8_LEAD
 #define W8_LEAD 12  // steps of record loads in flight
 #endif
-#ifndef W8_E
-#define W8_E 1      // steps of edge loads in flight (each step of lead adds to every tile-to-tile hop)
-#endif
+
 
 static inline void w8_tiles(const Geo& g, int* ntj, int* ntk) {
     *ntj = (g.nj + 7) / 8;
@@ -653,7 +656,7 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
 
 template <class T, bool FWD>
 __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Planes<T> P, const PcgScalars* S,
-                                                  uint32_t tag, int* err) {
+                                                  uint32_t tagbase, int* err) {
     constexpr int NS = sizeof(T) / 4;
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
@@ -664,7 +667,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
     constexpr int PERD = RECB / 1024 + 1;        // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
     __shared__ __attribute__((aligned(16))) unsigned char ring_[W8_R * SLOT];
-    __shared__ int ctr_[3];  // [0] steps whose records are in, [1] steps consumed, [2] steps whose edges are in
+    __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
     if (S && S->done) return;
     int J, K;
     wv_tile(blockIdx.x, ntj, ntk, FWD, &J, &K);
@@ -677,93 +680,90 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
     lds_u8* ring = (lds_u8*)ring_;
     lds_int* ctr = (lds_int*)ctr_;
     const uint32_t ctrA = (uint32_t)(uintptr_t)ctr, ringA = (uint32_t)(uintptr_t)ring;
-    if (threadIdx.x < 3) ctr[threadIdx.x] = 0;
+    // tag of the word an upstream tile stored at its step u
+    auto tag_of = [tagbase](int u) __attribute__((always_inline)) { return tagbase | (uint32_t)(u & 0xff); };
+    if (threadIdx.x < 2) ctr[threadIdx.x] = 0;
+    // the edge words of every slot start out invalid (tags are never zero)
+    for (int b = threadIdx.x * 4; b < W8_R * 2 * ROWB; b += 192 * 4)
+        *(__attribute__((address_space(3))) uint32_t*)(ring + (b / (2 * ROWB)) * SLOT + EOFF + b % (2 * ROWB)) = 0u;
     __syncthreads();
 
-    if (wid > 0) {
-        // ------------------------ load waves (records, edges) ------------------------
-        // Two waves, because a wave's vector-memory counter completes in issue
-        // order: waiting for an edge load would also wait for every record load
-        // issued before it.
+    if (wid == 1) {
+        // ------------------------------- record wave -------------------------------
+        // records and input of step x + LEAD issued at iteration x (LDS-DMA into
+        // the ring), step x waited for and published then
         int cons_seen = 0;
-        // per-lane edge sources (plain values: the lambdas below select nothing by reference)
-        const bool ijs = lane < ROWB / 16, vjs = lane < 8;          // copy lane / check lane on the j side
-        const bool iHas = ijs ? hasJ : hasK, vHas = vjs ? hasJ : hasK;
-        const long iT = ijs ? tJ : tK, vT = vjs ? tJ : tK;
-        const int iLo = (ijs ? lane : lane - ROWB / 16) * 16;
-        const int vW = vjs ? lane : w8_kword<FWD>(lane - 8);
-        const int vOff = EOFF + (vjs ? 0 : ROWB) + vW * EWB;
-        auto issue_data = [&](int s) __attribute__((always_inline)) {
-            w8_wait_slot(s, cons_seen, ctrA + 4, err);
-            const long e0 = (myTile * SL + (FWD ? s : SL - 1 - s)) * 64;
-            lds_u8* dst = ring + (s % W8_R) * SLOT;
+        for (int x = -W8_LEAD; x < SL; ++x) {
+            const int s = x + W8_LEAD;
+            if (s < SL) {
+                w8_wait_slot(s, cons_seen, ctrA + 4, err);
+                const long e0 = (myTile * SL + (FWD ? s : SL - 1 - s)) * 64;
+                lds_u8* dst = ring + (s % W8_R) * SLOT;
 #pragma unroll
-            for (int h = 0; h < RECB / 1024; ++h)
-                __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + 4 * e0) + h * 1024 + lane * 16,
-                                                 dst + h * 1024, 16, 0, 0);
-            if (lane < INB / 16)
-                __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + e0) + lane * 16, dst + RECB, 16, 0, 0);
-        };
-        auto issue_edge = [&](int s) __attribute__((always_inline)) {
-            w8_wait_slot(s, cons_seen, ctrA + 4, err);
-            // both upstream rows of step s + 7 (an own-row address where none is
-            // needed keeps the instruction count fixed; those words are not read)
-            if (lane < 2 * ROWB / 16) {
-                const int us = s + 7;
-                const bool need = iHas && us < SL;
-                const long urow = need ? iT * SL + us : myTile * SL + s;
-                __builtin_amdgcn_global_load_lds((const unsigned char*)P.edge + urow * ROWB + iLo,
-                                                 ring + (s % W8_R) * SLOT + EOFF, 16, 0, WV_SC1);
+                for (int h = 0; h < RECB / 1024; ++h)
+                    __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + 4 * e0) + h * 1024 + lane * 16,
+                                                     dst + h * 1024, 16, 0, 0);
+                if (lane < INB / 16)
+                    __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + e0) + lane * 16, dst + RECB, 16, 0, 0);
             }
+            if (x < 0) continue;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (x + W8_LEAD < SL) vm_wait<W8_LEAD * PERD>();
+            else vm_wait<0>();
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            lds_st32(ctrA, x + 1);
+        }
+        return;
+    }
+    if (wid == 2) {
+        // -------------------------------- edge wave --------------------------------
+        // Lanes 0-15 each follow one upstream edge word (0-7: the j-upstream's
+        // column word kl, 8-15: the k-upstream's row word for jl): the word of
+        // step x + 1 is loaded while step x is checked; a stale word (older tag)
+        // is re-read until it carries the tag of its step, then copied into the
+        // ring slot, where the compute wave checks the tag once more.
+        const bool mine = lane < 16, sideJ = lane < 8;
+        const bool has = mine && (sideJ ? hasJ : hasK);
+        const long ut = has ? (sideJ ? tJ : tK) : myTile;
+        const int w = mine ? (sideJ ? lane : w8_kword<FWD>(lane - 8)) : 0;
+        const int off = EOFF + (sideJ ? 0 : ROWB) + w * EWB;
+        auto src = [&](int x) __attribute__((always_inline)) {
+            const int us = x + 7 < SL ? x + 7 : SL - 1;
+            return (const uint64_t*)((const unsigned char*)P.edge + (ut * SL + us) * ROWB + w * EWB);
         };
-        auto validate = [&](int x) __attribute__((always_inline)) {
-            if (lane >= 16) return;
-            const int us = x + 7;
-            if (!(vHas && us < SL)) return;
-            const uint32_t ea = ringA + (x % W8_R) * SLOT + vOff;
-            bool ok = (uint32_t)(lds_ld64(ea) >> 32) == tag;
-            if (NS == 2) ok = ok && (uint32_t)(lds_ld64(ea + 8) >> 32) == tag;
-            if (ok) return;
-            const uint64_t* src = (const uint64_t*)((const unsigned char*)P.edge + (vT * SL + us) * ROWB + vW * EWB);
-            int spins = 0;
-            for (;;) {
-                if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                const uint64_t a0 = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t a1 = NS == 2 ? __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a0;
-                if ((uint32_t)(a0 >> 32) == tag && (uint32_t)(a1 >> 32) == tag) {
-                    lds_st64(ea, a0);
-                    if (NS == 2) lds_st64(ea + 8, a1);
-                    break;
+        int cons_seen = 0;
+        // one step: load the words of step x + 1 into (n0, n1) while checking and
+        // placing those of step x (c0, c1); two register sets alternate, so no
+        // copy makes the compiler wait for the load just issued
+        auto step = [&](int x, uint64_t& c0, uint64_t& c1, uint64_t& n0, uint64_t& n1) __attribute__((always_inline)) {
+            const int xn = x + 1 < SL ? x + 1 : x;
+            n0 = __hip_atomic_load(src(xn), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (NS == 2) n1 = __hip_atomic_load(src(xn) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t want = tag_of(x + 7);
+            if (has && x + 7 < SL) {
+                int spins = 0;
+                while (!((uint32_t)(c0 >> 32) == want && (NS == 1 || (uint32_t)(c1 >> 32) == want))) {
+                    if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                    if (++spins > WV_SPIN_EDGE) { atomicOr(err, 2); break; }
+                    __builtin_amdgcn_s_sleep(2);
+                    c0 = __hip_atomic_load(src(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (NS == 2) c1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > WV_SPIN_EDGE) { atomicOr(err, 2); break; }
+            }
+            w8_wait_slot(x, cons_seen, ctrA + 4, err);
+            if (mine && x < SL) {
+                const uint32_t ea = ringA + (x % W8_R) * SLOT + off;
+                asm volatile("ds_write_b64 %0, %1" ::"v"(ea), "v"(c0) : "memory");
+                if (NS == 2) asm volatile("ds_write_b64 %0, %1 offset:8" ::"v"(ea), "v"(c1) : "memory");
             }
         };
-        // (one call site per lambda keeps them inlined; with more the compiler
-        // builds their closures in scratch memory)
-        if (wid == 1) {
-            // records: step x + LEAD issued at iteration x, step x waited for then
-            for (int x = -W8_LEAD; x < SL; ++x) {
-                if (x + W8_LEAD < SL) issue_data(x + W8_LEAD);
-                if (x < 0) continue;
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (x + W8_LEAD < SL) vm_wait<W8_LEAD * PERD>();
-                else vm_wait<0>();
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                lds_st32(ctrA, x + 1);
-            }
-        } else {
-            // edges: step x + E issued at iteration x, step x checked then
-            for (int x = -W8_E; x < SL; ++x) {
-                if (x + W8_E < SL) issue_edge(x + W8_E);
-                if (x < 0) continue;
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (x + W8_E < SL) vm_wait<W8_E>();
-                else vm_wait<0>();
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                validate(x);
-                lds_st32(ctrA + 8, x + 1);
-            }
+        uint64_t a0 = __hip_atomic_load(src(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), a1 = 0, b0 = 0, b1 = 0;
+        if (NS == 2) a1 = __hip_atomic_load(src(0) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // (an even number of steps keeps the register state the same at the loop
+        // head; a step x >= SL only issues a harmless load)
+        for (int x = 0; x < SL; x += 2) {
+            step(x, a0, a1, b0, b1);
+            step(x + 1, b0, b1, a0, a1);
         }
         return;
     }
@@ -774,72 +774,139 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
     const bool eK = FWD ? kl == 0 : kl == 7;  // k-neighbour in the upstream tile
     const bool oJ = jl == CJ;                 // edge column (read by the j-downstream tile)
     const bool oK = kl == (FWD ? 7 : 0);      // edge row (read by the k-downstream tile)
+    const bool nJ = eJ && hasJ, nK = eK && hasK;  // the edge values this lane uses
     const int bp = ((lane + (FWD ? 56 : 8)) & 63) * 4;  // lane -8 / +8 (edge lanes are overridden)
     const int ejo = EOFF + kl * EWB, eko = EOFF + ROWB + w8_kword<FWD>(jl) * EWB;
     const int eout = oJ ? kl * EWB : (8 + jl) * EWB;
     const __amdgpu_buffer_rsrc_t erc = __builtin_amdgcn_make_buffer_rsrc(P.edge, (short)0, 0x7fffffff, 0x00020000);
     typedef __attribute__((address_space(3))) const T ldsT;
-    T qp = 0;
+    typedef __attribute__((address_space(3))) const uint64_t ldsW;
+    typedef T T2v __attribute__((ext_vector_type(2)));
+    typedef T T4v __attribute__((ext_vector_type(4)));
+    struct Rec { T L0, L1, L2, ic, rin; };
     int seen = 0;
-    for (int s = 0; s < SL; ++s) {
+    // wait until the record wave has published step s
+    auto need_rec = [&](int s) __attribute__((always_inline)) {
         if (seen < s + 1) {
             int spins = 0;
             for (;;) {
-                const int r0 = lds_ld32(ctrA), r2 = lds_ld32(ctrA + 8);
-                seen = __builtin_amdgcn_readfirstlane(r0 < r2 ? r0 : r2);
+                seen = __builtin_amdgcn_readfirstlane(lds_ld32(ctrA));
                 if (seen >= s + 1) break;
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > 4 * WV_SPIN) { atomicOr(err, 8); seen = SL; break; }
             }
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    };
+    // records + input of step s from its ring slot
+    auto load_rec = [&](int s, Rec& R) __attribute__((always_inline)) {
         const lds_u8* sl = ring + (s % W8_R) * SLOT;
-        T L0, L1, L2, ic;
-        typedef T T2v __attribute__((ext_vector_type(2)));
-        typedef T T4v __attribute__((ext_vector_type(4)));
         if constexpr (NS == 1) {
             const T4v r4 = *(__attribute__((address_space(3))) const T4v*)(sl + lane * 16);
-            L0 = r4.x; L1 = r4.y; L2 = r4.z; ic = r4.w;
+            R.L0 = r4.x; R.L1 = r4.y; R.L2 = r4.z; R.ic = r4.w;
         } else {
             const T2v a = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32);
             const T2v b = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32 + 16);
-            L0 = a.x; L1 = a.y; L2 = b.x; ic = b.y;
+            R.L0 = a.x; R.L1 = a.y; R.L2 = b.x; R.ic = b.y;
         }
-        const T rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
-        const T ej = wv_edge_val(sl + ejo, (T*)nullptr), ek = wv_edge_val(sl + eko, (T*)nullptr);
-        // neighbours' results of step s-1
+        R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
+    };
+    T qp = 0;
+    // One step.  The neighbour exchange of the previous step's results, the edge
+    // words and the next step's records are all requested before the first
+    // wait, so their latencies overlap; then the edge tags are checked and the
+    // dependent chain runs on records read one step earlier.
+    auto step = [&](int s, Rec& cur, Rec& nxt) __attribute__((always_inline)) {
+        const int sn = s + 1 < SL ? s + 1 : s;  // (the last step re-reads its own records)
+        need_rec(sn);
+        const lds_u8* sl = ring + (s % W8_R) * SLOT;
+        // upstream edge values of this step: wait (bounded) until the edge wave
+        // has placed words with this step's tag (value and tag of a word come
+        // from one 8-byte read: read apart, the edge wave could replace the word
+        // between the two)
+        const bool live = s + 7 < SL;
+        const uint32_t want = tag_of(s + 7);
+        const uint64_t wj0 = *(ldsW*)(sl + ejo), wk0 = *(ldsW*)(sl + eko);
+        const uint64_t wj1 = NS == 2 ? *(ldsW*)(sl + ejo + 8) : wj0, wk1 = NS == 2 ? *(ldsW*)(sl + eko + 8) : wk0;
+        // issue order: edge words, exchange, next step's records (the waits that
+        // follow are in that order, and each leaves the later requests in flight)
+        __builtin_amdgcn_sched_barrier(0);
         T qj = wv_shift<FWD>(qp);
         T qk = wv_perm(bp, qp);
+        __builtin_amdgcn_sched_barrier(0);
+        load_rec(sn, nxt);
+        __builtin_amdgcn_sched_barrier(0);
+        T ej, ek;
+        if constexpr (NS == 1) {
+            ej = __uint_as_float((uint32_t)wj0);
+            ek = __uint_as_float((uint32_t)wk0);
+        } else {
+            ej = __longlong_as_double((long long)((wj1 << 32) | (wj0 & 0xffffffffull)));
+            ek = __longlong_as_double((long long)((wk1 << 32) | (wk0 & 0xffffffffull)));
+        }
+        bool okJ = !(nJ && live) || ((uint32_t)(wj0 >> 32) == want && (uint32_t)(wj1 >> 32) == want);
+        bool okK = !(nK && live) || ((uint32_t)(wk0 >> 32) == want && (uint32_t)(wk1 >> 32) == want);
+        if (__builtin_amdgcn_ballot_w64(!(okJ && okK))) {
+            int spins = 0;
+            const uint32_t aj = ringA + (s % W8_R) * SLOT + ejo, ak = ringA + (s % W8_R) * SLOT + eko;
+            while (__builtin_amdgcn_ballot_w64(!(okJ && okK))) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > 4 * WV_SPIN) { atomicOr(err, 16); break; }
+                const uint64_t wj = lds_ld64(aj), wk = lds_ld64(ak);
+                const uint64_t hj = NS == 2 ? lds_ld64(aj + 8) : wj, hk = NS == 2 ? lds_ld64(ak + 8) : wk;
+                okJ = !(nJ && live) || ((uint32_t)(wj >> 32) == want && (uint32_t)(hj >> 32) == want);
+                okK = !(nK && live) || ((uint32_t)(wk >> 32) == want && (uint32_t)(hk >> 32) == want);
+                if constexpr (NS == 1) {
+                    ej = __uint_as_float((uint32_t)wj);
+                    ek = __uint_as_float((uint32_t)wk);
+                } else {
+                    ej = __longlong_as_double((long long)((hj << 32) | (wj & 0xffffffffull)));
+                    ek = __longlong_as_double((long long)((hk << 32) | (wk & 0xffffffffull)));
+                }
+            }
+        }
         qj = eJ ? ej : qj;
         qk = eK ? ek : qk;
-        T v = rin, w;
+        T v = cur.rin, w;
         if (FWD) {
             // column order c-nn, c-ni, c-1
-            w = v - L0 * qk; v = wv_sel<T>(w8_present(L0), w, v);
-            w = v - L1 * qj; v = wv_sel<T>(w8_present(L1), w, v);
-            w = v - L2 * qp; v = wv_sel<T>(w8_present(L2), w, v);
+            w = v - cur.L0 * qk; v = wv_sel<T>(w8_present(cur.L0), w, v);
+            w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
+            w = v - cur.L2 * qp; v = wv_sel<T>(w8_present(cur.L2), w, v);
         } else {
             // row order c+1, c+ni, c+nn
-            w = v - L0 * qp; v = wv_sel<T>(w8_present(L0), w, v);
-            w = v - L1 * qj; v = wv_sel<T>(w8_present(L1), w, v);
-            w = v - L2 * qk; v = wv_sel<T>(w8_present(L2), w, v);
+            w = v - cur.L0 * qp; v = wv_sel<T>(w8_present(cur.L0), w, v);
+            w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
+            w = v - cur.L2 * qk; v = wv_sel<T>(w8_present(cur.L2), w, v);
         }
-        const T q = v * ic;
+        const T q = v * cur.ic;
         P.out[(myTile * SL + (FWD ? s : SL - 1 - s)) * 64 + lane] = q;
         if (oJ || oK) {
             const int row = (int)((myTile * SL + s) * ROWB);
+            const uint32_t t = tag_of(s);
             if constexpr (NS == 1) {
-                const wv_u32x2 a = {__float_as_uint((float)q), tag};
+                const wv_u32x2 a = {__float_as_uint((float)q), t};
                 __builtin_amdgcn_raw_buffer_store_b64(a, erc, row + eout, 0, WV_SC1);
             } else {
                 const uint64_t u = (uint64_t)__double_as_longlong((double)q);
-                const wv_u32x4 a = {(uint32_t)u, tag, (uint32_t)(u >> 32), tag};
+                const wv_u32x4 a = {(uint32_t)u, t, (uint32_t)(u >> 32), t};
                 __builtin_amdgcn_raw_buffer_store_b128(a, erc, row + eout, 0, WV_SC1);
             }
         }
         qp = q;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        lds_st32_nowait(ctrA + 4, s + 1);
+        lds_st32_nowait(ctrA + 4, s + 1);  // step s consumed (step s + 1's reads are issued, in order)
+    };
+    Rec ra, rb;
+    need_rec(0);
+    load_rec(0, ra);
+    // (settle every load issued so far, kernel arguments included, so the step
+    // loop's waits only ever concern its own requests)
+    __builtin_amdgcn_s_waitcnt(0);
+    // two record sets alternate (no register copies between steps)
+    for (int s = 0; s < SL; s += 2) {
+        step(s, ra, rb);
+        if (s + 1 < SL) step(s + 1, rb, ra);
     }
 }
 
@@ -860,7 +927,9 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     const long ebytes = (long)ntj * ntk * w8_slots(g) * 16 * 8 * (long)(sizeof(T) / 4);
     if (!flags || !epoch || !sk.oRec || ebytes >= 0x7fffffffL) return false;
     k_to_oct<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, r, sk.oR, S);
-    const uint32_t tag1 = wv_tag((*epoch)++), tag2 = wv_tag((*epoch)++);
+    // per-sweep tag base; the low 8 bits carry the step of each edge word
+    const uint32_t tag1 = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
+    const uint32_t tag2 = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
     const W8Planes<T> f{sk.oR, sk.oQ, sk.oRec, sk.oEdge};
     const W8Planes<T> b{sk.oQ, sk.oZ, sk.oRec + 4 * n, sk.oEdge};
     const int tiles = ntj * ntk;
