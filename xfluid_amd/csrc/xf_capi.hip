@@ -352,6 +352,8 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     s->pw.nflags = sweep_flag_words(s->g);
     s->pw.epoch = 1;
     if (dalloc(&s->pw.sweep_flags, s->pw.nflags) != hipSuccess) return fail(set_err(-2, "alloc flags"));
+    if (dalloc(&s->pw.box_dev, 8) != hipSuccess || hipHostMalloc((void**)&s->pw.box_host, 8 * sizeof(int)) != hipSuccess)
+        return fail(set_err(-2, "alloc box"));
     s->pw.nskew = sweep_skewed_size(s->g);
     {
         const long M = s->pw.nskew;
@@ -422,6 +424,8 @@ void xf_sim_destroy(xf_sim* s) {
                       &s->P.key, &s->P.perm, &s->surf_cnt, &s->surf_start};
     for (auto* p : is) dfree(*p);
     dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
+    dfree(s->pw.box_dev);
+    if (s->pw.box_host) { hipHostFree(s->pw.box_host); s->pw.box_host = nullptr; }
     dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q); dfree(s->pw.sk64.z);
     dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q); dfree(s->pw.sk32.z);
     dfree(s->pw.sk64.eb); dfree(s->pw.sk32.eb);

@@ -44,6 +44,8 @@ struct SweepSkew {
     T* oRec;
     T *oR, *oQ, *oZ;
     uint32_t* oEdge;
+    // bounding box of the active rows (i0, i1, j0, j1, k0, k1; host copy), see pcg_active_box
+    int box[6];
 };
 // Optional device timing of the sweep launches (pairs of events around the
 // forward + backward launches of each preconditioner application).
@@ -68,6 +70,8 @@ struct PcgWork {
     long nskew;
     SweepTimer timer;
     unsigned long long epoch;  // sweep launch counter (tags progress counters)
+    int* box_dev;   // 6 ints: bounding box of the active rows
+    int* box_host;  // pinned mirror
 };
 struct PcgResult {
     int status, iterations, outer;
@@ -77,7 +81,12 @@ struct PcgResult {
 PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,
                     double sigma, int precision, double inner_tol, int max_outer, int sweep_kernel, hipStream_t st);
 // Individual pieces, for parity checks.
-void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, hipStream_t st);
+void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const int* box,
+              hipStream_t st);
+// Bounding box of the active rows (rows with a diagonal or an off-diagonal
+// entry); false when there are none.  Rows outside it are empty: their
+// pivots, residuals and preconditioned values are zero.
+bool pcg_active_box(const Geo& g, const uint16_t* code, PcgWork& w, int box[6], hipStream_t st);
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
                  int sweep_kernel, PcgWork& w, hipStream_t st);
 void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st);

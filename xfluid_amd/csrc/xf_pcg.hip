@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include "xf_internal.h"
+#include <climits>
 
 #define RED_BLOCKS 1024
 
@@ -87,9 +88,66 @@ static void plane_range(const Geo& g, int t, int* kmin, long* nthreads) {
     *nthreads = (long)g.nj * (k1 - k0 + 1);
 }
 
-void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, hipStream_t st) {
-    const int planes = g.ni + g.nj + g.nk - 2;
-    for (int t = 0; t < planes; ++t) {
+template <class T>
+__global__ void k_fill(long n, T* __restrict__ d, T v);
+
+// Active rows' bounding box: one block-level min/max per block, then atomics.
+__global__ void k_active_box(Geo g, const uint16_t* __restrict__ code, int* box) {
+    __shared__ int red[6][XF_BLOCK / 64];
+    const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int v[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
+    if (c < g.N) {
+        const uint16_t cc = code[c];
+        bool act = code_diag(cc) != 0.0;
+        for (int d = 0; d < 6; ++d) act = act || present(code_dir(cc, d));
+        if (act) {
+            const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+            v[0] = v[1] = i; v[2] = v[3] = j; v[4] = v[5] = k;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 6; ++d) {
+            const int w = __shfl_xor(v[d], o, 64);
+            v[d] = (d & 1) ? (w > v[d] ? w : v[d]) : (w < v[d] ? w : v[d]);
+        }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 6; ++d) red[d][wv] = v[d];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        int m = red[d][0];
+        for (int q = 1; q < XF_BLOCK / 64; ++q) m = (d & 1) ? (red[d][q] > m ? red[d][q] : m) : (red[d][q] < m ? red[d][q] : m);
+        if ((d & 1) ? m >= 0 : m != INT_MAX) {
+            if (d & 1) atomicMax(box + d, m);
+            else atomicMin(box + d, m);
+        }
+    }
+}
+__global__ void k_box_init(int* box) {
+    if (threadIdx.x < 6) box[threadIdx.x] = (threadIdx.x & 1) ? -1 : INT_MAX;
+}
+
+bool pcg_active_box(const Geo& g, const uint16_t* code, PcgWork& w, int box[6], hipStream_t st) {
+    k_box_init<<<1, 64, 0, st>>>(w.box_dev);
+    k_active_box<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, code, w.box_dev);
+    hipMemcpyAsync(w.box_host, w.box_dev, 6 * sizeof(int), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    for (int d = 0; d < 6; ++d) box[d] = w.box_host[d];
+    return box[1] >= box[0] && box[3] >= box[2] && box[5] >= box[4];
+}
+
+// box: when given, only the hyperplanes that meet the active rows' bounding
+// box are factored and every other pivot is set to zero (their rows are empty).
+void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const int* box,
+              hipStream_t st) {
+    int t0 = 0, t1 = g.ni + g.nj + g.nk - 3;
+    if (box) {
+        k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g.N, inv, 0.0);
+        t0 = box[0] + box[2] + box[4];
+        t1 = box[1] + box[3] + box[5];
+    }
+    for (int t = t0; t <= t1; ++t) {
         int kmin;
         long nt;
         plane_range(g, t, &kmin, &nt);
@@ -177,6 +235,10 @@ static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
 
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
                  int sweep_kernel, PcgWork& w, hipStream_t st) {
+    if (!pcg_active_box(g, code, w, w.sk64.box, st)) {
+        w.sk64.box[0] = w.sk64.box[2] = w.sk64.box[4] = 0;
+        w.sk64.box[1] = g.ni - 1; w.sk64.box[3] = g.nj - 1; w.sk64.box[5] = g.nk - 1;
+    }
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, inv, w.sk64, st);
     else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, inv, w.sk64, st);
     else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, inv, w.sk64, st);
@@ -402,7 +464,13 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     const long N = g.N;
     if (tol_factor < 1e-30) tol_factor = 1e-30;
     PcgResult res{};
-    pcg_mic0(g, code, w.inv, tau, sigma, st);
+    int box[6];
+    if (!pcg_active_box(g, code, w, box, st)) {
+        box[0] = box[2] = box[4] = 0;
+        box[1] = g.ni - 1; box[3] = g.nj - 1; box[5] = g.nk - 1;
+    }
+    for (int d = 0; d < 6; ++d) w.sk64.box[d] = w.sk32.box[d] = box[d];
+    pcg_mic0(g, code, w.inv, tau, sigma, box, st);
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
     else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, w.inv, w.sk64, st);
     else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, w.inv, w.sk64, st);

@@ -581,12 +581,16 @@ __global__ void k_to_oct(Geo g, int ntk, long n, const T* __restrict__ src, T* _
     dst[e] = v;
 }
 
+struct Box8 { int i0, i1, j0, j1, k0, k1; };
+// cells outside the active box are zero (the sweep did not visit their slots)
 template <class T>
-__global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S) {
+__global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S,
+                           Box8 b) {
     if (S && S->done) return;
     const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    if (i < b.i0 || i > b.i1 || j < b.j0 || j > b.j1 || k < b.k0 || k > b.k1) { dst[c] = (T)0; return; }
     const long t = (long)(j >> 3) * ntk + (k >> 3);
     const int jl = j & 7, kl = k & 7;
     dst[c] = src[(t * w8_slots(g) + i + jl + kl) * 64 + jl + 8 * kl];
@@ -655,8 +659,12 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
 }
 
 template <class T, bool FWD>
-__global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Planes<T> P, const PcgScalars* S,
-                                                  uint32_t tagbase, int* err) {
+// Runs the tiles [J0, J0 + ntjb) x [K0, K0 + ntkb) of the ntk-wide tile grid over
+// the slots [s0, s0 + nS): the tiles and slots that hold active rows (all other
+// rows are empty and their results zero).  Step t works on slot s0 + t forward,
+// s0 + nS - 1 - t backward; edge rows are indexed by step.
+__global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0, int nS,
+                                                  W8Planes<T> P, const PcgScalars* S, uint32_t tagbase, int* err) {
     constexpr int NS = sizeof(T) / 4;
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
@@ -670,12 +678,15 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
     __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
     if (S && S->done) return;
     int J, K;
-    wv_tile(blockIdx.x, ntj, ntk, FWD, &J, &K);
+    wv_tile(blockIdx.x, ntjb, ntkb, FWD, &J, &K);
+    J += J0;
+    K += K0;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int SL = w8_slots(g);
+    const int SL = w8_slots(g);  // slots per tile in the layout
     const long myTile = (long)J * ntk + K;
     const int uJ = FWD ? J - 1 : J + 1, uK = FWD ? K - 1 : K + 1;
-    const bool hasJ = uJ >= 0 && uJ < ntj, hasK = uK >= 0 && uK < ntk;
+    const bool hasJ = uJ >= J0 && uJ < J0 + ntjb, hasK = uK >= K0 && uK < K0 + ntkb;
+    auto slot_of = [s0, nS](int t) __attribute__((always_inline)) { return FWD ? s0 + t : s0 + nS - 1 - t; };
     const long tJ = (long)uJ * ntk + K, tK = (long)J * ntk + uK;
     lds_u8* ring = (lds_u8*)ring_;
     lds_int* ctr = (lds_int*)ctr_;
@@ -693,11 +704,11 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
         // records and input of step x + LEAD issued at iteration x (LDS-DMA into
         // the ring), step x waited for and published then
         int cons_seen = 0;
-        for (int x = -W8_LEAD; x < SL; ++x) {
+        for (int x = -W8_LEAD; x < nS; ++x) {
             const int s = x + W8_LEAD;
-            if (s < SL) {
+            if (s < nS) {
                 w8_wait_slot(s, cons_seen, ctrA + 4, err);
-                const long e0 = (myTile * SL + (FWD ? s : SL - 1 - s)) * 64;
+                const long e0 = (myTile * SL + slot_of(s)) * 64;
                 lds_u8* dst = ring + (s % W8_R) * SLOT;
 #pragma unroll
                 for (int h = 0; h < RECB / 1024; ++h)
@@ -708,7 +719,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
             }
             if (x < 0) continue;
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (x + W8_LEAD < SL) vm_wait<W8_LEAD * PERD>();
+            if (x + W8_LEAD < nS) vm_wait<W8_LEAD * PERD>();
             else vm_wait<0>();
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             lds_st32(ctrA, x + 1);
@@ -728,7 +739,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
         const int w = mine ? (sideJ ? lane : w8_kword<FWD>(lane - 8)) : 0;
         const int off = EOFF + (sideJ ? 0 : ROWB) + w * EWB;
         auto src = [&](int x) __attribute__((always_inline)) {
-            const int us = x + 7 < SL ? x + 7 : SL - 1;
+            const int us = x + 7 < nS ? x + 7 : nS - 1;
             return (const uint64_t*)((const unsigned char*)P.edge + (ut * SL + us) * ROWB + w * EWB);
         };
         int cons_seen = 0;
@@ -736,11 +747,11 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
         // placing those of step x (c0, c1); two register sets alternate, so no
         // copy makes the compiler wait for the load just issued
         auto step = [&](int x, uint64_t& c0, uint64_t& c1, uint64_t& n0, uint64_t& n1) __attribute__((always_inline)) {
-            const int xn = x + 1 < SL ? x + 1 : x;
+            const int xn = x + 1 < nS ? x + 1 : x;
             n0 = __hip_atomic_load(src(xn), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (NS == 2) n1 = __hip_atomic_load(src(xn) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t want = tag_of(x + 7);
-            if (has && x + 7 < SL) {
+            if (has && x + 7 < nS) {
                 int spins = 0;
                 while (!((uint32_t)(c0 >> 32) == want && (NS == 1 || (uint32_t)(c1 >> 32) == want))) {
                     if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
@@ -751,7 +762,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
                 }
             }
             w8_wait_slot(x, cons_seen, ctrA + 4, err);
-            if (mine && x < SL) {
+            if (mine && x < nS) {
                 const uint32_t ea = ringA + (x % W8_R) * SLOT + off;
                 asm volatile("ds_write_b64 %0, %1" ::"v"(ea), "v"(c0) : "memory");
                 if (NS == 2) asm volatile("ds_write_b64 %0, %1 offset:8" ::"v"(ea), "v"(c1) : "memory");
@@ -760,8 +771,8 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
         uint64_t a0 = __hip_atomic_load(src(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), a1 = 0, b0 = 0, b1 = 0;
         if (NS == 2) a1 = __hip_atomic_load(src(0) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // (an even number of steps keeps the register state the same at the loop
-        // head; a step x >= SL only issues a harmless load)
-        for (int x = 0; x < SL; x += 2) {
+        // head; a step x >= nS only issues a harmless load)
+        for (int x = 0; x < nS; x += 2) {
             step(x, a0, a1, b0, b1);
             step(x + 1, b0, b1, a0, a1);
         }
@@ -793,7 +804,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
                 seen = __builtin_amdgcn_readfirstlane(lds_ld32(ctrA));
                 if (seen >= s + 1) break;
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > 4 * WV_SPIN) { atomicOr(err, 8); seen = SL; break; }
+                if (++spins > 4 * WV_SPIN) { atomicOr(err, 8); seen = nS; break; }
             }
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -817,14 +828,14 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
     // wait, so their latencies overlap; then the edge tags are checked and the
     // dependent chain runs on records read one step earlier.
     auto step = [&](int s, Rec& cur, Rec& nxt) __attribute__((always_inline)) {
-        const int sn = s + 1 < SL ? s + 1 : s;  // (the last step re-reads its own records)
+        const int sn = s + 1 < nS ? s + 1 : s;  // (the last step re-reads its own records)
         need_rec(sn);
         const lds_u8* sl = ring + (s % W8_R) * SLOT;
         // upstream edge values of this step: wait (bounded) until the edge wave
         // has placed words with this step's tag (value and tag of a word come
         // from one 8-byte read: read apart, the edge wave could replace the word
         // between the two)
-        const bool live = s + 7 < SL;
+        const bool live = s + 7 < nS;
         const uint32_t want = tag_of(s + 7);
         const uint64_t wj0 = *(ldsW*)(sl + ejo), wk0 = *(ldsW*)(sl + eko);
         const uint64_t wj1 = NS == 2 ? *(ldsW*)(sl + ejo + 8) : wj0, wk1 = NS == 2 ? *(ldsW*)(sl + eko + 8) : wk0;
@@ -880,7 +891,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
             w = v - cur.L2 * qk; v = wv_sel<T>(w8_present(cur.L2), w, v);
         }
         const T q = v * cur.ic;
-        P.out[(myTile * SL + (FWD ? s : SL - 1 - s)) * 64 + lane] = q;
+        P.out[(myTile * SL + slot_of(s)) * 64 + lane] = q;
         if (oJ || oK) {
             const int row = (int)((myTile * SL + s) * ROWB);
             const uint32_t t = tag_of(s);
@@ -904,9 +915,9 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntj, int ntk, W8Pla
     // loop's waits only ever concern its own requests)
     __builtin_amdgcn_s_waitcnt(0);
     // two record sets alternate (no register copies between steps)
-    for (int s = 0; s < SL; s += 2) {
+    for (int s = 0; s < nS; s += 2) {
         step(s, ra, rb);
-        if (s + 1 < SL) step(s + 1, rb, ra);
+        if (s + 1 < nS) step(s + 1, rb, ra);
     }
 }
 
@@ -927,17 +938,21 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     const long ebytes = (long)ntj * ntk * w8_slots(g) * 16 * 8 * (long)(sizeof(T) / 4);
     if (!flags || !epoch || !sk.oRec || ebytes >= 0x7fffffffL) return false;
     k_to_oct<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, r, sk.oR, S);
+    const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
+    const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
     // per-sweep tag base; the low 8 bits carry the step of each edge word
     const uint32_t tag1 = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
     const uint32_t tag2 = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
     const W8Planes<T> f{sk.oR, sk.oQ, sk.oRec, sk.oEdge};
     const W8Planes<T> b{sk.oQ, sk.oZ, sk.oRec + 4 * n, sk.oEdge};
-    const int tiles = ntj * ntk;
+    const int ntjb = J1 - J0 + 1, ntkb = K1 - K0 + 1, nS = s1 - s0 + 1;
+    const int tiles = ntjb * ntkb;
     if (t0) hipEventRecord(t0, st);
-    k_sweep_w8<T, true><<<tiles, 192, 0, st>>>(g, ntj, ntk, f, S, tag1, flags);
-    k_sweep_w8<T, false><<<tiles, 192, 0, st>>>(g, ntj, ntk, b, S, tag2, flags);
+    k_sweep_w8<T, true><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags);
+    k_sweep_w8<T, false><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags);
     if (t1) hipEventRecord(t1, st);
-    k_from_oct<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, S);
+    const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
+    k_from_oct<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, S, bx);
     return true;
 }
 
