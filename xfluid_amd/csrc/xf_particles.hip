@@ -329,11 +329,122 @@ __global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, 
     out[t] = wsum > 0.0000001 ? vsum / wsum : 0.0;
 }
 
+// The same gather over bricks of 8x4x4 faces.  A workgroup walks the brick's
+// candidate cells one k-slab at a time: it copies the slab's particles (one
+// contiguous run per cell row of the sorted arrays) into LDS, then every face
+// adds the slab's rows inside its support, reading particles from LDS.  Each
+// face still visits cells in k, j, i order and particles by index, with the
+// per-pair arithmetic of k_p2g, so the result is bit-identical to it; each
+// particle is fetched once per brick instead of once per face.  A slab with
+// more particles than fit is read from global memory in the same order.
+#define P2G_BI 8
+#define P2G_BJ 4
+#define P2G_BK 4
+#define P2G_CAP 1536
+#define P2G_MAXC 12  // cells along i in a slab row (8 faces + support)
+__global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
+    Geo g, int axis, int kern, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
+    double* __restrict__ out) {
+    __shared__ double lx[P2G_CAP], ly[P2G_CAP], lz[P2G_CAP], lv[P2G_CAP];
+    __shared__ int raw[P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slab's cells, per row
+    __shared__ int roff[P2G_BJ + 3];                // LDS offset of each slab row
+    __shared__ int nslab;
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    const long b = xcd_block(blockIdx.x, gridDim.x);
+    const int bi = (int)(b % nbi), bj = (int)((b / nbi) % nbj), bk = (int)(b / ((long)nbi * nbj));
+    const int F[3] = {bi * P2G_BI, bj * P2G_BJ, bk * P2G_BK};
+    const int tid = threadIdx.x;
+    const int fi = F[0] + tid % P2G_BI, fj = F[1] + (tid / P2G_BI) % P2G_BJ, fk = F[2] + tid / (P2G_BI * P2G_BJ);
+    const bool valid = fi < si && fj < sj && fk < sk;
+    const bool tri = kern == 1;
+    const double dx = g.dx;
+    const double r = tri ? dx : 1.5 * dx;
+    const Wyvill W = make_wyvill(r);
+    const int n[3] = {g.ni, g.nj, g.nk};
+    // this face's support cells and the brick's candidate cells, per direction
+    int lo[3], hi[3], clo[3], chi[3];
+    const int fc[3] = {fi, fj, fk}, bsz[3] = {P2G_BI, P2G_BJ, P2G_BK};
+    for (int a = 0; a < 3; ++a) {
+        const int dl = a == axis ? (tri ? 1 : 2) : 1, dh = a == axis ? (tri ? 0 : 1) : 1;
+        lo[a] = fc[a] - dl; hi[a] = fc[a] + dh;
+        clo[a] = F[a] - dl; chi[a] = F[a] + bsz[a] - 1 + dh;
+        if (lo[a] < 0) lo[a] = 0;
+        if (hi[a] > n[a] - 1) hi[a] = n[a] - 1;
+        if (clo[a] < 0) clo[a] = 0;
+        if (chi[a] > n[a] - 1) chi[a] = n[a] - 1;
+    }
+    const double cx = idx2pos(fi, dx), cy = idx2pos(fj, dx), cz = idx2pos(fk, dx);
+    const double sx = axis == 0 ? 0.5 * dx : 0.0, sy = axis == 1 ? 0.5 * dx : 0.0, sz = axis == 2 ? 0.5 * dx : 0.0;
+    double wsum = 0.0, vsum = 0.0;
+    auto pair = [&](double qx, double qy, double qz, double v) __attribute__((always_inline)) {
+        const double ex = (qx + sx) - cx, ey = (qy + sy) - cy, ez = (qz + sz) - cz;
+        double cw;
+        if (tri) {
+            const double ax = fabs(ex) / dx, ay = fabs(ey) / dx, az = fabs(ez) / dx;
+            cw = (ax < 1.0 && ay < 1.0 && az < 1.0) ? (1.0 - ax) * (1.0 - ay) * (1.0 - az) : 0.0;
+        } else {
+            const double s2 = ex * ex + ey * ey + ez * ez;
+            cw = wyvill(W, s2);
+        }
+        if (cw != 0.0) {
+            wsum += cw;
+            vsum += v * cw;
+        }
+    };
+    const int nrow = chi[1] - clo[1] + 1, ncell = chi[0] - clo[0] + 1;
+    for (int k = clo[2]; k <= chi[2]; ++k) {
+        // slab k: the cell starts of its rows, the rows' LDS offsets, then the particles
+        if (tid < nrow * (ncell + 1)) {
+            const int jr = tid / (ncell + 1), ic = tid % (ncell + 1);
+            raw[jr][ic] = start[(long)g.ni * (clo[1] + jr) + g.nn * k + clo[0] + ic];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int jr = 0; jr < nrow; ++jr) {
+                roff[jr] = acc;
+                acc += raw[jr][ncell] - raw[jr][0];
+            }
+            nslab = acc;
+        }
+        __syncthreads();
+        const bool fits = nslab <= P2G_CAP;
+        if (fits) {
+            for (int jr = 0; jr < nrow; ++jr) {
+                const int a = raw[jr][0], m = raw[jr][ncell] - a, o = roff[jr];
+                for (int q = tid; q < m; q += blockDim.x) {
+                    lx[o + q] = px[a + q]; ly[o + q] = py[a + q]; lz[o + q] = pz[a + q]; lv[o + q] = vc[a + q];
+                }
+            }
+        }
+        __syncthreads();
+        if (valid && k >= lo[2] && k <= hi[2]) {
+            for (int j = lo[1]; j <= hi[1]; ++j) {
+                const int jr = j - clo[1];
+                if (fits) {
+                    const int base = roff[jr] - raw[jr][0];
+                    const int pa = base + raw[jr][lo[0] - clo[0]], pb = base + raw[jr][hi[0] + 1 - clo[0]];
+                    for (int p = pa; p < pb; ++p) pair(lx[p], ly[p], lz[p], lv[p]);
+                } else {
+                    const long row = (long)g.ni * j + g.nn * k;
+                    const int pa = start[row + lo[0]], pb = start[row + hi[0] + 1];
+                    for (int p = pa; p < pb; ++p) pair(px[p], py[p], pz[p], vc[p]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) out[fi + (long)si * (fj + (long)sj * fk)] = wsum > 0.0000001 ? vsum / wsum : 0.0;
+}
+
 void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f, int axis, int kernel,
                 hipStream_t st) {
-    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
     const double* vc = axis == 0 ? P.vx : axis == 1 ? P.vy : P.vz;
-    k_p2g<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, axis, kernel, P.px, P.py, P.pz, vc, ci.start, f);
+    const int nbi = (si + P2G_BI - 1) / P2G_BI, nbj = (sj + P2G_BJ - 1) / P2G_BJ, nbk = (sk + P2G_BK - 1) / P2G_BK;
+    k_p2g_brick<<<nbi * nbj * nbk, P2G_BI * P2G_BJ * P2G_BK, 0, st>>>(g, axis, kernel, nbi, nbj, P.px, P.py, P.pz,
+                                                                       vc, ci.start, f);
 }
 
 // ---------------------------------------------------------------------------
