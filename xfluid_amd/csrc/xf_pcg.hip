@@ -378,9 +378,15 @@ __device__ inline double reduce_partials(const double* p, int n, double* sh) {
 }
 
 enum { FIN_RES0, FIN_RHO0, FIN_ALPHA, FIN_RES, FIN_BETA };
-__global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, double tol_factor, int it) {
+// mirror (optional): pinned host copy of S, written at the end (the host reads
+// it after synchronising the stream, with no copy kernel in between)
+__global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, double tol_factor, int it,
+                         PcgScalars* mirror = nullptr) {
     __shared__ double sh[16];
-    if (what != FIN_RES0 && S->done) return;
+    if (what != FIN_RES0 && S->done) {
+        if (mirror && threadIdx.x == 0) *mirror = *S;
+        return;
+    }
     const bool mx = (what == FIN_RES0 || what == FIN_RES);
     double v = mx ? reduce_partials<true>(partial, n, sh) : reduce_partials<false>(partial, n, sh);
     if (threadIdx.x != 0) return;
@@ -410,6 +416,7 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
             S->rho = v;
             break;
     }
+    if (mirror) *mirror = *S;
 }
 
 // ---------------------------------------------------------------------------
@@ -436,14 +443,14 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, s, z, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_ALPHA, partial, RED_BLOCKS, S, tol_factor, it);
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, S);
-            k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it);
+            k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it,
+                                         it == end - 1 ? S_host : nullptr);
             precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st);
             k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_BETA, partial, RED_BLOCKS, S, tol_factor, it);
             k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
         }
-        hipMemcpyAsync(S_host, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
-        hipStreamSynchronize(st);
+        hipStreamSynchronize(st);  // S_host was written by the chunk's last residual step
         if (S_host->done) break;
     }
     if (!S_host->done) {
@@ -498,8 +505,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
     k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
     k_absmax<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.partial);
-    k_finish<<<1, 1024, 0, st>>>(FIN_RES0, w.partial, RED_BLOCKS, S_outer, tol_factor, 0);
-    hipMemcpyAsync(S_outer_h, S_outer, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
+    k_finish<<<1, 1024, 0, st>>>(FIN_RES0, w.partial, RED_BLOCKS, S_outer, tol_factor, 0, S_outer_h);
     hipStreamSynchronize(st);
     res.rhs_norm = S_outer_h->res0;
     res.residual = S_outer_h->res0;
@@ -522,8 +528,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         ++outer;
         k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
         k_residual<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.b, w.x, w.r, w.partial);
-        k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0);
-        hipMemcpyAsync(S_outer_h, S_outer, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
+        k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0, S_outer_h);
         hipStreamSynchronize(st);
         res.residual = S_outer_h->res;
         if (S_outer_h->res <= tol) { res.status = 0; break; }
