@@ -105,6 +105,9 @@ void sweep_prepare_wave(const Geo& g, const uint16_t* code, const T* inv, const 
 template <class T>
 bool sweep_precond_wave(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                         unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
+// MIC(0) pivots through the 8x8-tile pipeline over sk.box (false: not run)
+bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const SweepSkew<double>& sk,
+             int* flags, unsigned long long* epoch, hipStream_t st);
 long sweep_w8_size(const Geo& g);
 long sweep_w8_edge_words(const Geo& g);
 template <class T>

@@ -477,7 +477,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         box[1] = g.ni - 1; box[3] = g.nj - 1; box[5] = g.nk - 1;
     }
     for (int d = 0; d < 6; ++d) w.sk64.box[d] = w.sk32.box[d] = box[d];
-    pcg_mic0(g, code, w.inv, tau, sigma, box, st);
+    if (!(sweep_kernel == 9 && mic0_w8(g, code, w.inv, tau, sigma, w.sk64, w.sweep_flags, &w.epoch, st)))
+        pcg_mic0(g, code, w.inv, tau, sigma, box, st);
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
     else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, w.inv, w.sk64, st);
     else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, w.inv, w.sk64, st);

@@ -658,21 +658,31 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
     }
 }
 
-template <class T, bool FWD>
 // Runs the tiles [J0, J0 + ntjb) x [K0, K0 + ntkb) of the ntk-wide tile grid over
 // the slots [s0, s0 + nS): the tiles and slots that hold active rows (all other
 // rows are empty and their results zero).  Step t works on slot s0 + t forward,
 // s0 + nS - 1 - t backward; edge rows are indexed by step.
+//
+// MIC: the same pipeline computes the MIC(0) pivots (factor_modified_incomplete_
+// cholesky0, pcg_solver.h:85-186, in the order of k_mic_plane): the value
+// exchanged between neighbours is the pivot, the records are {cL0, cL1, cL2,
+// diag, cM0, cM1, cM2, 0} with cL the entry's coefficient (+0 when absent: with
+// finite neighbour pivots both updates then leave d unchanged bit for bit) and
+// cM the sum of the neighbour's coefficients that the fill-in drops, and
+// there is no input vector.
+template <class T, bool FWD, bool MIC = false>
 __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0, int nS,
-                                                  W8Planes<T> P, const PcgScalars* S, uint32_t tagbase, int* err) {
+                                                  W8Planes<T> P, const PcgScalars* S, uint32_t tagbase, int* err,
+                                                  double tau = 0.0, double sigma = 0.0) {
+    static_assert(!MIC || (FWD && sizeof(T) == 8), "pivots: forward, FP64");
     constexpr int NS = sizeof(T) / 4;
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
-    constexpr int RECB = 64 * 4 * sizeof(T);     // record bytes per step (1 or 2 KB)
-    constexpr int INB = 64 * sizeof(T);          // input bytes per step
+    constexpr int RECB = 64 * (MIC ? 8 : 4) * sizeof(T);  // record bytes per step (1, 2 or 4 KB)
+    constexpr int INB = MIC ? 0 : 64 * sizeof(T);         // input bytes per step
     constexpr int EOFF = RECB + INB;             // upstream edge rows (j, then k) within a slot
     constexpr int SLOT = EOFF + 2 * ROWB;
-    constexpr int PERD = RECB / 1024 + 1;        // record + input loads per step
+    constexpr int PERD = RECB / 1024 + (INB ? 1 : 0);  // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
     __shared__ __attribute__((aligned(16))) unsigned char ring_[W8_R * SLOT];
     __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
@@ -712,9 +722,10 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
                 lds_u8* dst = ring + (s % W8_R) * SLOT;
 #pragma unroll
                 for (int h = 0; h < RECB / 1024; ++h)
-                    __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + 4 * e0) + h * 1024 + lane * 16,
+                    __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + (MIC ? 8 : 4) * e0) + h * 1024 +
+                                                         lane * 16,
                                                      dst + h * 1024, 16, 0, 0);
-                if (lane < INB / 16)
+                if (INB && lane < INB / 16)
                     __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + e0) + lane * 16, dst + RECB, 16, 0, 0);
             }
             if (x < 0) continue;
@@ -794,7 +805,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     typedef __attribute__((address_space(3))) const uint64_t ldsW;
     typedef T T2v __attribute__((ext_vector_type(2)));
     typedef T T4v __attribute__((ext_vector_type(4)));
-    struct Rec { T L0, L1, L2, ic, rin; };
+    struct Rec { T L0, L1, L2, ic, rin, M0, M1, M2; };
     int seen = 0;
     // wait until the record wave has published step s
     auto need_rec = [&](int s) __attribute__((always_inline)) {
@@ -812,15 +823,23 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     // records + input of step s from its ring slot
     auto load_rec = [&](int s, Rec& R) __attribute__((always_inline)) {
         const lds_u8* sl = ring + (s % W8_R) * SLOT;
-        if constexpr (NS == 1) {
+        if constexpr (MIC) {
+            // (diag in the ic field)
+            const T2v a = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 64);
+            const T2v b = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 64 + 16);
+            const T2v c = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 64 + 32);
+            const T2v d = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 64 + 48);
+            R.L0 = a.x; R.L1 = a.y; R.L2 = b.x; R.ic = b.y; R.M0 = c.x; R.M1 = c.y; R.M2 = d.x; R.rin = 0;
+        } else if constexpr (NS == 1) {
             const T4v r4 = *(__attribute__((address_space(3))) const T4v*)(sl + lane * 16);
             R.L0 = r4.x; R.L1 = r4.y; R.L2 = r4.z; R.ic = r4.w;
+            R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
         } else {
             const T2v a = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32);
             const T2v b = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32 + 16);
             R.L0 = a.x; R.L1 = a.y; R.L2 = b.x; R.ic = b.y;
+            R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
         }
-        R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
     };
     T qp = 0;
     // One step.  The neighbour exchange of the previous step's results, the edge
@@ -878,7 +897,22 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
         }
         qj = eJ ? ej : qj;
         qk = eK ? ek : qk;
-        T v = cur.rin, w;
+        T v = cur.rin, w, q;
+        if constexpr (MIC) {
+            // d = diag - sum over the -k, -j, -i entries of L^2 + tau L miss
+            const T diag = cur.ic;
+            T d = diag;
+            T L = cur.L0 * qk, miss = cur.M0 * qk;
+            d -= L * L;
+            d -= (T)tau * L * miss;
+            L = cur.L1 * qj; miss = cur.M1 * qj;
+            d -= L * L;
+            d -= (T)tau * L * miss;
+            L = cur.L2 * qp; miss = cur.M2 * qp;
+            d -= L * L;
+            d -= (T)tau * L * miss;
+            q = diag == 0 ? (T)0 : (d < (T)sigma * diag) ? 1 / sqrt(diag) : 1 / sqrt(d);
+        } else {
         if (FWD) {
             // column order c-nn, c-ni, c-1
             w = v - cur.L0 * qk; v = wv_sel<T>(w8_present(cur.L0), w, v);
@@ -890,7 +924,8 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
             w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
             w = v - cur.L2 * qk; v = wv_sel<T>(w8_present(cur.L2), w, v);
         }
-        const T q = v * cur.ic;
+        q = v * cur.ic;
+        }
         P.out[(myTile * SL + slot_of(s)) * 64 + lane] = q;
         if (oJ || oK) {
             const int row = (int)((myTile * SL + s) * ROWB);
@@ -919,6 +954,62 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
         step(s, ra, rb);
         if (s + 1 < nS) step(s + 1, rb, ra);
     }
+}
+
+// MIC(0) records (k_sweep_w8<double, true, true>): {cL0, cL1, cL2, diag, cM0, cM1,
+// cM2, 0} per element, directions -k, -j, -i; cM sums the neighbour's
+// coefficients in the two directions k_mic_plane drops (exact halves, so
+// cM * inv rounds like the sum of the two products).
+__global__ void k_mic_records(Geo g, int ntk, long n, const uint16_t* __restrict__ code, double* __restrict__ rec) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    int i, j, k;
+    w8_cell(g, ntk, e, &i, &j, &k);
+    double cL[3] = {0, 0, 0}, cM[3] = {0, 0, 0}, diag = 0;
+    if (j < g.nj && k < g.nk && i >= 0 && i < g.ni) {
+        const long c = cidx(g, i, j, k);
+        const uint16_t cc = code[c];
+        diag = code_diag(cc);
+        const int dF[3] = {D_ZM, D_YM, D_XM};
+        const int ma[3] = {D_XP, D_XP, D_YP}, mb[3] = {D_YP, D_ZP, D_ZP};
+        const long oF[3] = {-g.nn, -(long)g.ni, -1};
+        const bool inF[3] = {k > 0, j > 0, i > 0};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int cl = code_dir(cc, dF[d]);
+            if ((cl == 1 || cl == 2) && inF[d]) {
+                cL[d] = code_coef(cl);
+                const uint16_t nc = code[c + oF[d]];
+                const int a = code_dir(nc, ma[d]), b = code_dir(nc, mb[d]);
+                double m = 0.0;
+                if (a == 1 || a == 2) m += code_coef(a);
+                if (b == 1 || b == 2) m += code_coef(b);
+                cM[d] = m;
+            }
+        }
+    }
+    double* r = rec + 8 * e;
+    r[0] = cL[0]; r[1] = cL[1]; r[2] = cL[2]; r[3] = diag;
+    r[4] = cM[0]; r[5] = cM[1]; r[6] = cM[2]; r[7] = 0.0;
+}
+
+bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const SweepSkew<double>& sk,
+             int* flags, unsigned long long* epoch, hipStream_t st) {
+    int ntj, ntk;
+    w8_tiles(g, &ntj, &ntk);
+    const long n = sweep_w8_size(g);
+    const long ebytes = (long)ntj * ntk * w8_slots(g) * 16 * 16;
+    if (!flags || !epoch || !sk.oRec || ebytes >= 0x7fffffffL) return false;
+    k_mic_records<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, sk.oRec);
+    const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
+    const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
+    const uint32_t tag = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
+    const W8Planes<double> P{nullptr, sk.oQ, sk.oRec, sk.oEdge};
+    k_sweep_w8<double, true, true><<<(J1 - J0 + 1) * (K1 - K0 + 1), 192, 0, st>>>(
+        g, ntk, J0, K0, J1 - J0 + 1, K1 - K0 + 1, s0, s1 - s0 + 1, P, nullptr, tag, flags, tau, sigma);
+    const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
+    k_from_oct<double><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oQ, inv, nullptr, bx);
+    return true;
 }
 
 template <class T>
