@@ -44,6 +44,7 @@ struct SweepSkew {
     T* oRec;
     T *oR, *oQ, *oZ;
     uint32_t* oEdge;
+    mutable uint32_t w8_launch;  // launches on oEdge (their tags)
     // bounding box of the active rows (i0, i1, j0, j1, k0, k1; host copy), see pcg_active_box
     int box[6];
 };

@@ -515,7 +515,7 @@ template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, con
 // CU fetches only some 20-25 GB/s from memory, so the step of a 16x16 tile is
 // bound by its bytes rather than its arithmetic.  Tiles of 8x8 lines (one cell
 // per lane) put four times as many CUs on the same grid with a quarter of the
-// bytes per step each.  A step's records arrive W8_LEAD steps ahead of the
+// bytes per step each.  A step's records arrive LEAD steps ahead of the
 // compute wave through a record wave; an edge wave follows the upstream edge
 // words one step ahead (every step of lead would be paid again on each
 // tile-to-tile hop).  The two are separate waves because a wave's vector-memory
@@ -530,15 +530,10 @@ template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, con
 // entry present).  Edge rows: 16 words per tile and step, 0-7 the edge column
 // by kl, 8-15 the edge row by jl, whose corner cell is not repeated there
 // (readers take it from the column word).  An edge word's tag is the sweep's
-// tag base with the producing step in its low 8 bits, so a ring slot's copy of
-// an older step never passes for the current one.
+// tag base plus the producing step (w8_next_tag), so a ring slot's copy of an
+// older step, or a word of an earlier launch, never passes for the current one.
 #define W8_PAD 14
-#ifndef W8_R
-#define W8_R 16     // LDS ring depth in steps
-#endif
-#ifndef W8_LEAD
-#define W8_LEAD 12  // steps of record loads in flight
-#endif
+#define W8_B 16  // steps per unrolled block of the compute wave
 
 
 static inline void w8_tiles(const Geo& g, int* ntj, int* ntk) {
@@ -546,6 +541,8 @@ static inline void w8_tiles(const Geo& g, int* ntj, int* ntk) {
     *ntk = (g.nk + 7) / 8;
 }
 __host__ __device__ inline int w8_slots(const Geo& g) { return g.ni + W8_PAD; }
+// edge rows per tile: one per step and 7 written after the last step (+1 spare)
+__host__ __device__ inline int w8_erows(const Geo& g) { return w8_slots(g) + 8; }
 long sweep_w8_size(const Geo& g) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
@@ -554,7 +551,7 @@ long sweep_w8_size(const Geo& g) {
 long sweep_w8_edge_words(const Geo& g) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
-    return (long)ntj * ntk * w8_slots(g) * 16 * 4;  // u32 units, sized for FP64 words
+    return (long)ntj * ntk * w8_erows(g) * 16 * 4;  // u32 units, sized for FP64 words
 }
 
 __device__ inline void w8_cell(const Geo& g, int ntk, long e, int* i, int* j, int* k) {
@@ -647,21 +644,37 @@ __device__ inline int w8_kword(int jl) {
     return jl == CJ ? CJ : 8 + jl;
 }
 // wait until ring slot s is free (step s - R consumed); cons = last count seen
+template <int R>
 __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) {
-    if (s < W8_R) return;
+    if (s < R) return;
     int spins = 0;
-    while (cons < s - W8_R + 1) {
+    while (cons < s - R + 1) {
         cons = lds_ld32(consA);
-        if (cons >= s - W8_R + 1) break;
+        if (cons >= s - R + 1) break;
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > WV_SPIN) { atomicOr(err, 4); cons = s; }
+        if ((++spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) cons = s;
+        else if (spins > WV_SPIN) { atomicOr(err, 4); cons = s; }
     }
 }
 
 // Runs the tiles [J0, J0 + ntjb) x [K0, K0 + ntkb) of the ntk-wide tile grid over
 // the slots [s0, s0 + nS): the tiles and slots that hold active rows (all other
 // rows are empty and their results zero).  Step t works on slot s0 + t forward,
-// s0 + nS - 1 - t backward; edge rows are indexed by step.
+// s0 + nS - 1 - t backward; edge rows are indexed by step.  After its last step
+// a tile writes 7 more edge rows of zeros (steps nS .. nS + 6): a downstream
+// tile's last 7 steps wait for those, for cells outside the box whose entries
+// are absent, so every step checks its edge words the same way.
+//
+// The ring holds R steps, the records of LEAD steps in flight (32 and 24 or
+// 20 for the sweeps: some 4 us of lead at 0.16 us a step; 16 and 12 for the
+// pivots, whose records are twice as long).  The compute wave runs in blocks
+// of W8_B steps unrolled so that every ring slot sits at a fixed LDS offset
+// from the block's base.  With one wave per SIMD a step costs about
+// its instruction count times the issue interval, so the step is kept short:
+// lanes without an upstream edge read the slot's dummy word (value 0, the
+// step's tag), which makes the tag check two compares for every lane; the
+// record count is checked every 4 steps and the consumed count published every
+// 4 steps (which needs R - LEAD >= 4 slots of slack).
 //
 // MIC: the same pipeline computes the MIC(0) pivots (factor_modified_incomplete_
 // cholesky0, pcg_solver.h:85-186, in the order of k_mic_plane): the value
@@ -676,15 +689,20 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
                                                   double tau = 0.0, double sigma = 0.0) {
     static_assert(!MIC || (FWD && sizeof(T) == 8), "pivots: forward, FP64");
     constexpr int NS = sizeof(T) / 4;
+    constexpr int R = MIC ? 16 : 32;                      // ring depth in steps
+    constexpr int LEAD = MIC ? 12 : NS == 1 ? 24 : 20;    // steps of record loads in flight
+    static_assert((R & (R - 1)) == 0 && R % W8_B == 0 && LEAD + 4 <= R, "ring depth");
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
     constexpr int RECB = 64 * (MIC ? 8 : 4) * sizeof(T);  // record bytes per step (1, 2 or 4 KB)
     constexpr int INB = MIC ? 0 : 64 * sizeof(T);         // input bytes per step
     constexpr int EOFF = RECB + INB;             // upstream edge rows (j, then k) within a slot
-    constexpr int SLOT = EOFF + 2 * ROWB;
+    constexpr int DOFF = EOFF + 2 * ROWB;        // the slot's dummy word
+    constexpr int EB = 2 * ROWB + 16;            // edge bytes per slot
+    constexpr int SLOT = EOFF + EB;
     constexpr int PERD = RECB / 1024 + (INB ? 1 : 0);  // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
-    __shared__ __attribute__((aligned(16))) unsigned char ring_[W8_R * SLOT];
+    __shared__ __attribute__((aligned(16))) unsigned char ring_[R * SLOT];
     __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
     if (S && S->done) return;
     int J, K;
@@ -692,7 +710,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     J += J0;
     K += K0;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int SL = w8_slots(g);  // slots per tile in the layout
+    const int SL = w8_slots(g), ER = w8_erows(g);
     const long myTile = (long)J * ntk + K;
     const int uJ = FWD ? J - 1 : J + 1, uK = FWD ? K - 1 : K + 1;
     const bool hasJ = uJ >= J0 && uJ < J0 + ntjb, hasK = uK >= K0 && uK < K0 + ntkb;
@@ -701,25 +719,34 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     lds_u8* ring = (lds_u8*)ring_;
     lds_int* ctr = (lds_int*)ctr_;
     const uint32_t ctrA = (uint32_t)(uintptr_t)ctr, ringA = (uint32_t)(uintptr_t)ring;
-    // tag of the word an upstream tile stored at its step u
-    auto tag_of = [tagbase](int u) __attribute__((always_inline)) { return tagbase | (uint32_t)(u & 0xff); };
     if (threadIdx.x < 2) ctr[threadIdx.x] = 0;
-    // the edge words of every slot start out invalid (tags are never zero)
-    for (int b = threadIdx.x * 4; b < W8_R * 2 * ROWB; b += 192 * 4)
-        *(__attribute__((address_space(3))) uint32_t*)(ring + (b / (2 * ROWB)) * SLOT + EOFF + b % (2 * ROWB)) = 0u;
+    // the edge and dummy words of every slot start out invalid (tags are never zero)
+    for (int b = threadIdx.x * 4; b < R * EB; b += 192 * 4)
+        *(__attribute__((address_space(3))) uint32_t*)(ring + (b / EB) * SLOT + EOFF + b % EB) = 0u;
     __syncthreads();
 
     if (wid == 1) {
         // ------------------------------- record wave -------------------------------
-        // records and input of step x + LEAD issued at iteration x (LDS-DMA into
-        // the ring), step x waited for and published then
+        // at iteration x: step x waited for and published, then the records and
+        // input of step x + LEAD issued (LDS-DMA into the ring).  Publishing
+        // first keeps the count independent of the slot that step x + LEAD
+        // waits for: the compute wave asks for up to 5 steps ahead of the last
+        // count it published, with R - LEAD >= 4 slots of slack.
         int cons_seen = 0;
-        for (int x = -W8_LEAD; x < nS; ++x) {
-            const int s = x + W8_LEAD;
+        for (int x = -LEAD; x < nS; ++x) {
+            if (x >= 0) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                // (steps x + 1 .. x + LEAD - 1 may stay in flight)
+                if (x + LEAD - 1 < nS) vm_wait<(LEAD - 1) * PERD>();
+                else vm_wait<0>();
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                lds_st32(ctrA, x + 1);
+            }
+            const int s = x + LEAD;
             if (s < nS) {
-                w8_wait_slot(s, cons_seen, ctrA + 4, err);
+                w8_wait_slot<R>(s, cons_seen, ctrA + 4, err);
                 const long e0 = (myTile * SL + slot_of(s)) * 64;
-                lds_u8* dst = ring + (s % W8_R) * SLOT;
+                lds_u8* dst = ring + (s % R) * SLOT;
 #pragma unroll
                 for (int h = 0; h < RECB / 1024; ++h)
                     __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + (MIC ? 8 : 4) * e0) + h * 1024 +
@@ -728,13 +755,9 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
                 if (INB && lane < INB / 16)
                     __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + e0) + lane * 16, dst + RECB, 16, 0, 0);
             }
-            if (x < 0) continue;
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (x + W8_LEAD < nS) vm_wait<W8_LEAD * PERD>();
-            else vm_wait<0>();
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            lds_st32(ctrA, x + 1);
         }
+        // (the last step reads "next" records from slot nS, which nothing uses)
+        lds_st32(ctrA, nS + 1);
         return;
     }
     if (wid == 2) {
@@ -743,15 +766,16 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
         // column word kl, 8-15: the k-upstream's row word for jl): the word of
         // step x + 1 is loaded while step x is checked; a stale word (older tag)
         // is re-read until it carries the tag of its step, then copied into the
-        // ring slot, where the compute wave checks the tag once more.
-        const bool mine = lane < 16, sideJ = lane < 8;
+        // ring slot, where the compute wave checks the tag once more.  Lane 16
+        // writes the slot's dummy word.
+        const bool mine = lane < 16, sideJ = lane < 8, dummy = lane == 16;
         const bool has = mine && (sideJ ? hasJ : hasK);
         const long ut = has ? (sideJ ? tJ : tK) : myTile;
         const int w = mine ? (sideJ ? lane : w8_kword<FWD>(lane - 8)) : 0;
-        const int off = EOFF + (sideJ ? 0 : ROWB) + w * EWB;
+        const int off = dummy ? DOFF : EOFF + (sideJ ? 0 : ROWB) + w * EWB;
+        // the upstream's word of its step x + 7 (its rows run to step nS + 6)
         auto src = [&](int x) __attribute__((always_inline)) {
-            const int us = x + 7 < nS ? x + 7 : nS - 1;
-            return (const uint64_t*)((const unsigned char*)P.edge + (ut * SL + us) * ROWB + w * EWB);
+            return (const uint64_t*)((const unsigned char*)P.edge + (ut * ER + x + 7) * ROWB + w * EWB);
         };
         int cons_seen = 0;
         // one step: load the words of step x + 1 into (n0, n1) while checking and
@@ -761,8 +785,8 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
             const int xn = x + 1 < nS ? x + 1 : x;
             n0 = __hip_atomic_load(src(xn), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (NS == 2) n1 = __hip_atomic_load(src(xn) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t want = tag_of(x + 7);
-            if (has && x + 7 < nS) {
+            const uint32_t want = tagbase + (uint32_t)(x + 7);
+            if (has && x < nS) {
                 int spins = 0;
                 while (!((uint32_t)(c0 >> 32) == want && (NS == 1 || (uint32_t)(c1 >> 32) == want))) {
                     if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
@@ -772,11 +796,12 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
                     if (NS == 2) c1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
-            w8_wait_slot(x, cons_seen, ctrA + 4, err);
-            if (mine && x < nS) {
-                const uint32_t ea = ringA + (x % W8_R) * SLOT + off;
-                asm volatile("ds_write_b64 %0, %1" ::"v"(ea), "v"(c0) : "memory");
-                if (NS == 2) asm volatile("ds_write_b64 %0, %1 offset:8" ::"v"(ea), "v"(c1) : "memory");
+            w8_wait_slot<R>(x, cons_seen, ctrA + 4, err);
+            if ((mine || dummy) && x < nS) {
+                const uint64_t d = (uint64_t)want << 32;
+                const uint32_t ea = ringA + (x % R) * SLOT + off;
+                asm volatile("ds_write_b64 %0, %1" ::"v"(ea), "v"(dummy ? d : c0) : "memory");
+                if (NS == 2) asm volatile("ds_write_b64 %0, %1 offset:8" ::"v"(ea), "v"(dummy ? d : c1) : "memory");
             }
         };
         uint64_t a0 = __hip_atomic_load(src(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), a1 = 0, b0 = 0, b1 = 0;
@@ -796,33 +821,45 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     const bool eK = FWD ? kl == 0 : kl == 7;  // k-neighbour in the upstream tile
     const bool oJ = jl == CJ;                 // edge column (read by the j-downstream tile)
     const bool oK = kl == (FWD ? 7 : 0);      // edge row (read by the k-downstream tile)
-    const bool nJ = eJ && hasJ, nK = eK && hasK;  // the edge values this lane uses
     const int bp = ((lane + (FWD ? 56 : 8)) & 63) * 4;  // lane -8 / +8 (edge lanes are overridden)
-    const int ejo = EOFF + kl * EWB, eko = EOFF + ROWB + w8_kword<FWD>(jl) * EWB;
+    // each lane reads two edge words per step: its upstream words where it has
+    // them, the dummy word otherwise
+    const int ejo = eJ && hasJ ? EOFF + kl * EWB : DOFF;
+    const int eko = eK && hasK ? EOFF + ROWB + w8_kword<FWD>(jl) * EWB : DOFF;
     const int eout = oJ ? kl * EWB : (8 + jl) * EWB;
-    const __amdgpu_buffer_rsrc_t erc = __builtin_amdgcn_make_buffer_rsrc(P.edge, (short)0, 0x7fffffff, 0x00020000);
-    typedef __attribute__((address_space(3))) const T ldsT;
-    typedef __attribute__((address_space(3))) const uint64_t ldsW;
+    // this tile's output plane and edge rows (offsets in bytes from the tile's start)
+    const __amdgpu_buffer_rsrc_t orc =
+        __builtin_amdgcn_make_buffer_rsrc(P.out + myTile * SL * 64, (short)0, SL * 64 * (int)sizeof(T), 0x00020000);
+    const __amdgpu_buffer_rsrc_t erc = __builtin_amdgcn_make_buffer_rsrc(
+        (unsigned char*)P.edge + myTile * ER * ROWB, (short)0, ER * ROWB, 0x00020000);
     typedef T T2v __attribute__((ext_vector_type(2)));
     typedef T T4v __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) const uint64_t ldsW;
+    typedef __attribute__((address_space(3))) const wv_u32x4 ldsW4;
+    typedef __attribute__((address_space(3))) const T ldsT;
     struct Rec { T L0, L1, L2, ic, rin, M0, M1, M2; };
     int seen = 0;
-    // wait until the record wave has published step s
-    auto need_rec = [&](int s) __attribute__((always_inline)) {
-        if (seen < s + 1) {
+    // wait until the record wave has published `upto` steps
+    auto need_rec = [&](int upto) __attribute__((always_inline)) {
+        if (seen < upto) {
             int spins = 0;
             for (;;) {
                 seen = __builtin_amdgcn_readfirstlane(lds_ld32(ctrA));
-                if (seen >= s + 1) break;
+                if (seen >= upto) break;
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > 4 * WV_SPIN) { atomicOr(err, 8); seen = nS; break; }
+                // (another wave's failed wait ends this one too)
+                if ((++spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    seen = nS + 1;
+                    break;
+                }
+                if (spins > 4 * WV_SPIN) { atomicOr(err, 8); seen = nS + 1; break; }
             }
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
     };
-    // records + input of step s from its ring slot
-    auto load_rec = [&](int s, Rec& R) __attribute__((always_inline)) {
-        const lds_u8* sl = ring + (s % W8_R) * SLOT;
+    // records + input of the step in ring slot `slot`
+    auto load_rec = [&](int slot, Rec& R) __attribute__((always_inline)) {
+        const lds_u8* sl = ring + slot * SLOT;
         if constexpr (MIC) {
             // (diag in the ic field)
             const T2v a = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 64);
@@ -841,62 +878,56 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
             R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
         }
     };
+    // an edge word as read from the ring: FP32 {value, tag}, FP64 {lo, tag, hi, tag}
+    auto ew_bad = [](wv_u32x4 w, uint32_t want) __attribute__((always_inline)) {
+        return NS == 1 ? w.y != want : (w.y != want) | (w.w != want);
+    };
+    auto ew_val = [](wv_u32x4 w) __attribute__((always_inline)) {
+        if constexpr (NS == 1) return (T)__uint_as_float(w.x);
+        else return (T)__longlong_as_double((long long)(((uint64_t)w.z << 32) | w.x));
+    };
+    auto ew_read = [&](const lds_u8* a) __attribute__((always_inline)) {
+        wv_u32x4 w;
+        if constexpr (NS == 1) {
+            const uint64_t v = *(ldsW*)a;
+            w = wv_u32x4{(uint32_t)v, (uint32_t)(v >> 32), 0u, 0u};
+        } else {
+            w = *(ldsW4*)a;
+        }
+        return w;
+    };
     T qp = 0;
-    // One step.  The neighbour exchange of the previous step's results, the edge
-    // words and the next step's records are all requested before the first
+    // One step.  The edge words, the neighbour exchange of the previous step's
+    // results and the next step's records are all requested before the first
     // wait, so their latencies overlap; then the edge tags are checked and the
     // dependent chain runs on records read one step earlier.
-    auto step = [&](int s, Rec& cur, Rec& nxt) __attribute__((always_inline)) {
-        const int sn = s + 1 < nS ? s + 1 : s;  // (the last step re-reads its own records)
-        need_rec(sn);
-        const lds_u8* sl = ring + (s % W8_R) * SLOT;
-        // upstream edge values of this step: wait (bounded) until the edge wave
-        // has placed words with this step's tag (value and tag of a word come
-        // from one 8-byte read: read apart, the edge wave could replace the word
-        // between the two)
-        const bool live = s + 7 < nS;
-        const uint32_t want = tag_of(s + 7);
-        const uint64_t wj0 = *(ldsW*)(sl + ejo), wk0 = *(ldsW*)(sl + eko);
-        const uint64_t wj1 = NS == 2 ? *(ldsW*)(sl + ejo + 8) : wj0, wk1 = NS == 2 ? *(ldsW*)(sl + eko + 8) : wk0;
-        // issue order: edge words, exchange, next step's records (the waits that
-        // follow are in that order, and each leaves the later requests in flight)
+    auto step = [&](int s, int slot, Rec& cur, Rec& nxt) __attribute__((always_inline)) {
+        const lds_u8* sl = ring + slot * SLOT;
+        const uint32_t want = tagbase + (uint32_t)(s + 7);
+        wv_u32x4 wj = ew_read(sl + ejo), wk = ew_read(sl + eko);
         __builtin_amdgcn_sched_barrier(0);
         T qj = wv_shift<FWD>(qp);
         T qk = wv_perm(bp, qp);
         __builtin_amdgcn_sched_barrier(0);
-        load_rec(sn, nxt);
+        load_rec((slot + 1) & (R - 1), nxt);
         __builtin_amdgcn_sched_barrier(0);
-        T ej, ek;
-        if constexpr (NS == 1) {
-            ej = __uint_as_float((uint32_t)wj0);
-            ek = __uint_as_float((uint32_t)wk0);
-        } else {
-            ej = __longlong_as_double((long long)((wj1 << 32) | (wj0 & 0xffffffffull)));
-            ek = __longlong_as_double((long long)((wk1 << 32) | (wk0 & 0xffffffffull)));
-        }
-        bool okJ = !(nJ && live) || ((uint32_t)(wj0 >> 32) == want && (uint32_t)(wj1 >> 32) == want);
-        bool okK = !(nK && live) || ((uint32_t)(wk0 >> 32) == want && (uint32_t)(wk1 >> 32) == want);
-        if (__builtin_amdgcn_ballot_w64(!(okJ && okK))) {
+        if (__builtin_amdgcn_ballot_w64(ew_bad(wj, want) | ew_bad(wk, want))) {
+            // not yet placed by the edge wave: re-read (bounded) until the tags match
             int spins = 0;
-            const uint32_t aj = ringA + (s % W8_R) * SLOT + ejo, ak = ringA + (s % W8_R) * SLOT + eko;
-            while (__builtin_amdgcn_ballot_w64(!(okJ && okK))) {
+            const uint32_t aj = ringA + slot * SLOT + ejo, ak = ringA + slot * SLOT + eko;
+            do {
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > 4 * WV_SPIN) { atomicOr(err, 16); break; }
-                const uint64_t wj = lds_ld64(aj), wk = lds_ld64(ak);
-                const uint64_t hj = NS == 2 ? lds_ld64(aj + 8) : wj, hk = NS == 2 ? lds_ld64(ak + 8) : wk;
-                okJ = !(nJ && live) || ((uint32_t)(wj >> 32) == want && (uint32_t)(hj >> 32) == want);
-                okK = !(nK && live) || ((uint32_t)(wk >> 32) == want && (uint32_t)(hk >> 32) == want);
-                if constexpr (NS == 1) {
-                    ej = __uint_as_float((uint32_t)wj);
-                    ek = __uint_as_float((uint32_t)wk);
-                } else {
-                    ej = __longlong_as_double((long long)((hj << 32) | (wj & 0xffffffffull)));
-                    ek = __longlong_as_double((long long)((hk << 32) | (wk & 0xffffffffull)));
-                }
-            }
+                if ((++spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (spins > 4 * WV_SPIN) { atomicOr(err, 16); break; }
+                // (each 8-byte read holds a value half with its own tag)
+                const uint64_t j0 = lds_ld64(aj), k0 = lds_ld64(ak);
+                const uint64_t j1 = NS == 2 ? lds_ld64(aj + 8) : 0, k1 = NS == 2 ? lds_ld64(ak + 8) : 0;
+                wj = wv_u32x4{(uint32_t)j0, (uint32_t)(j0 >> 32), (uint32_t)j1, (uint32_t)(j1 >> 32)};
+                wk = wv_u32x4{(uint32_t)k0, (uint32_t)(k0 >> 32), (uint32_t)k1, (uint32_t)(k1 >> 32)};
+            } while (__builtin_amdgcn_ballot_w64(ew_bad(wj, want) | ew_bad(wk, want)));
         }
-        qj = eJ ? ej : qj;
-        qk = eK ? ek : qk;
+        qj = eJ ? ew_val(wj) : qj;
+        qk = eK ? ew_val(wk) : qk;
         T v = cur.rin, w, q;
         if constexpr (MIC) {
             // d = diag - sum over the -k, -j, -i entries of L^2 + tau L miss
@@ -913,46 +944,80 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
             d -= (T)tau * L * miss;
             q = diag == 0 ? (T)0 : (d < (T)sigma * diag) ? 1 / sqrt(diag) : 1 / sqrt(d);
         } else {
-        if (FWD) {
-            // column order c-nn, c-ni, c-1
-            w = v - cur.L0 * qk; v = wv_sel<T>(w8_present(cur.L0), w, v);
-            w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
-            w = v - cur.L2 * qp; v = wv_sel<T>(w8_present(cur.L2), w, v);
+            if (FWD) {
+                // column order c-nn, c-ni, c-1
+                w = v - cur.L0 * qk; v = wv_sel<T>(w8_present(cur.L0), w, v);
+                w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
+                w = v - cur.L2 * qp; v = wv_sel<T>(w8_present(cur.L2), w, v);
+            } else {
+                // row order c+1, c+ni, c+nn
+                w = v - cur.L0 * qp; v = wv_sel<T>(w8_present(cur.L0), w, v);
+                w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
+                w = v - cur.L2 * qk; v = wv_sel<T>(w8_present(cur.L2), w, v);
+            }
+            q = v * cur.ic;
+        }
+        const int so = slot_of(s) * 64 * (int)sizeof(T);
+        if constexpr (NS == 1) {
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint((float)q), orc, lane * 4, so, 0);
         } else {
-            // row order c+1, c+ni, c+nn
-            w = v - cur.L0 * qp; v = wv_sel<T>(w8_present(cur.L0), w, v);
-            w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
-            w = v - cur.L2 * qk; v = wv_sel<T>(w8_present(cur.L2), w, v);
+            const uint64_t u = (uint64_t)__double_as_longlong((double)q);
+            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)u, (uint32_t)(u >> 32)}, orc, lane * 8, so, 0);
         }
-        q = v * cur.ic;
-        }
-        P.out[(myTile * SL + slot_of(s)) * 64 + lane] = q;
         if (oJ || oK) {
-            const int row = (int)((myTile * SL + s) * ROWB);
-            const uint32_t t = tag_of(s);
+            const uint32_t t = tagbase + (uint32_t)s;
             if constexpr (NS == 1) {
-                const wv_u32x2 a = {__float_as_uint((float)q), t};
-                __builtin_amdgcn_raw_buffer_store_b64(a, erc, row + eout, 0, WV_SC1);
+                __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{__float_as_uint((float)q), t}, erc, eout, s * ROWB, WV_SC1);
             } else {
                 const uint64_t u = (uint64_t)__double_as_longlong((double)q);
-                const wv_u32x4 a = {(uint32_t)u, t, (uint32_t)(u >> 32), t};
-                __builtin_amdgcn_raw_buffer_store_b128(a, erc, row + eout, 0, WV_SC1);
+                __builtin_amdgcn_raw_buffer_store_b128(wv_u32x4{(uint32_t)u, t, (uint32_t)(u >> 32), t}, erc, eout,
+                                                       s * ROWB, WV_SC1);
             }
         }
         qp = q;
+    };
+    // steps consumed: their slots may be refilled (their reads are issued, and a
+    // wave's LDS operations complete in order)
+    auto publish = [&](int n) __attribute__((always_inline)) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        lds_st32_nowait(ctrA + 4, s + 1);  // step s consumed (step s + 1's reads are issued, in order)
+        lds_st32_nowait(ctrA + 4, n);
     };
     Rec ra, rb;
-    need_rec(0);
+    need_rec(1);
     load_rec(0, ra);
     // (settle every load issued so far, kernel arguments included, so the step
     // loop's waits only ever concern its own requests)
     __builtin_amdgcn_s_waitcnt(0);
+    int s = 0;
     // two record sets alternate (no register copies between steps)
-    for (int s = 0; s < nS; s += 2) {
-        step(s, ra, rb);
-        if (s + 1 < nS) step(s + 1, rb, ra);
+    for (; s + W8_B <= nS; s += W8_B) {
+        const int h = s & (R - 1);  // the block's first slot
+#pragma unroll
+        for (int u = 0; u < W8_B; u += 4) {
+            need_rec(s + u + 5 < nS + 1 ? s + u + 5 : nS + 1);
+            step(s + u, h + u, ra, rb);
+            step(s + u + 1, h + u + 1, rb, ra);
+            step(s + u + 2, h + u + 2, ra, rb);
+            step(s + u + 3, h + u + 3, rb, ra);
+            publish(s + u + 4);
+        }
+    }
+    for (; s < nS; s += 2) {
+        need_rec(s + 3 < nS + 1 ? s + 3 : nS + 1);
+        step(s, s & (R - 1), ra, rb);
+        if (s + 1 < nS) step(s + 1, (s + 1) & (R - 1), rb, ra);
+        publish(s + 2 < nS ? s + 2 : nS);
+    }
+    // edge rows of the steps after the last: zeros, for the downstream tiles' last 7 steps
+    if (oJ || oK) {
+        for (int d = 0; d < 7; ++d) {
+            const uint32_t t = tagbase + (uint32_t)(nS + d);
+            if constexpr (NS == 1) {
+                __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{0u, t}, erc, eout, (nS + d) * ROWB, WV_SC1);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b128(wv_u32x4{0u, t, 0u, t}, erc, eout, (nS + d) * ROWB, WV_SC1);
+            }
+        }
     }
 }
 
@@ -993,17 +1058,38 @@ __global__ void k_mic_records(Geo g, int ntk, long n, const uint16_t* __restrict
     r[4] = cM[0]; r[5] = cM[1]; r[6] = cM[2]; r[7] = 0.0;
 }
 
+__global__ void k_w8_clear(long n, uint32_t* __restrict__ p) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e < n) p[e] = 0u;
+}
+
+// Tag base of the next launch on this edge buffer: bit 31 set, a 15-bit launch
+// count in bits 16-30 and the step in the low 16 bits.  When the count wraps
+// the buffer is cleared, so no word left from 32768 launches earlier can carry
+// a current tag.
+template <class T>
+static bool w8_next_tag(const Geo& g, const SweepSkew<T>& sk, uint32_t* tag, hipStream_t st) {
+    if (w8_erows(g) >= 0xffff) return false;
+    const uint32_t n = sk.w8_launch++ & 0x7fffu;
+    if (n == 0 && sk.w8_launch > 1) {
+        const long words = sweep_w8_edge_words(g);
+        k_w8_clear<<<xf_blocks(words), XF_BLOCK, 0, st>>>(words, sk.oEdge);
+    }
+    *tag = 0x80000000u | (n << 16);
+    return true;
+}
+
 bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const SweepSkew<double>& sk,
              int* flags, unsigned long long* epoch, hipStream_t st) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
-    const long ebytes = (long)ntj * ntk * w8_slots(g) * 16 * 16;
-    if (!flags || !epoch || !sk.oRec || ebytes >= 0x7fffffffL) return false;
+    const long ebytes = (long)ntj * ntk * w8_erows(g) * 16 * 16;
+    uint32_t tag;
+    if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, sk, &tag, st)) return false;
     k_mic_records<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, sk.oRec);
     const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
     const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
-    const uint32_t tag = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
     const W8Planes<double> P{nullptr, sk.oQ, sk.oRec, sk.oEdge};
     k_sweep_w8<double, true, true><<<(J1 - J0 + 1) * (K1 - K0 + 1), 192, 0, st>>>(
         g, ntk, J0, K0, J1 - J0 + 1, K1 - K0 + 1, s0, s1 - s0 + 1, P, nullptr, tag, flags, tau, sigma);
@@ -1026,14 +1112,14 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
-    const long ebytes = (long)ntj * ntk * w8_slots(g) * 16 * 8 * (long)(sizeof(T) / 4);
-    if (!flags || !epoch || !sk.oRec || ebytes >= 0x7fffffffL) return false;
+    const long ebytes = (long)ntj * ntk * w8_erows(g) * 16 * 8 * (long)(sizeof(T) / 4);
+    uint32_t tag1, tag2;
+    if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, sk, &tag1, st) ||
+        !w8_next_tag(g, sk, &tag2, st))
+        return false;
     k_to_oct<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, r, sk.oR, S);
     const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
     const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
-    // per-sweep tag base; the low 8 bits carry the step of each edge word
-    const uint32_t tag1 = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
-    const uint32_t tag2 = 0x80000000u | (uint32_t)(((*epoch)++ & 0x7fffffull) << 8);
     const W8Planes<T> f{sk.oR, sk.oQ, sk.oRec, sk.oEdge};
     const W8Planes<T> b{sk.oQ, sk.oZ, sk.oRec + 4 * n, sk.oEdge};
     const int ntjb = J1 - J0 + 1, ntkb = K1 - K0 + 1, nS = s1 - s0 + 1;
