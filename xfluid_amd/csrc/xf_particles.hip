@@ -329,14 +329,16 @@ __global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, 
     out[t] = wsum > 0.0000001 ? vsum / wsum : 0.0;
 }
 
-// The same gather over bricks of 8x4x4 faces.  A workgroup walks the brick's
-// candidate cells one k-slab at a time: it copies the slab's particles (one
-// contiguous run per cell row of the sorted arrays) into LDS, then every face
-// adds the slab's rows inside its support, reading particles from LDS.  Each
-// face still visits cells in k, j, i order and particles by index, with the
-// per-pair arithmetic of k_p2g, so the result is bit-identical to it; each
-// particle is fetched once per brick instead of once per face.  A slab with
-// more particles than fit is read from global memory in the same order.
+// The same gather over bricks of 8x4x4 faces.  A workgroup first reads the
+// cell starts of all its candidate slabs (one round of loads), then walks the
+// k-slabs that hold particles: it copies the slab's particles (one contiguous
+// run per cell row of the sorted arrays) into LDS with several loads in flight
+// per thread, then every face adds the slab's rows inside its support, reading
+// particles from LDS.  Each face still visits cells in k, j, i order and
+// particles by index, with the per-pair arithmetic of k_p2g, so the result is
+// bit-identical to it; each particle is fetched once per brick instead of once
+// per face, and bricks in air cost one round of loads.  A slab with more
+// particles than fit is read from global memory in the same order.
 #define P2G_BI 8
 #define P2G_BJ 4
 #define P2G_BK 4
@@ -347,9 +349,8 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
     double* __restrict__ out) {
     __shared__ double lx[P2G_CAP], ly[P2G_CAP], lz[P2G_CAP], lv[P2G_CAP];
-    __shared__ int raw[P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slab's cells, per row
-    __shared__ int roff[P2G_BJ + 3];                // LDS offset of each slab row
-    __shared__ int nslab;
+    __shared__ int raw[P2G_BK + 3][P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slabs' cells, per row
+    __shared__ int roff[P2G_BK + 3][P2G_BJ + 4];               // LDS offset of each slab row, then the slab's total
     const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const int bi = (int)(b % nbi), bj = (int)((b / nbi) % nbj), bk = (int)(b / ((long)nbi * nbj));
@@ -392,29 +393,47 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
             vsum += v * cw;
         }
     };
-    const int nrow = chi[1] - clo[1] + 1, ncell = chi[0] - clo[0] + 1;
-    for (int k = clo[2]; k <= chi[2]; ++k) {
-        // slab k: the cell starts of its rows, the rows' LDS offsets, then the particles
-        if (tid < nrow * (ncell + 1)) {
-            const int jr = tid / (ncell + 1), ic = tid % (ncell + 1);
-            raw[jr][ic] = start[(long)g.ni * (clo[1] + jr) + g.nn * k + clo[0] + ic];
+    const int nrow = chi[1] - clo[1] + 1, ncell = chi[0] - clo[0] + 1, nsl = chi[2] - clo[2] + 1;
+    // the cell starts of every candidate slab row
+    const int per = nrow * (ncell + 1);
+    for (int q = tid; q < nsl * per; q += blockDim.x) {
+        const int ks = q / per, rem = q - ks * per, jr = rem / (ncell + 1), ic = rem - jr * (ncell + 1);
+        raw[ks][jr][ic] = start[(long)g.ni * (clo[1] + jr) + g.nn * (clo[2] + ks) + clo[0] + ic];
+    }
+    __syncthreads();
+    if (tid < nsl) {
+        int acc = 0;
+        for (int jr = 0; jr < nrow; ++jr) {
+            roff[tid][jr] = acc;
+            acc += raw[tid][jr][ncell] - raw[tid][jr][0];
         }
-        __syncthreads();
-        if (tid == 0) {
-            int acc = 0;
-            for (int jr = 0; jr < nrow; ++jr) {
-                roff[jr] = acc;
-                acc += raw[jr][ncell] - raw[jr][0];
-            }
-            nslab = acc;
-        }
-        __syncthreads();
-        const bool fits = nslab <= P2G_CAP;
+        roff[tid][nrow] = acc;
+    }
+    __syncthreads();
+    constexpr int NB = P2G_BI * P2G_BJ * P2G_BK, UNR = 4;
+    for (int ks = 0; ks < nsl; ++ks) {
+        const int k = clo[2] + ks;
+        const int m = roff[ks][nrow];
+        if (m == 0) continue;  // (uniform) no particles: nothing to add
+        const bool fits = m <= P2G_CAP;
         if (fits) {
-            for (int jr = 0; jr < nrow; ++jr) {
-                const int a = raw[jr][0], m = raw[jr][ncell] - a, o = roff[jr];
-                for (int q = tid; q < m; q += blockDim.x) {
-                    lx[o + q] = px[a + q]; ly[o + q] = py[a + q]; lz[o + q] = pz[a + q]; lv[o + q] = vc[a + q];
+            // slab particle q: row jr with roff[jr] <= q < roff[jr + 1]
+            for (int q0 = tid; q0 < m; q0 += UNR * NB) {
+                double vx[UNR], vy[UNR], vz[UNR], vv[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    const int q = q0 + u * NB;
+                    if (q < m) {
+                        int jr = 0;
+                        while (jr + 1 < nrow && roff[ks][jr + 1] <= q) ++jr;
+                        const int a = raw[ks][jr][0] + (q - roff[ks][jr]);
+                        vx[u] = px[a]; vy[u] = py[a]; vz[u] = pz[a]; vv[u] = vc[a];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    const int q = q0 + u * NB;
+                    if (q < m) { lx[q] = vx[u]; ly[q] = vy[u]; lz[q] = vz[u]; lv[q] = vv[u]; }
                 }
             }
         }
@@ -423,8 +442,8 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
             for (int j = lo[1]; j <= hi[1]; ++j) {
                 const int jr = j - clo[1];
                 if (fits) {
-                    const int base = roff[jr] - raw[jr][0];
-                    const int pa = base + raw[jr][lo[0] - clo[0]], pb = base + raw[jr][hi[0] + 1 - clo[0]];
+                    const int base = roff[ks][jr] - raw[ks][jr][0];
+                    const int pa = base + raw[ks][jr][lo[0] - clo[0]], pb = base + raw[ks][jr][hi[0] + 1 - clo[0]];
                     for (int p = pa; p < pb; ++p) pair(lx[p], ly[p], lz[p], lv[p]);
                 } else {
                     const long row = (long)g.ni * j + g.nn * k;
