@@ -329,7 +329,7 @@ __global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, 
     out[t] = wsum > 0.0000001 ? vsum / wsum : 0.0;
 }
 
-// The same gather over bricks of 8x4x4 faces.  A workgroup first reads the
+// The same gather over bricks of 8x8x4 faces.  A workgroup first reads the
 // cell starts of all its candidate slabs (one round of loads), then walks the
 // k-slabs that hold particles: it copies the slab's particles (one contiguous
 // run per cell row of the sorted arrays) into LDS with several loads in flight
@@ -339,16 +339,23 @@ __global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, 
 // bit-identical to it; each particle is fetched once per brick instead of once
 // per face, and bricks in air cost one round of loads.  A slab with more
 // particles than fit is read from global memory in the same order.
-#define P2G_BI 8
-#define P2G_BJ 4
+#ifndef P2G_BJ
+#define P2G_BJ 8
+#endif
+#ifndef P2G_BK
 #define P2G_BK 4
-#define P2G_CAP 1536
+#endif
+#ifndef P2G_CAP
+#define P2G_CAP 1024  // 32 KB of staged particles: 4 workgroups (16 waves) per CU
+#endif
+#define P2G_BI 8
 #define P2G_MAXC 12  // cells along i in a slab row (8 faces + support)
 __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     Geo g, int axis, int kern, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
     const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
     double* __restrict__ out) {
-    __shared__ double lx[P2G_CAP], ly[P2G_CAP], lz[P2G_CAP], lv[P2G_CAP];
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    __shared__ d2 lp[2 * P2G_CAP];  // slab particle q: {x, y} at 2q, {z, v} at 2q + 1
     __shared__ int raw[P2G_BK + 3][P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slabs' cells, per row
     __shared__ int roff[P2G_BK + 3][P2G_BJ + 4];               // LDS offset of each slab row, then the slab's total
     const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
@@ -433,7 +440,10 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     const int q = q0 + u * NB;
-                    if (q < m) { lx[q] = vx[u]; ly[q] = vy[u]; lz[q] = vz[u]; lv[q] = vv[u]; }
+                    if (q < m) {
+                        lp[2 * q] = d2{vx[u], vy[u]};
+                        lp[2 * q + 1] = d2{vz[u], vv[u]};
+                    }
                 }
             }
         }
@@ -444,7 +454,10 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
                 if (fits) {
                     const int base = roff[ks][jr] - raw[ks][jr][0];
                     const int pa = base + raw[ks][jr][lo[0] - clo[0]], pb = base + raw[ks][jr][hi[0] + 1 - clo[0]];
-                    for (int p = pa; p < pb; ++p) pair(lx[p], ly[p], lz[p], lv[p]);
+                    for (int p = pa; p < pb; ++p) {
+                        const d2 a = lp[2 * p], b = lp[2 * p + 1];
+                        pair(a.x, a.y, b.x, b.y);
+                    }
                 } else {
                     const long row = (long)g.ni * j + g.nn * k;
                     const int pa = start[row + lo[0]], pb = start[row + hi[0] + 1];
