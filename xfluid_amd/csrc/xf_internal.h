@@ -114,8 +114,11 @@ long sweep_w8_edge_words(const Geo& g);
 template <class T>
 void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
 template <class T>
+// dot_partial: also the block partials of dot(z, r) over dot_blocks blocks of
+// XF_BLOCK threads (k_dot's cells and order)
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
-                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
+                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st,
+                      double* dot_partial = nullptr, int dot_blocks = 0);
 template <class T>
 bool sweep_precond_pipelined(const Geo& g, int kind, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S,
                              int* flags, unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);

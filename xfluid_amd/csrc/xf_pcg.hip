@@ -202,20 +202,23 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     z[c] = v * ic;
 }
 
+// dot_partial: when given, the 8x8-tile schedule also writes the block
+// partials of dot(z, r) (returns true then; the caller runs k_dot otherwise)
 template <class T>
-static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
+static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
                       int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
-                      SweepTimer* tm, hipStream_t st) {
+                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr) {
     // pipelined schedule (xf_sweep.hip) on the skewed copies prepared by the caller
     if (sweep_kernel >= 2 && sk) {
         hipEvent_t a = nullptr, b = nullptr;
         if (tm && tm->on && tm->used + 2 <= tm->cap) { a = tm->ev[tm->used]; b = tm->ev[tm->used + 1]; }
-        const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
+        const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st,
+                                                                 dot_partial, RED_BLOCKS)
                          : sweep_kernel >= 8 ? sweep_precond_wave<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
                                            : sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st);
         if (ran) {
             if (a) tm->used += 2;
-            return;
+            return sweep_kernel == 9 && dot_partial;
         }
     }
     const int planes = g.ni + g.nj + g.nk - 2;
@@ -231,6 +234,7 @@ static void precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
         plane_range(g, t, &kmin, &nt);
         k_bwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, q, z, S);
     }
+    return false;
 }
 
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
@@ -250,21 +254,28 @@ void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const do
 // added as a*x to a sum starting at 0.  Optional FP64 partial dot(x, y).
 template <class T>
 __device__ inline T spmv_cell(const Geo& g, uint16_t cc, long c, const T* __restrict__ x) {
+    // the seven operands are read unconditionally (indices clamped into the
+    // grid), so the loads of several cells can be in flight together; the
+    // present entries are then added in column order
+    const T xkm = x[c >= g.nn ? c - g.nn : c], xjm = x[c >= g.ni ? c - g.ni : c], xim = x[c >= 1 ? c - 1 : c];
+    const T xc = x[c];
+    const T xip = x[c + 1 < g.N ? c + 1 : c], xjp = x[c + g.ni < g.N ? c + g.ni : c];
+    const T xkp = x[c + g.nn < g.N ? c + g.nn : c];
     T y = 0;
     int cl = code_dir(cc, D_ZM);
-    if (present(cl)) y += (T)code_coef(cl) * x[c - g.nn];
+    if (present(cl)) y += (T)code_coef(cl) * xkm;
     cl = code_dir(cc, D_YM);
-    if (present(cl)) y += (T)code_coef(cl) * x[c - g.ni];
+    if (present(cl)) y += (T)code_coef(cl) * xjm;
     cl = code_dir(cc, D_XM);
-    if (present(cl)) y += (T)code_coef(cl) * x[c - 1];
+    if (present(cl)) y += (T)code_coef(cl) * xim;
     double dg = code_diag(cc);
-    if (dg != 0.0) y += (T)dg * x[c];
+    if (dg != 0.0) y += (T)dg * xc;
     cl = code_dir(cc, D_XP);
-    if (present(cl)) y += (T)code_coef(cl) * x[c + 1];
+    if (present(cl)) y += (T)code_coef(cl) * xip;
     cl = code_dir(cc, D_YP);
-    if (present(cl)) y += (T)code_coef(cl) * x[c + g.ni];
+    if (present(cl)) y += (T)code_coef(cl) * xjp;
     cl = code_dir(cc, D_ZP);
-    if (present(cl)) y += (T)code_coef(cl) * x[c + g.nn];
+    if (present(cl)) y += (T)code_coef(cl) * xkp;
     return y;
 }
 
@@ -274,7 +285,24 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
     __shared__ double sh[16];
     if (S && S->done) return;
     double acc = 0.0;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    // four cells of the grid-stride sequence at a time (the same cells, in the
+    // same order, as one at a time)
+    for (; c + 3 * stride < g.N; c += 4 * stride) {
+        uint16_t cc[4];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cc[u] = code[c + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = spmv_cell<T>(g, cc[u], c + u * stride, x);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            y[c + u * stride] = v[u];
+            acc += (double)x[c + u * stride] * (double)v[u];
+        }
+    }
+    for (; c < g.N; c += stride) {
         T v = spmv_cell<T>(g, code[c], c, x);
         y[c] = v;
         acc += (double)x[c] * (double)v;
@@ -431,8 +459,8 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     const int tm_start = tm ? tm->used : 0;
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
-    precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st);
-    k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
+    if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
+        k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
     k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, RED_BLOCKS, S, tol_factor, 0);
     k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
     const int CH = 4;
@@ -445,8 +473,8 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it,
                                          it == end - 1 ? S_host : nullptr);
-            precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st);
-            k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
+            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
+                k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_BETA, partial, RED_BLOCKS, S, tol_factor, it);
             k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
         }

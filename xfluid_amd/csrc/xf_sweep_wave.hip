@@ -593,6 +593,30 @@ __global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restr
     dst[c] = src[(t * w8_slots(g) + i + jl + kl) * 64 + jl + 8 * kl];
 }
 
+// z from the oct layout as k_from_oct, and the block partials of dot(z, r)
+// over the same grid-stride cells in the same order as k_dot (pcg_solver.h:
+// 273, rho = dot(z, r)), so the partials are bit-identical to k_dot's
+template <class T>
+__global__ void k_from_oct_dot(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ z, const T* __restrict__ r,
+                               double* __restrict__ partial, const PcgScalars* S, Box8 b) {
+    __shared__ double sh[16];
+    if (S && S->done) return;
+    double acc = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+        T v = (T)0;
+        if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
+            const long t = (long)(j >> 3) * ntk + (k >> 3);
+            const int jl = j & 7, kl = k & 7;
+            v = src[(t * w8_slots(g) + i + jl + kl) * 64 + jl + 8 * kl];
+        }
+        z[c] = v;
+        acc += (double)v * (double)r[c];
+    }
+    acc = block_reduce<double, false>(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
 // records: rec[4e..4e+3] forward, rec[4n + 4e..] backward
 template <class T>
 __global__ void k_w8_records(Geo g, int ntk, long n, const uint16_t* __restrict__ code, const T* __restrict__ inv,
@@ -690,7 +714,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     static_assert(!MIC || (FWD && sizeof(T) == 8), "pivots: forward, FP64");
     constexpr int NS = sizeof(T) / 4;
     constexpr int R = MIC ? 16 : 32;                      // ring depth in steps
-    constexpr int LEAD = MIC ? 12 : NS == 1 ? 24 : 20;    // steps of record loads in flight
+    constexpr int LEAD = MIC ? 12 : NS == 1 ? 24 : 16;    // steps of record loads in flight
     static_assert((R & (R - 1)) == 0 && R % W8_B == 0 && LEAD + 4 <= R, "ring depth");
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
@@ -700,7 +724,11 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     constexpr int DOFF = EOFF + 2 * ROWB;        // the slot's dummy word
     constexpr int EB = 2 * ROWB + 16;            // edge bytes per slot
     constexpr int SLOT = EOFF + EB;
-    constexpr int PERD = RECB / 1024 + (INB ? 1 : 0);  // record + input loads per step
+    // the forward sweep gathers its input from the natural layout (one 4-byte
+    // load per lane and word: no separate layout pass); the backward sweep
+    // reads the forward output in the oct layout (16-byte loads)
+    constexpr bool NATIN = FWD && !MIC;
+    constexpr int PERD = RECB / 1024 + (INB ? (NATIN ? NS : 1) : 0);  // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
     __shared__ __attribute__((aligned(16))) unsigned char ring_[R * SLOT];
     __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
@@ -733,6 +761,9 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
         // waits for: the compute wave asks for up to 5 steps ahead of the last
         // count it published, with R - LEAD >= 4 slots of slack.
         int cons_seen = 0;
+        const int rjl = lane & 7, rkl = lane >> 3;
+        const bool rjk = 8 * J + rjl < g.nj && 8 * K + rkl < g.nk;
+        const long rbase = rjk ? (long)g.ni * (8 * J + rjl) + (long)g.nn * (8 * K + rkl) : 0;
         for (int x = -LEAD; x < nS; ++x) {
             if (x >= 0) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -752,8 +783,17 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
                     __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + (MIC ? 8 : 4) * e0) + h * 1024 +
                                                          lane * 16,
                                                      dst + h * 1024, 16, 0, 0);
-                if (INB && lane < INB / 16)
+                if constexpr (NATIN) {
+                    // lane's cell at this slot, or cell 0 for a lane outside the grid
+                    // (its records are zero, so its input only has to be finite)
+                    const int i = slot_of(s) - rjl - rkl;
+                    const long c = rjk && (unsigned)i < (unsigned)g.ni ? rbase + i : 0;
+                    const unsigned char* a = (const unsigned char*)(P.in + c);
+                    __builtin_amdgcn_global_load_lds(a, dst + RECB, 4, 0, 0);
+                    if (NS == 2) __builtin_amdgcn_global_load_lds(a + 4, dst + RECB + 256, 4, 0, 0);
+                } else if (INB && lane < INB / 16) {
                     __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + e0) + lane * 16, dst + RECB, 16, 0, 0);
+                }
             }
         }
         // (the last step reads "next" records from slot nS, which nothing uses)
@@ -875,7 +915,13 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
             const T2v a = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32);
             const T2v b = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32 + 16);
             R.L0 = a.x; R.L1 = a.y; R.L2 = b.x; R.ic = b.y;
-            R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
+            if constexpr (NATIN) {
+                typedef __attribute__((address_space(3))) const uint32_t ldsU;
+                const uint32_t lo = *(ldsU*)(sl + RECB + lane * 4), hi = *(ldsU*)(sl + RECB + 256 + lane * 4);
+                R.rin = (T)__longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+            } else {
+                R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
+            }
         }
     };
     // an edge word as read from the ring: FP32 {value, tag}, FP64 {lo, tag, hi, tag}
@@ -1108,7 +1154,8 @@ void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const Sw
 
 template <class T>
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
-                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st) {
+                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
+                      int dot_blocks) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
@@ -1117,10 +1164,9 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, sk, &tag1, st) ||
         !w8_next_tag(g, sk, &tag2, st))
         return false;
-    k_to_oct<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, r, sk.oR, S);
     const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
     const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
-    const W8Planes<T> f{sk.oR, sk.oQ, sk.oRec, sk.oEdge};
+    const W8Planes<T> f{r, sk.oQ, sk.oRec, sk.oEdge};  // (natural-layout input)
     const W8Planes<T> b{sk.oQ, sk.oZ, sk.oRec + 4 * n, sk.oEdge};
     const int ntjb = J1 - J0 + 1, ntkb = K1 - K0 + 1, nS = s1 - s0 + 1;
     const int tiles = ntjb * ntkb;
@@ -1129,13 +1175,16 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     k_sweep_w8<T, false><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags);
     if (t1) hipEventRecord(t1, st);
     const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
-    k_from_oct<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, S, bx);
+    if (dot_partial)
+        k_from_oct_dot<T><<<dot_blocks, XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, r, dot_partial, S, bx);
+    else
+        k_from_oct<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, S, bx);
     return true;
 }
 
 template void sweep_prepare_w8<double>(const Geo&, const uint16_t*, const double*, const SweepSkew<double>&, hipStream_t);
 template void sweep_prepare_w8<float>(const Geo&, const uint16_t*, const float*, const SweepSkew<float>&, hipStream_t);
 template bool sweep_precond_w8<double>(const Geo&, const SweepSkew<double>&, const double*, double*, const PcgScalars*,
-                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t);
+                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int);
 template bool sweep_precond_w8<float>(const Geo&, const SweepSkew<float>&, const float*, float*, const PcgScalars*,
-                                      int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t);
+                                      int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int);
