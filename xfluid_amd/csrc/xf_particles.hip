@@ -522,8 +522,7 @@ __global__ void k_density(Geo g, const double* __restrict__ px, const double* __
         for (int j = lj; j <= hj; ++j) {
             long row = (long)g.ni * j + g.nn * k;
             int a0 = start[row + li], b0 = start[row + hi_ + 1];
-            for (int p = a0; p < b0; ++p) {
-                const double x = px[p], y = py[p], z = pz[p];
+            auto one = [&](double x, double y, double z) __attribute__((always_inline)) {
                 double ex2[D], ey2[D], ez2[D];
                 for (int a = 0; a < D; ++a) {
                     double e = xs[a] - x; ex2[a] = e * e;
@@ -536,7 +535,17 @@ __global__ void k_density(Geo g, const double* __restrict__ px, const double* __
                             double s = ex2[a] + ey2[b] + ez2[c];
                             if (s < W.r2) acc[c][b][a] += wyvill(W, s);
                         }
+            };
+            // four particles' positions requested together, then added in index order
+            int p = a0;
+            for (; p + 4 <= b0; p += 4) {
+                double X[4], Y[4], Z[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) { X[u] = px[p + u]; Y[u] = py[p + u]; Z[u] = pz[p + u]; }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) one(X[u], Y[u], Z[u]);
             }
+            for (; p < b0; ++p) one(px[p], py[p], pz[p]);
         }
     for (int c = 0; c < nk_v; ++c)
         for (int b = 0; b < nj_v; ++b)

@@ -867,6 +867,9 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     const int ejo = eJ && hasJ ? EOFF + kl * EWB : DOFF;
     const int eko = eK && hasK ? EOFF + ROWB + w8_kword<FWD>(jl) * EWB : DOFF;
     const int eout = oJ ? kl * EWB : (8 + jl) * EWB;
+    // a lane off both edges stores its words out of the edge buffer's range
+    // (buffer stores beyond num_records are dropped by the hardware)
+    const int estore = oJ || oK ? eout : 0x7ffff000;
     // this tile's output plane and edge rows (offsets in bytes from the tile's start)
     const __amdgpu_buffer_rsrc_t orc =
         __builtin_amdgcn_make_buffer_rsrc(P.out + myTile * SL * 64, (short)0, SL * 64 * (int)sizeof(T), 0x00020000);
@@ -947,7 +950,9 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     // results and the next step's records are all requested before the first
     // wait, so their latencies overlap; then the edge tags are checked and the
     // dependent chain runs on records read one step earlier.
-    auto step = [&](int s, int slot, Rec& cur, Rec& nxt) __attribute__((always_inline)) {
+    // (s = sb + u, u <= ub: the store offsets split into a per-block part and
+    // a constant one that the instructions carry)
+    auto step = [&](int s, int slot, Rec& cur, Rec& nxt, int sb, int u, int ub) __attribute__((always_inline)) {
         const lds_u8* sl = ring + slot * SLOT;
         const uint32_t want = tagbase + (uint32_t)(s + 7);
         wv_u32x4 wj = ew_read(sl + ejo), wk = ew_read(sl + eko);
@@ -1003,22 +1008,21 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
             }
             q = v * cur.ic;
         }
-        const int so = slot_of(s) * 64 * (int)sizeof(T);
+        constexpr int SB = 64 * (int)sizeof(T);
+        const int so = (FWD ? s0 + sb : s0 + nS - 1 - sb - ub) * SB, oi = (FWD ? u : ub - u) * SB;
+        const uint64_t qb = NS == 1 ? (uint64_t)__float_as_uint((float)q) : (uint64_t)__double_as_longlong((double)q);
         if constexpr (NS == 1) {
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint((float)q), orc, lane * 4, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)qb, orc, lane * 4 + oi, so, 0);
         } else {
-            const uint64_t u = (uint64_t)__double_as_longlong((double)q);
-            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)u, (uint32_t)(u >> 32)}, orc, lane * 8, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, (uint32_t)(qb >> 32)}, orc, lane * 8 + oi, so, 0);
         }
-        if (oJ || oK) {
-            const uint32_t t = tagbase + (uint32_t)s;
-            if constexpr (NS == 1) {
-                __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{__float_as_uint((float)q), t}, erc, eout, s * ROWB, WV_SC1);
-            } else {
-                const uint64_t u = (uint64_t)__double_as_longlong((double)q);
-                __builtin_amdgcn_raw_buffer_store_b128(wv_u32x4{(uint32_t)u, t, (uint32_t)(u >> 32), t}, erc, eout,
-                                                       s * ROWB, WV_SC1);
-            }
+        // edge words: every lane stores, those off the edge out of the buffer's range
+        const uint32_t t = tagbase + (uint32_t)s;
+        if constexpr (NS == 1) {
+            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, t}, erc, estore + u * ROWB, sb * ROWB, WV_SC1);
+        } else {
+            __builtin_amdgcn_raw_buffer_store_b128(wv_u32x4{(uint32_t)qb, t, (uint32_t)(qb >> 32), t}, erc,
+                                                   estore + u * ROWB, sb * ROWB, WV_SC1);
         }
         qp = q;
     };
@@ -1041,17 +1045,17 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
 #pragma unroll
         for (int u = 0; u < W8_B; u += 4) {
             need_rec(s + u + 5 < nS + 1 ? s + u + 5 : nS + 1);
-            step(s + u, h + u, ra, rb);
-            step(s + u + 1, h + u + 1, rb, ra);
-            step(s + u + 2, h + u + 2, ra, rb);
-            step(s + u + 3, h + u + 3, rb, ra);
+            step(s + u, h + u, ra, rb, s, u, W8_B - 1);
+            step(s + u + 1, h + u + 1, rb, ra, s, u + 1, W8_B - 1);
+            step(s + u + 2, h + u + 2, ra, rb, s, u + 2, W8_B - 1);
+            step(s + u + 3, h + u + 3, rb, ra, s, u + 3, W8_B - 1);
             publish(s + u + 4);
         }
     }
     for (; s < nS; s += 2) {
         need_rec(s + 3 < nS + 1 ? s + 3 : nS + 1);
-        step(s, s & (R - 1), ra, rb);
-        if (s + 1 < nS) step(s + 1, (s + 1) & (R - 1), rb, ra);
+        step(s, s & (R - 1), ra, rb, s, 0, 0);
+        if (s + 1 < nS) step(s + 1, (s + 1) & (R - 1), rb, ra, s + 1, 0, 0);
         publish(s + 2 < nS ? s + 2 : nS);
     }
     // edge rows of the steps after the last: zeros, for the downstream tiles' last 7 steps
