@@ -345,7 +345,7 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
               dalloc(&s->pw.inv, N) == hipSuccess && dalloc(&s->pw.x32, N) == hipSuccess &&
               dalloc(&s->pw.r32, N) == hipSuccess && dalloc(&s->pw.s32, N) == hipSuccess &&
               dalloc(&s->pw.z32, N) == hipSuccess && dalloc(&s->pw.q32, N) == hipSuccess &&
-              dalloc(&s->pw.inv32, N) == hipSuccess && dalloc(&s->pw.partial, RED_N) == hipSuccess &&
+              dalloc(&s->pw.inv32, N) == hipSuccess && dalloc(&s->pw.partial, 2 * RED_N) == hipSuccess &&
               dalloc(&s->pw.S, 2) == hipSuccess && dalloc(&s->red, RED_N) == hipSuccess &&
               dalloc(&s->surf_cnt, N + 1) == hipSuccess && dalloc(&s->surf_start, N + 2) == hipSuccess;
     if (!ok) return fail(set_err(-2, "device allocation failed"));

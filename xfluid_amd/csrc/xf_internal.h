@@ -12,6 +12,7 @@ inline unsigned xf_blocks(long n, int per = XF_BLOCK) { return (unsigned)((n + p
 // iteration).  `done` stops all later kernels of the solve.
 struct PcgScalars {
     double rho, sz, alpha, beta, res, tol, rho_new, res0;
+    double rho_next;  // rho of the next iteration, moved into rho by the next spmv
     int done, iters, status, it;
 };
 

@@ -284,6 +284,9 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
                            double* partial, const PcgScalars* S) {
     __shared__ double sh[16];
     if (S && S->done) return;
+    // (the previous iteration's beta step left its rho in rho_next: no kernel
+    // reads rho while this one runs)
+    if (S && blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(S)->rho = S->rho_next;
     double acc = 0.0;
     const long stride = (long)gridDim.x * blockDim.x;
     long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -330,13 +333,31 @@ __global__ void k_residual(Geo g, const uint16_t* __restrict__ code, const doubl
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
-// x += alpha s; r += (-alpha) z; partial max |r|   (blas_wrapper.h:41-51)
+template <bool MAX>
+__device__ inline double reduce_partials(const double* p, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = MAX ? (p[i] > v ? p[i] : v) : v + p[i];
+    return block_reduce<double, MAX>(v, sh);
+}
+
+// x += alpha s; r += (-alpha) z; partial max |r|   (blas_wrapper.h:41-51).
+// The alpha step (alpha = rho / dot(s, A s), pcg_solver.h:262) runs in every
+// block from the spmv's partials (pin): each block reduces them in the same
+// order, so all agree, and block 0 records sz and alpha.
 template <class T>
 __global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ s,
-                           const T* __restrict__ z, double* partial, const PcgScalars* S) {
+                           const T* __restrict__ z, const double* __restrict__ pin, int npin, double* partial,
+                           PcgScalars* S) {
     __shared__ double sh[16];
+    __shared__ double alpha_sh;
     if (S->done) return;
-    const double alpha = S->alpha;
+    const double sz = reduce_partials<false>(pin, npin, sh);
+    if (threadIdx.x == 0) {
+        alpha_sh = S->rho / sz;
+        if (blockIdx.x == 0) { S->sz = sz; S->alpha = alpha_sh; }
+    }
+    __syncthreads();
+    const double alpha = alpha_sh;
     const T a = (T)alpha, na = (T)(-alpha);
     double m = 0.0;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
@@ -375,9 +396,21 @@ __global__ void k_absmax(long n, const T* __restrict__ a, double* partial) {
 
 // s = z + beta s  (the reference computes z_i + beta*s_i, then swaps)
 template <class T>
-__global__ void k_update_s(long n, T* __restrict__ s, const T* __restrict__ z, const PcgScalars* S) {
+// s = z + beta s (pcg_solver.h:279-281); the beta step (beta = rho_new / rho)
+// runs in every block from the dot's partials (pin), block 0 records it
+// (rho itself is replaced by the next spmv: other blocks still read it here)
+__global__ void k_update_s(long n, T* __restrict__ s, const T* __restrict__ z, const double* __restrict__ pin,
+                           int npin, PcgScalars* S) {
+    __shared__ double sh[16];
+    __shared__ double beta_sh;
     if (S->done) return;
-    const T beta = (T)S->beta;
+    const double v = reduce_partials<false>(pin, npin, sh);
+    if (threadIdx.x == 0) {
+        beta_sh = v / S->rho;
+        if (blockIdx.x == 0) { S->rho_new = v; S->beta = beta_sh; S->rho_next = v; }
+    }
+    __syncthreads();
+    const T beta = (T)beta_sh;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
         s[c] = z[c] + beta * s[c];
 }
@@ -398,12 +431,6 @@ __global__ void k_add_f2d(long n, double* __restrict__ x, const float* __restric
 }
 
 // ---- final reductions (one block, fixed order) and scalar updates ----
-template <bool MAX>
-__device__ inline double reduce_partials(const double* p, int n, double* sh) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = MAX ? (p[i] > v ? p[i] : v) : v + p[i];
-    return block_reduce<double, MAX>(v, sh);
-}
 
 enum { FIN_RES0, FIN_RHO0, FIN_ALPHA, FIN_RES, FIN_BETA };
 // mirror (optional): pinned host copy of S, written at the end (the host reads
@@ -427,6 +454,7 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
             break;
         case FIN_RHO0:
             S->rho = v;
+            S->rho_next = v;
             if (v == 0 || v != v) { S->done = 1; S->status = 2; S->iters = 0; }
             break;
         case FIN_ALPHA:
@@ -457,6 +485,7 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
                      hipStream_t st) {
     const long N = g.N;
     const int tm_start = tm ? tm->used : 0;
+    double* partial2 = partial + RED_BLOCKS;
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
     if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
@@ -468,15 +497,15 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     while (it < max_iter) {
         int end = it + CH < max_iter ? it + CH : max_iter;
         for (; it < end; ++it) {
+            // (the alpha and beta steps run inside k_axpy_max and k_update_s;
+            // the spmv and dot partials go to `partial`, the residual's to `partial2`)
             k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, s, z, partial, S);
-            k_finish<<<1, 1024, 0, st>>>(FIN_ALPHA, partial, RED_BLOCKS, S, tol_factor, it);
-            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, S);
-            k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial, RED_BLOCKS, S, tol_factor, it,
+            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, RED_BLOCKS, partial2, S);
+            k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial2, RED_BLOCKS, S, tol_factor, it,
                                          it == end - 1 ? S_host : nullptr);
             if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
                 k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
-            k_finish<<<1, 1024, 0, st>>>(FIN_BETA, partial, RED_BLOCKS, S, tol_factor, it);
-            k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
+            k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, partial, RED_BLOCKS, S);
         }
         hipStreamSynchronize(st);  // S_host was written by the chunk's last residual step
         if (S_host->done) break;
