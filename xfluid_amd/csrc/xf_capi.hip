@@ -340,7 +340,8 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
               dalloc(&s->marker, s->maxface()) == hipSuccess && dalloc(&s->ci.count, N + 1) == hipSuccess &&
               dalloc(&s->ci.start, N + 2) == hipSuccess && dalloc(&s->ci.cursor, N + 1) == hipSuccess &&
               dalloc(&s->pw.x, N) == hipSuccess && dalloc(&s->pw.r, N) == hipSuccess &&
-              dalloc(&s->pw.s, N) == hipSuccess && dalloc(&s->pw.z, N) == hipSuccess &&
+              dalloc(&s->pw.s, N) == hipSuccess && dalloc(&s->pw.s2, N) == hipSuccess &&
+              dalloc(&s->pw.z, N) == hipSuccess && dalloc(&s->pw.s2_32, N) == hipSuccess &&
               dalloc(&s->pw.q, N) == hipSuccess && dalloc(&s->pw.b, N) == hipSuccess &&
               dalloc(&s->pw.inv, N) == hipSuccess && dalloc(&s->pw.x32, N) == hipSuccess &&
               dalloc(&s->pw.r32, N) == hipSuccess && dalloc(&s->pw.s32, N) == hipSuccess &&
@@ -414,11 +415,11 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
 void xf_sim_destroy(xf_sim* s) {
     if (!s) return;
     if (s->st) hipStreamSynchronize(s->st);
-    double** ds[] = {&s->u, &s->v, &s->w, &s->us, &s->vs, &s->ws, &s->phi, &s->pw.x, &s->pw.r, &s->pw.s, &s->pw.z,
+    double** ds[] = {&s->u, &s->v, &s->w, &s->us, &s->vs, &s->ws, &s->phi, &s->pw.x, &s->pw.r, &s->pw.s, &s->pw.s2, &s->pw.z,
                      &s->pw.q, &s->pw.b, &s->pw.inv, &s->pw.partial, &s->red, &s->P.px, &s->P.py, &s->P.pz,
                      &s->P.vx, &s->P.vy, &s->P.vz, &s->P.tx, &s->P.ty, &s->P.tz, &s->P.tvx, &s->P.tvy, &s->P.tvz};
     for (auto* p : ds) dfree(*p);
-    float** fs[] = {&s->pw.x32, &s->pw.r32, &s->pw.s32, &s->pw.z32, &s->pw.q32, &s->pw.inv32};
+    float** fs[] = {&s->pw.x32, &s->pw.r32, &s->pw.s32, &s->pw.s2_32, &s->pw.z32, &s->pw.q32, &s->pw.inv32};
     for (auto* p : fs) dfree(*p);
     int32_t** is[] = {&s->ci.count, &s->ci.start, &s->ci.cursor, &s->ci.scan_tmp, &s->P.pid, &s->P.tpid,
                       &s->P.key, &s->P.perm, &s->surf_cnt, &s->surf_start};

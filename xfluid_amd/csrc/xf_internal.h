@@ -59,8 +59,10 @@ struct SweepTimer {
 struct PcgWork {
     // FP64 vectors
     double *x, *r, *s, *z, *q, *b, *inv;
+    double* s2;  // second search-direction buffer (s alternates between s and s2)
     // FP32 copies for the mixed-precision inner solve
     float *x32, *r32, *s32, *z32, *q32, *inv32;
+    float* s2_32;
     double* partial;     // per-block partial sums / maxima
     int npartial;
     PcgScalars* S;       // device scalars

@@ -252,15 +252,22 @@ void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const do
 // ---------------------------------------------------------------------------
 // y = A x (sparse_matrix.h:266-276): present entries in column order, each
 // added as a*x to a sum starting at 0.  Optional FP64 partial dot(x, y).
+template <class T, class Get>
+__device__ inline T spmv_cell_g(const Geo& g, uint16_t cc, long c, Get get);
 template <class T>
 __device__ inline T spmv_cell(const Geo& g, uint16_t cc, long c, const T* __restrict__ x) {
+    return spmv_cell_g<T>(g, cc, c, [x](long e) __attribute__((always_inline)) { return x[e]; });
+}
+// the same with the operand x[e] given by get(e)
+template <class T, class Get>
+__device__ inline T spmv_cell_g(const Geo& g, uint16_t cc, long c, Get get) {
     // the seven operands are read unconditionally (indices clamped into the
     // grid), so the loads of several cells can be in flight together; the
     // present entries are then added in column order
-    const T xkm = x[c >= g.nn ? c - g.nn : c], xjm = x[c >= g.ni ? c - g.ni : c], xim = x[c >= 1 ? c - 1 : c];
-    const T xc = x[c];
-    const T xip = x[c + 1 < g.N ? c + 1 : c], xjp = x[c + g.ni < g.N ? c + g.ni : c];
-    const T xkp = x[c + g.nn < g.N ? c + g.nn : c];
+    const T xkm = get(c >= g.nn ? c - g.nn : c), xjm = get(c >= g.ni ? c - g.ni : c), xim = get(c >= 1 ? c - 1 : c);
+    const T xc = get(c);
+    const T xip = get(c + 1 < g.N ? c + 1 : c), xjp = get(c + g.ni < g.N ? c + g.ni : c);
+    const T xkp = get(c + g.nn < g.N ? c + g.nn : c);
     T y = 0;
     int cl = code_dir(cc, D_ZM);
     if (present(cl)) y += (T)code_coef(cl) * xkm;
@@ -284,9 +291,6 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
                            double* partial, const PcgScalars* S) {
     __shared__ double sh[16];
     if (S && S->done) return;
-    // (the previous iteration's beta step left its rho in rho_next: no kernel
-    // reads rho while this one runs)
-    if (S && blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(S)->rho = S->rho_next;
     double acc = 0.0;
     const long stride = (long)gridDim.x * blockDim.x;
     long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -340,10 +344,10 @@ __device__ inline double reduce_partials(const double* p, int n, double* sh) {
     return block_reduce<double, MAX>(v, sh);
 }
 
-// x += alpha s; r += (-alpha) z; partial max |r|   (blas_wrapper.h:41-51).
-// The alpha step (alpha = rho / dot(s, A s), pcg_solver.h:262) runs in every
-// block from the spmv's partials (pin): each block reduces them in the same
-// order, so all agree, and block 0 records sz and alpha.
+// x += alpha s; r += (-alpha) z; partial max |r|   (blas_wrapper.h:41-51),
+// z = A s here.  The alpha step (alpha = rho / dot(s, A s), pcg_solver.h:262)
+// runs in every block from the spmv's partials (pin): each block reduces them
+// in the same order, so all agree, and block 0 records sz and alpha.
 template <class T>
 __global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ s,
                            const T* __restrict__ z, const double* __restrict__ pin, int npin, double* partial,
@@ -353,8 +357,9 @@ __global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T
     if (S->done) return;
     const double sz = reduce_partials<false>(pin, npin, sh);
     if (threadIdx.x == 0) {
-        alpha_sh = S->rho / sz;
-        if (blockIdx.x == 0) { S->sz = sz; S->alpha = alpha_sh; }
+        alpha_sh = S->rho_next / sz;
+        // (rho_next is this iteration's rho; nothing reads rho while this runs)
+        if (blockIdx.x == 0) { S->sz = sz; S->alpha = alpha_sh; S->rho = S->rho_next; }
     }
     __syncthreads();
     const double alpha = alpha_sh;
@@ -394,13 +399,17 @@ __global__ void k_absmax(long n, const T* __restrict__ a, double* partial) {
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
-// s = z + beta s  (the reference computes z_i + beta*s_i, then swaps)
+// s_new = z + beta s_old (pcg_solver.h:279-281; the reference computes
+// z_i + beta*s_i, then swaps) fused with the next
+// iteration's y = A s_new and the partials of dot(s_new, y): each cell's
+// s_new (and its neighbours') is formed with the T arithmetic of the separate
+// update, so the values are the same.  The beta step (beta = rho_new / rho)
+// runs in every block from the dot(z, r) partials (pin); block 0 records it
+// and leaves the new rho in rho_next (other blocks still read rho here).
 template <class T>
-// s = z + beta s (pcg_solver.h:279-281); the beta step (beta = rho_new / rho)
-// runs in every block from the dot's partials (pin), block 0 records it
-// (rho itself is replaced by the next spmv: other blocks still read it here)
-__global__ void k_update_s(long n, T* __restrict__ s, const T* __restrict__ z, const double* __restrict__ pin,
-                           int npin, PcgScalars* S) {
+__global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __restrict__ z,
+                           const T* __restrict__ s_old, T* __restrict__ s_new, T* __restrict__ y,
+                           const double* __restrict__ pin, int npin, double* partial, PcgScalars* S) {
     __shared__ double sh[16];
     __shared__ double beta_sh;
     if (S->done) return;
@@ -411,8 +420,35 @@ __global__ void k_update_s(long n, T* __restrict__ s, const T* __restrict__ z, c
     }
     __syncthreads();
     const T beta = (T)beta_sh;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
-        s[c] = z[c] + beta * s[c];
+    auto sv = [=](long e) __attribute__((always_inline)) { return z[e] + beta * s_old[e]; };
+    double acc = 0.0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; c + 3 * stride < g.N; c += 4 * stride) {
+        uint16_t cc[4];
+        T yv[4], sc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cc[u] = code[c + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            yv[u] = spmv_cell_g<T>(g, cc[u], c + u * stride, sv);
+            sc[u] = sv(c + u * stride);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s_new[c + u * stride] = sc[u];
+            y[c + u * stride] = yv[u];
+            acc += (double)sc[u] * (double)yv[u];
+        }
+    }
+    for (; c < g.N; c += stride) {
+        const T yv = spmv_cell_g<T>(g, code[c], c, sv), sc = sv(c);
+        s_new[c] = sc;
+        y[c] = yv;
+        acc += (double)sc * (double)yv;
+    }
+    acc = block_reduce<double, false>(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
 template <class D, class Sx>
@@ -479,7 +515,8 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
 // One PCG solve (pcg_solver.h:243-285) of A x = rhs with T-precision vectors.
 // x must be zeroed by the caller; rhs is consumed as the initial residual r.
 template <class T>
-static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* z, T* q, double* partial,
+static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* s2, T* z, T* q,
+                     double* partial,
                      PcgScalars* S, PcgScalars* S_host, double tol_factor, int max_iter, int sweep_kernel,
                      const SweepSkew<T>* sk, int* flags, unsigned long long* epoch, SweepTimer* tm,
                      hipStream_t st) {
@@ -494,18 +531,29 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
     const int CH = 4;
     int it = 0;
+    // s alternates between two buffers: iteration it >= 1 forms its s from z and
+    // the previous s inside the spmv (k_spmv_upd), A s goes to q
+    T* sc = s;
+    T* sn = s2;
     while (it < max_iter) {
         int end = it + CH < max_iter ? it + CH : max_iter;
         for (; it < end; ++it) {
-            // (the alpha and beta steps run inside k_axpy_max and k_update_s;
-            // the spmv and dot partials go to `partial`, the residual's to `partial2`)
-            k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, s, z, partial, S);
-            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, s, z, partial, RED_BLOCKS, partial2, S);
-            k_finish<<<1, 1024, 0, st>>>(FIN_RES, partial2, RED_BLOCKS, S, tol_factor, it,
+            // (the spmv and dot partials go to `partial`, the residual's to `partial2`)
+            if (it == 0) {
+                k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, sc, q, partial, S);
+            } else {
+                k_spmv_upd<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, z, sc, sn, q, partial, RED_BLOCKS,
+                                                               partial2, S);
+                T* t = sc;
+                sc = sn;
+                sn = t;
+            }
+            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
+                                                            it == 0 ? partial2 : partial, S);
+            k_finish<<<1, 1024, 0, st>>>(FIN_RES, it == 0 ? partial2 : partial, RED_BLOCKS, S, tol_factor, it,
                                          it == end - 1 ? S_host : nullptr);
             if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
                 k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
-            k_update_s<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, partial, RED_BLOCKS, S);
         }
         hipStreamSynchronize(st);  // S_host was written by the chunk's last residual step
         if (S_host->done) break;
@@ -542,7 +590,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
-        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.z, w.q, w.partial, w.S, w.S_host, tol_factor, max_iter,
+        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.S, w.S_host, tol_factor, max_iter,
                          sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, &w.timer, st);
         res.status = w.S_host->status;
         res.iterations = w.S_host->iters;
@@ -580,7 +628,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
-        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.z32, w.q32, w.partial, w.S, w.S_host, itol,
+        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.S, w.S_host, itol,
                         max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
         total += w.S_host->iters;
         ++outer;
