@@ -707,10 +707,21 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
 // finite neighbour pivots both updates then leave d unchanged bit for bit) and
 // cM the sum of the neighbour's coefficients that the fill-in drops, and
 // there is no input vector.
-template <class T, bool FWD, bool MIC = false>
-__global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0, int nS,
-                                                  W8Planes<T> P, const PcgScalars* S, uint32_t tagbase, int* err,
-                                                  double tau = 0.0, double sigma = 0.0) {
+// LDS ring of the 8x8-tile pipeline: depth in steps and bytes per step
+template <class T, bool MIC>
+struct W8Ring {
+    static constexpr int NS = sizeof(T) / 4;
+    static constexpr int R = MIC ? 16 : 32;
+    static constexpr int SLOT = 64 * (MIC ? 8 : 4) * (int)sizeof(T) + (MIC ? 0 : 64 * (int)sizeof(T)) + 2 * 16 * 8 * NS + 16;
+};
+
+// One sweep of tile (J, K) by the workgroup's three waves (ring_ and ctr_ are
+// the workgroup's LDS).
+template <class T, bool FWD, bool MIC>
+__device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, int J, int K, int J0, int K0, int ntjb,
+                                                               int ntkb, int s0, int nS, W8Planes<T> P,
+                                                               uint32_t tagbase, int* err, double tau, double sigma,
+                                                               unsigned char* ring_, int* ctr_) {
     static_assert(!MIC || (FWD && sizeof(T) == 8), "pivots: forward, FP64");
     constexpr int NS = sizeof(T) / 4;
     constexpr int R = MIC ? 16 : 32;                      // ring depth in steps
@@ -730,13 +741,7 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     constexpr bool NATIN = FWD && !MIC;
     constexpr int PERD = RECB / 1024 + (INB ? (NATIN ? NS : 1) : 0);  // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
-    __shared__ __attribute__((aligned(16))) unsigned char ring_[R * SLOT];
-    __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
-    if (S && S->done) return;
-    int J, K;
-    wv_tile(blockIdx.x, ntjb, ntkb, FWD, &J, &K);
-    J += J0;
-    K += K0;
+    static_assert(SLOT == W8Ring<T, MIC>::SLOT && R == W8Ring<T, MIC>::R, "ring layout");
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int SL = w8_slots(g), ER = w8_erows(g);
     const long myTile = (long)J * ntk + K;
@@ -1071,6 +1076,39 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
     }
 }
 
+template <class T, bool FWD, bool MIC = false>
+__global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0, int nS,
+                                                  W8Planes<T> P, const PcgScalars* S, uint32_t tagbase, int* err,
+                                                  double tau = 0.0, double sigma = 0.0) {
+    __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, MIC>::R * W8Ring<T, MIC>::SLOT];
+    __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
+    if (S && S->done) return;
+    int J, K;
+    wv_tile(blockIdx.x, ntjb, ntkb, FWD, &J, &K);
+    w8_sweep<T, FWD, MIC>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, P, tagbase, err, tau, sigma, ring_, ctr_);
+}
+
+// Forward and backward sweep in one launch: each workgroup sweeps its tile
+// forward, then the same tile backward (the tile that ends the forward sweep
+// begins the backward one, with no launch in between).  Every tile waits on
+// tiles that start later, so all tiles must be resident at once: the host
+// uses this kernel only when they fit (w8_fused_fits).
+template <class T>
+__global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0,
+                                                     int nS, W8Planes<T> Pf, W8Planes<T> Pb, const PcgScalars* S,
+                                                     uint32_t tag1, uint32_t tag2, int* err) {
+    __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, false>::R * W8Ring<T, false>::SLOT];
+    __shared__ int ctr_[2];
+    if (S && S->done) return;
+    int J, K;
+    wv_tile(blockIdx.x, ntjb, ntkb, true, &J, &K);
+    w8_sweep<T, true, false>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_, ctr_);
+    // this tile's forward output is its backward input: stores complete, LDS free
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    w8_sweep<T, false, false>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pb, tag2, err, 0.0, 0.0, ring_, ctr_);
+}
+
 // MIC(0) records (k_sweep_w8<double, true, true>): {cL0, cL1, cL2, diag, cM0, cM1,
 // cM2, 0} per element, directions -k, -j, -i; cM sums the neighbour's
 // coefficients in the two directions k_mic_plane drops (exact halves, so
@@ -1156,6 +1194,21 @@ void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const Sw
     k_w8_records<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, inv, sk.oRec);
 }
 
+// whether all `tiles` workgroups of k_sweep_w8_fb<T> can be resident at once
+template <class T>
+static bool w8_fused_fits(int tiles) {
+    static int cap = -1;
+    if (cap < 0) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep_w8_fb<T>, 192, 0) != hipSuccess)
+            cus = per = 0;
+        cap = cus * per;
+    }
+    return tiles <= cap;
+}
+
 template <class T>
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
@@ -1175,8 +1228,12 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     const int ntjb = J1 - J0 + 1, ntkb = K1 - K0 + 1, nS = s1 - s0 + 1;
     const int tiles = ntjb * ntkb;
     if (t0) hipEventRecord(t0, st);
-    k_sweep_w8<T, true><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags);
-    k_sweep_w8<T, false><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags);
+    if (w8_fused_fits<T>(tiles)) {
+        k_sweep_w8_fb<T><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, b, S, tag1, tag2, flags);
+    } else {
+        k_sweep_w8<T, true><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags);
+        k_sweep_w8<T, false><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags);
+    }
     if (t1) hipEventRecord(t1, st);
     const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
     if (dot_partial)
