@@ -49,12 +49,17 @@ struct SweepSkew {
     // bounding box of the active rows (i0, i1, j0, j1, k0, k1; host copy), see pcg_active_box
     int box[6];
 };
-// Optional device timing of the sweep launches (pairs of events around the
-// forward + backward launches of each preconditioner application).
+// Optional device timing of the sweep launches: pairs of events around the
+// forward + backward launches of every SWEEP_TIMER_STRIDE-th preconditioner
+// application of a solve (application k of the solve, k % stride == 0; each
+// timestamp event drains the stream, so timing every application would slow
+// the solve it measures).
+#define SWEEP_TIMER_STRIDE 8
 struct SweepTimer {
     hipEvent_t* ev;
     int cap, used;
     bool on;
+    int k;  // applications so far in the current solve
 };
 struct PcgWork {
     // FP64 vectors

@@ -211,7 +211,10 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
     // pipelined schedule (xf_sweep.hip) on the skewed copies prepared by the caller
     if (sweep_kernel >= 2 && sk) {
         hipEvent_t a = nullptr, b = nullptr;
-        if (tm && tm->on && tm->used + 2 <= tm->cap) { a = tm->ev[tm->used]; b = tm->ev[tm->used + 1]; }
+        if (tm && tm->on && tm->k++ % SWEEP_TIMER_STRIDE == 0 && tm->used + 2 <= tm->cap) {
+            a = tm->ev[tm->used];
+            b = tm->ev[tm->used + 1];
+        }
         const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st,
                                                                  dot_partial, RED_BLOCKS)
                          : sweep_kernel >= 8 ? sweep_precond_wave<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
@@ -522,6 +525,7 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
                      hipStream_t st) {
     const long N = g.N;
     const int tm_start = tm ? tm->used : 0;
+    if (tm) tm->k = 0;
     double* partial2 = partial + RED_BLOCKS;
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
@@ -565,9 +569,11 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     }
     if (tm && tm->used > tm_start) {
         // keep the timings of the applications that ran: the initial one and one per
-        // iteration before the converged one (later launches returned at once)
+        // iteration before the converged one (later launches returned at once);
+        // applications 0, stride, 2 stride, ... were timed
         const int apps = S_host->res0 == 0 ? 0 : S_host->status == 1 ? 1 + max_iter : S_host->iters;
-        if (tm->used > tm_start + 2 * apps) tm->used = tm_start + 2 * apps;
+        const int timed = apps > 0 ? (apps - 1) / SWEEP_TIMER_STRIDE + 1 : 0;
+        if (tm->used > tm_start + 2 * timed) tm->used = tm_start + 2 * timed;
     }
 }
 
