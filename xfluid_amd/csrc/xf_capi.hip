@@ -389,10 +389,12 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
     if (dalloc(&s->ci.scan_tmp, s->ci.scan_tmp_n) != hipSuccess) return fail(set_err(-2, "alloc scan"));
     s->pw.npartial = RED_N;
-    if (hipHostMalloc((void**)&s->pw.S_host, 2 * sizeof(PcgScalars)) != hipSuccess ||
+    if (hipHostMalloc((void**)&s->pw.S_host, 3 * sizeof(PcgScalars)) != hipSuccess ||
+        hipEventCreateWithFlags(&s->pw.chunk_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->pw.chunk_ev[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&s->h_red, RED_N * sizeof(double)) != hipSuccess)
         return fail(set_err(-2, "pinned alloc"));
-    std::memset(s->pw.S_host, 0, 2 * sizeof(PcgScalars));
+    std::memset(s->pw.S_host, 0, 3 * sizeof(PcgScalars));
     hipMemsetAsync(s->u, 0, s->nu() * 8, s->st);
     hipMemsetAsync(s->v, 0, s->nv() * 8, s->st);
     hipMemsetAsync(s->w, 0, s->nw() * 8, s->st);
@@ -442,6 +444,8 @@ void xf_sim_destroy(xf_sim* s) {
         delete[] s->pw.timer.ev;
     }
     if (s->pw.S_host) hipHostFree(s->pw.S_host);
+    for (hipEvent_t e : s->pw.chunk_ev)
+        if (e) hipEventDestroy(e);
     if (s->h_red) hipHostFree(s->h_red);
     for (auto& e : s->ev) if (e) hipEventDestroy(e);
     if (s->st) hipStreamDestroy(s->st);

@@ -71,7 +71,8 @@ struct PcgWork {
     double* partial;     // per-block partial sums / maxima
     int npartial;
     PcgScalars* S;       // device scalars
-    PcgScalars* S_host;  // pinned mirror
+    PcgScalars* S_host;  // pinned mirror: [0] inner solve, [1] outer, [2] inner (odd chunks)
+    hipEvent_t chunk_ev[2];  // ends of the PCG's iteration chunks in flight
     int* sweep_flags;    // error word + progress counters of the pipelined sweep
     int nflags;
     SweepSkew<double> sk64;  // skewed copies for the pipelined sweeps (FP64 solve)

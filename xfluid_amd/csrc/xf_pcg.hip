@@ -519,10 +519,9 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
 // x must be zeroed by the caller; rhs is consumed as the initial residual r.
 template <class T>
 static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* s2, T* z, T* q,
-                     double* partial,
-                     PcgScalars* S, PcgScalars* S_host, double tol_factor, int max_iter, int sweep_kernel,
-                     const SweepSkew<T>* sk, int* flags, unsigned long long* epoch, SweepTimer* tm,
-                     hipStream_t st) {
+                     double* partial, PcgScalars* S, PcgScalars* S_host, hipEvent_t* chunk_ev, double tol_factor,
+                     int max_iter, int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
+                     SweepTimer* tm, hipStream_t st) {
     const long N = g.N;
     const int tm_start = tm ? tm->used : 0;
     if (tm) tm->k = 0;
@@ -539,8 +538,13 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     // the previous s inside the spmv (k_spmv_upd), A s goes to q
     T* sc = s;
     T* sn = s2;
-    while (it < max_iter) {
+    // Chunks of CH iterations; the host enqueues chunk c + 1 before it waits for
+    // chunk c (whose last residual step writes its status into the pinned slot
+    // S_host[c & 1 ? 2 : 0]), so the GPU does not idle while the host checks.
+    // After convergence the kernels of the chunk in flight return at once.
+    auto launch_chunk = [&](int c) {
         int end = it + CH < max_iter ? it + CH : max_iter;
+        PcgScalars* slot = S_host + ((c & 1) ? 2 : 0);
         for (; it < end; ++it) {
             // (the spmv and dot partials go to `partial`, the residual's to `partial2`)
             if (it == 0) {
@@ -555,13 +559,23 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
                                                             it == 0 ? partial2 : partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_RES, it == 0 ? partial2 : partial, RED_BLOCKS, S, tol_factor, it,
-                                         it == end - 1 ? S_host : nullptr);
+                                         it == end - 1 ? slot : nullptr);
             if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
                 k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
         }
-        hipStreamSynchronize(st);  // S_host was written by the chunk's last residual step
-        if (S_host->done) break;
+        hipEventRecord(chunk_ev[c & 1], st);
+    };
+    int chunk = 0;
+    launch_chunk(0);
+    for (;;) {
+        if (it < max_iter) launch_chunk(chunk + 1);
+        hipEventSynchronize(chunk_ev[chunk & 1]);
+        const PcgScalars* got = S_host + ((chunk & 1) ? 2 : 0);
+        if (got != S_host) *S_host = *got;
+        if (S_host->done || chunk * CH + CH >= max_iter) break;
+        ++chunk;
     }
+    hipStreamSynchronize(st);
     if (!S_host->done) {
         hipMemcpyAsync(S_host, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
@@ -596,7 +610,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
-        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.S, w.S_host, tol_factor, max_iter,
+        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.S, w.S_host, w.chunk_ev, tol_factor,
+                         max_iter,
                          sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, &w.timer, st);
         res.status = w.S_host->status;
         res.iterations = w.S_host->iters;
@@ -634,7 +649,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
-        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.S, w.S_host, itol,
+        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.S, w.S_host,
+                        w.chunk_ev, itol,
                         max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
         total += w.S_host->iters;
         ++outer;
