@@ -151,3 +151,24 @@ def test_particle_adjoints_match_oracle(integrator, kernel):
     gvd, gpd = np.zeros((n, 3)), np.zeros((n, 3))
     s.p2g_adjoint(P, V, *cf, gvd, gpd)
     assert _rel([gvd], [gvo]) <= 1e-12 and _rel([gpd], [gpo]) <= 1e-11
+
+
+def test_sphere_scene_projection_matches_oracle():
+    """The projection benchmark's scene (union of seeded spheres in a walled
+    cube, float32 face velocities) at 24^3: FP64 and mixed-precision GPU
+    projection against the oracle's FP64 projection."""
+    import xfluid_amd as xf
+    from xfluid_amd.scenes import sphere_scene
+    mat, u, v, w = sphere_scene(24, 6, 2)
+    say(f"[projection] 24^3 sphere scene, {int((mat == 1).sum())} liquid cells")
+    ref = project(mat, u, v, w)
+    # (FP64 to the oracle's tolerance; mixed precision to 1e-9 relative residual)
+    for prec, ptol, tol in ((0, TOL, 1e-10), (1, 1e-9, 1e-6)):
+        p = xf.default_params(pcg_tol=ptol, pcg_max_iter=MAXIT, mic_tau=TAU, mic_sigma=SIGMA, precision=prec)
+        s = xf.FluidSimulator(24, 24, 24, p)
+        s.setCellWidth(1.0 / 24)
+        s.set_field(FIELD["MATERIAL"], mat)
+        g = [a.copy() for a in (u, v, w)]
+        assert s.project_fields(*g) == 0
+        assert _rel(g, ref) <= tol, (prec, _rel(g, ref))
+        s.close()
