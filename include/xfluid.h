@@ -225,6 +225,12 @@ int  xf_sim_advect_adjoint(xf_sim* s, int64_t n, const double* pos, double dt, c
  * (gradient on the face arrays) -> gvel (written) and gpos (added). */
 int  xf_sim_p2g_adjoint(xf_sim* s, int64_t n, const double* pos, const double* vel, const double* gu,
                         const double* gv, const double* gw, double* gvel, double* gpos, int device_ptrs);
+/* VJP of the particle-blob density (DirectedDistanceField::updateFeild,
+ * directeddistancefield.cpp:122-169, 300-328) for the given particles: gphi
+ * (gradient on the (d*ni+1)(d*nj+1)(d*nk+1) density lattice, d =
+ * sub_division) -> gpos (added). */
+int  xf_sim_density_adjoint(xf_sim* s, int64_t n, const double* pos, const double* gphi, double* gpos,
+                            int device_ptrs);
 
 /* ---- operator diagnostics on the last assembled system (parity hooks):
  *      y = A x (FixedSparseMatrix multiply, sparse_matrix.h:266-276) and

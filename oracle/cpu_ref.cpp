@@ -968,6 +968,42 @@ struct Sim {
                 }
     }
     // VJP of P2G: g on the face arrays -> gvel (n x 3, overwritten), gpos (n x 3, accumulated)
+    // VJP of density() for the particles pos (n x 3): gpos += sum_v gphi(v) W'(s)
+    // (-2)(x_v - p) over each particle's vertex box (directeddistancefield.cpp:
+    // 300-328), W'(s) = -3 c0 s^2 + 2 c1 s - c2 the derivative of the Wyvill
+    // polynomial (directeddistancefield.cpp:246-256); test infrastructure only.
+    void density_adjoint(long n, const double* pos, const double* gphi, double* gpos) {
+        const int dd = d();
+        const int si = dd * ni + 1, sj = dd * nj + 1, sk = dd * nk + 1;
+        const double R = blob_radius(), R2 = R * R;
+        const double c0 = (4.0 / 9.0) * (1.0 / (R2 * R2 * R2));
+        const double c1 = (17.0 / 9.0) * (1.0 / (R2 * R2));
+        const double c2 = (22.0 / 9.0) * (1.0 / (R2));
+        const double subdx = dx / (double)dd;
+        for (long p = 0; p < n; ++p) {
+            const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+            int g0i = pos2idx(-R + x, dx) * dd, g0j = pos2idx(-R + y, dx) * dd, g0k = pos2idx(-R + z, dx) * dd;
+            int g1i = (pos2idx(x + R, dx) + 1) * dd, g1j = (pos2idx(y + R, dx) + 1) * dd,
+                g1k = (pos2idx(z + R, dx) + 1) * dd;
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int i = g0i; i <= g1i; ++i)
+                for (int j = g0j; j <= g1j; ++j)
+                    for (int k = g0k; k <= g1k; ++k) {
+                        if (i < 0 || j < 0 || k < 0 || i >= si || j >= sj || k >= sk) continue;
+                        double ex = idx2pos(i, subdx) - x, ey = idx2pos(j, subdx) - y, ez = idx2pos(k, subdx) - z;
+                        double rs = ex * ex + ey * ey + ez * ez;
+                        if (!(rs < R2)) continue;
+                        double dw = -3.0 * c0 * rs * rs + 2.0 * c1 * rs - c2;
+                        double f = gphi[i + (long)si * (j + (long)sj * k)] * dw * -2.0;
+                        ax += f * ex;
+                        ay += f * ey;
+                        az += f * ez;
+                    }
+            gpos[3 * p] += ax;
+            gpos[3 * p + 1] += ay;
+            gpos[3 * p + 2] += az;
+        }
+    }
     void p2g_adjoint(long n, const double* pos, const double* vel, const double* const g[3], double* gvel,
                      double* gpos) const {
         std::fill(gvel, gvel + 3 * n, 0.0);
@@ -1303,6 +1339,9 @@ void xo_p2g_points(void* h, int64_t n, const double* pos, const double* vel, dou
         std::vector<double> wt((size_t)(s->ni + (axis == 0)) * (s->nj + (axis == 1)) * (s->nk + (axis == 2)));
         s->p2g_points(axis, n, pos, vel, out[axis], wt.data());
     }
+}
+void xo_density_adjoint(void* h, int64_t n, const double* pos, const double* gphi, double* gpos) {
+    ((Sim*)h)->density_adjoint(n, pos, gphi, gpos);
 }
 void xo_p2g_adjoint(void* h, int64_t n, const double* pos, const double* vel, const double* gu, const double* gv,
                     const double* gw, double* gvel, double* gpos) {

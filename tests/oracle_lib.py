@@ -90,6 +90,7 @@ def _load_oracle():
     lib.xo_advect_adjoint.argtypes = [P, C.c_int64, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]
     lib.xo_p2g_points.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp]
     lib.xo_p2g_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.xo_density_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp]
     lib.xo_project.argtypes = g + [_u8, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double]
     lib.xo_project_adjoint.argtypes = g + [_u8, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double]
     return lib

@@ -172,3 +172,21 @@ def test_sphere_scene_projection_matches_oracle():
         assert s.project_fields(*g) == 0
         assert _rel(g, ref) <= tol, (prec, _rel(g, ref))
         s.close()
+
+
+def test_density_adjoint_matches_oracle():
+    """GPU density VJP against the FP64 CPU adjoint (same vertex order)."""
+    import xfluid_amd as xf
+    from test_adjoint import particle_scene
+    o, P, V = particle_scene()
+    say(f"[adjoint] density VJP, {len(P)} particles")
+    from scenes import gpu_like
+    g = gpu_like(o)
+    o.stage(0)
+    rng = np.random.default_rng(31)
+    c = np.ascontiguousarray(rng.standard_normal(o.get(FIELD["PHI"]).shape))
+    ref = np.zeros((len(P), 3))
+    oracle().xo_density_adjoint(o.h, len(P), P, c, ref)
+    got = np.zeros((len(P), 3))
+    assert g.density_adjoint(P, c, got) == 0
+    assert np.array_equal(got, ref), float(np.abs(got - ref).max())

@@ -114,6 +114,7 @@ def lib():
     L.xf_sim_sample_adjoint.argtypes = [V, C.c_int64, V, V, V, V, V, V, C.c_int, C.c_int]
     L.xf_sim_advect_adjoint.argtypes = [V, C.c_int64, V, C.c_double, V, V, V, V, V, C.c_int]
     L.xf_sim_p2g_adjoint.argtypes = [V, C.c_int64, V, V, V, V, V, V, V, C.c_int]
+    L.xf_sim_density_adjoint.argtypes = [V, C.c_int64, V, V, V, C.c_int]
     L.xf_sim_generate_mesh.argtypes = [V, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xf_sim_get_mesh.argtypes = [V, V, V, V]
     L.xf_sim_write_mesh_ply.argtypes = [V, C.c_char_p]
@@ -324,6 +325,11 @@ class FluidSimulator:
         """VJP of P2G; gvel is written, gpos is added to."""
         p, dev = self._ptrs((pos, vel, gu, gv, gw, gvel, gpos), "p2g_adjoint")
         return _chk(self.L.xf_sim_p2g_adjoint(self.h, len(pos), *p, dev), "p2g_adjoint")
+
+    def density_adjoint(self, pos, gphi, gpos):
+        """VJP of the particle-blob density on the sub-vertex lattice; gpos is added to."""
+        p, dev = self._ptrs((pos, gphi, gpos), "density_adjoint")
+        return _chk(self.L.xf_sim_density_adjoint(self.h, len(pos), *p, dev), "density_adjoint")
 
     def generate_mesh(self, smooth=True):
         """FluidMesher::generateMesh on the current density field -> (verts, tris, normals)."""

@@ -322,3 +322,48 @@ void launch_p2g_adjoint(const Geo& g, int kernel, long n, const double* pos, con
         if (n > 0) k_p2g_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, axis, tri, n, pos, vel, s_wt, s_val, gvel, gpos);
     }
 }
+
+// ---------------------------------------------------------------------------
+// VJP of the particle-blob density (DirectedDistanceField::updateFeild,
+// directeddistancefield.cpp:122-169, 300-328): phi(v) = sum_p W(|x_v - p|^2),
+// W(s) = 1 - c0 s^3 + c1 s^2 - c2 s for s < R^2, x_v = (v + 1/2) dx/d.
+// gpos_p += sum_v gphi(v) W'(s) (-2)(x_v - p), W'(s) = -3 c0 s^2 + 2 c1 s - c2,
+// over the vertex box of the forward scatter (pos2idx(p -+ R) scaled by d), in
+// its i, j, k loop order.  One thread per particle.
+__global__ void k_density_adj(Geo g, int d, double R, long n, const double* __restrict__ pos,
+                              const double* __restrict__ gphi, double* __restrict__ gpos) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int si = d * g.ni + 1, sj = d * g.nj + 1, sk = d * g.nk + 1;
+    const double R2 = R * R;
+    const double c0 = (4.0 / 9.0) * (1.0 / (R2 * R2 * R2)), c1 = (17.0 / 9.0) * (1.0 / (R2 * R2)),
+                 c2 = (22.0 / 9.0) * (1.0 / R2);
+    const double subdx = g.dx / (double)d;
+    const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    const int g0i = pos2idx(-R + x, g.dx) * d, g0j = pos2idx(-R + y, g.dx) * d, g0k = pos2idx(-R + z, g.dx) * d;
+    const int g1i = (pos2idx(x + R, g.dx) + 1) * d, g1j = (pos2idx(y + R, g.dx) + 1) * d,
+              g1k = (pos2idx(z + R, g.dx) + 1) * d;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int i = g0i; i <= g1i; ++i)
+        for (int j = g0j; j <= g1j; ++j)
+            for (int k = g0k; k <= g1k; ++k) {
+                if (i < 0 || j < 0 || k < 0 || i >= si || j >= sj || k >= sk) continue;
+                const double ex = idx2pos(i, subdx) - x, ey = idx2pos(j, subdx) - y, ez = idx2pos(k, subdx) - z;
+                const double rs = ex * ex + ey * ey + ez * ez;
+                if (!(rs < R2)) continue;
+                const double dw = -3.0 * c0 * rs * rs + 2.0 * c1 * rs - c2;
+                const double f = gphi[i + (long)si * (j + (long)sj * k)] * dw * -2.0;
+                ax += f * ex;
+                ay += f * ey;
+                az += f * ez;
+            }
+    gpos[3 * p] += ax;
+    gpos[3 * p + 1] += ay;
+    gpos[3 * p + 2] += az;
+}
+
+void launch_density_adjoint(const Geo& g, int d, double R, long n, const double* pos, const double* gphi,
+                            double* gpos, hipStream_t st) {
+    if (n <= 0) return;
+    k_density_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, d, R, n, pos, gphi, gpos);
+}

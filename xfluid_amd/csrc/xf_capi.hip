@@ -1032,6 +1032,22 @@ int xf_sim_p2g_adjoint(xf_sim* s, int64_t n, const double* pos, const double* ve
     return 0;
 }
 
+int xf_sim_density_adjoint(xf_sim* s, int64_t n, const double* pos, const double* gphi, double* gpos,
+                           int device_ptrs) {
+    if (check(s) || n < 0 || (n > 0 && (!pos || !gpos)) || !gphi) return set_err(-1, "null argument");
+    hipError_t herr = hipSuccess;
+    const int d = s->prm.sub_division;
+    const long nphi = (long)(d * s->g.ni + 1) * (d * s->g.nj + 1) * (d * s->g.nk + 1);
+    STAGE_OR_FAIL(P, pos, 3 * n, true, false);
+    STAGE_OR_FAIL(GPHI, gphi, nphi, true, false);
+    STAGE_OR_FAIL(GP, gpos, 3 * n, true, true);
+    launch_density_adjoint(s->g, d, blob_radius(s), n, P.dev, GPHI.dev, GP.dev, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(GP.sync());
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
 int xf_sim_apply_matrix(xf_sim* s, const double* x, double* y) {
     if (check(s) || !x || !y) return set_err(-1, "null argument");
     const long N = s->g.N;

@@ -195,6 +195,8 @@ void launch_sample_adjoint(const Geo& g, const double* const F[3], double* const
                            const double* gvel, double* gpos, hipStream_t st);
 void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const G[3], long n, const double* pos,
                            double dt, int integrator, const double* gnew, double* gpos, hipStream_t st);
+void launch_density_adjoint(const Geo& g, int d, double R, long n, const double* pos, const double* gphi,
+                            double* gpos, hipStream_t st);
 void launch_p2g_adjoint(const Geo& g, int kernel, long n, const double* pos, const double* vel,
                         const double* const gf[3], double* gvel, double* gpos, double* s_wt, double* s_val,
                         hipStream_t st);
