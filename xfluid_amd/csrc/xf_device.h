@@ -177,3 +177,36 @@ __device__ inline void sample(const Geo& g, const Faces& F, const double* p, dou
     }
     out[2] = trilerp8(t, x, y, z);
 }
+
+// The cells of the planes k0..k1 and rows j0..j1 (all i) of the active rows
+// bounding box, walked by a grid-stride loop without a division per cell: cell
+// t is base + (t / blk) nn + t % blk; a thread steps t by `stride` (stride / blk
+// and stride % blk precomputed).  Outside these rows every PCG vector is zero.
+struct BoxRange {
+    long base, nn;
+    unsigned blk, n, sdiv, smod;
+};
+__host__ __device__ inline BoxRange box_range(long ni, long nj, long nk, const int box[6], unsigned stride) {
+    (void)nk;
+    BoxRange r;
+    r.nn = ni * nj;
+    r.base = (long)box[4] * r.nn + (long)box[2] * ni;
+    r.blk = (unsigned)((box[3] - box[2] + 1) * ni);
+    r.n = r.blk * (unsigned)(box[5] - box[4] + 1);
+    r.sdiv = stride / r.blk;
+    r.smod = stride % r.blk;
+    return r;
+}
+
+// walk of a BoxRange: position t as (t / blk, t % blk)
+__device__ inline void bw_init(const BoxRange& b, unsigned t, unsigned& kq, unsigned& off) {
+    kq = t / b.blk;
+    off = t - kq * b.blk;
+}
+__device__ inline void bw_next(const BoxRange& b, unsigned& kq, unsigned& off) {
+    kq += b.sdiv;
+    off += b.smod;
+    if (off >= b.blk) { off -= b.blk; ++kq; }
+}
+__device__ inline long bw_cell(const BoxRange& b, unsigned kq, unsigned off) { return b.base + (long)kq * b.nn + off; }
+

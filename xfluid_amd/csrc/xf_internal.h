@@ -127,7 +127,7 @@ template <class T>
 // XF_BLOCK threads (k_dot's cells and order)
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st,
-                      double* dot_partial = nullptr, int dot_blocks = 0);
+                      double* dot_partial = nullptr, int dot_blocks = 0, const BoxRange* dot_range = nullptr);
 template <class T>
 bool sweep_precond_pipelined(const Geo& g, int kind, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S,
                              int* flags, unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);

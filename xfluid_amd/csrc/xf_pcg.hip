@@ -207,7 +207,8 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
 template <class T>
 static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
                       int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
-                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr) {
+                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr,
+                      const BoxRange* dot_range = nullptr) {
     // pipelined schedule (xf_sweep.hip) on the skewed copies prepared by the caller
     if (sweep_kernel >= 2 && sk) {
         hipEvent_t a = nullptr, b = nullptr;
@@ -216,12 +217,12 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
             b = tm->ev[tm->used + 1];
         }
         const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st,
-                                                                 dot_partial, RED_BLOCKS)
+                                                                 dot_partial, RED_BLOCKS, dot_range)
                          : sweep_kernel >= 8 ? sweep_precond_wave<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
                                            : sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st);
         if (ran) {
             if (a) tm->used += 2;
-            return sweep_kernel == 9 && dot_partial;
+            return sweep_kernel == 9 && dot_partial && dot_range;
         }
     }
     const int planes = g.ni + g.nj + g.nk - 2;
@@ -291,28 +292,37 @@ __device__ inline T spmv_cell_g(const Geo& g, uint16_t cc, long c, Get get) {
 
 template <class T>
 __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __restrict__ x, T* __restrict__ y,
-                           double* partial, const PcgScalars* S) {
+                           double* partial, const PcgScalars* S, BoxRange bx) {
     __shared__ double sh[16];
     if (S && S->done) return;
     double acc = 0.0;
-    const long stride = (long)gridDim.x * blockDim.x;
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const unsigned stride = gridDim.x * blockDim.x;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
+    bw_init(bx, t, kq, off);
     // four cells of the grid-stride sequence at a time (the same cells, in the
     // same order, as one at a time)
-    for (; c + 3 * stride < g.N; c += 4 * stride) {
+    for (; t + 3 * stride < bx.n; t += 4 * stride) {
+        long cs[4];
         uint16_t cc[4];
         T v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cc[u] = code[c + u * stride];
+        for (int u = 0; u < 4; ++u) {
+            cs[u] = bw_cell(bx, kq, off);
+            bw_next(bx, kq, off);
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = spmv_cell<T>(g, cc[u], c + u * stride, x);
+        for (int u = 0; u < 4; ++u) cc[u] = code[cs[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = spmv_cell<T>(g, cc[u], cs[u], x);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            y[c + u * stride] = v[u];
-            acc += (double)x[c + u * stride] * (double)v[u];
+            y[cs[u]] = v[u];
+            acc += (double)x[cs[u]] * (double)v[u];
         }
     }
-    for (; c < g.N; c += stride) {
+    for (; t < bx.n; t += stride) {
+        const long c = bw_cell(bx, kq, off);
+        bw_next(bx, kq, off);
         T v = spmv_cell<T>(g, code[c], c, x);
         y[c] = v;
         acc += (double)x[c] * (double)v;
@@ -321,8 +331,13 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
     if (partial && threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+static BoxRange full_range(const Geo& g) {
+    const int all[6] = {0, g.ni - 1, 0, g.nj - 1, 0, g.nk - 1};
+    return box_range(g.ni, g.nj, g.nk, all, RED_BLOCKS * XF_BLOCK);
+}
+
 void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st) {
-    k_spmv_dot<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, x, y, nullptr, nullptr);
+    k_spmv_dot<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, x, y, nullptr, nullptr, full_range(g));
 }
 
 // r = b - A x (FP64), partial max |r| -- the refinement residual.
@@ -352,7 +367,7 @@ __device__ inline double reduce_partials(const double* p, int n, double* sh) {
 // runs in every block from the spmv's partials (pin): each block reduces them
 // in the same order, so all agree, and block 0 records sz and alpha.
 template <class T>
-__global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ s,
+__global__ void k_axpy_max(BoxRange bx, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ s,
                            const T* __restrict__ z, const double* __restrict__ pin, int npin, double* partial,
                            PcgScalars* S) {
     __shared__ double sh[16];
@@ -368,7 +383,12 @@ __global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T
     const double alpha = alpha_sh;
     const T a = (T)alpha, na = (T)(-alpha);
     double m = 0.0;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    const unsigned stride = gridDim.x * blockDim.x;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
+    bw_init(bx, t, kq, off);
+    for (; t < bx.n; t += stride) {
+        const long c = bw_cell(bx, kq, off);
+        bw_next(bx, kq, off);
         x[c] += a * s[c];
         T rv = r[c] + na * z[c];
         r[c] = rv;
@@ -380,12 +400,19 @@ __global__ void k_axpy_max(long n, T* __restrict__ x, T* __restrict__ r, const T
 }
 
 template <class T>
-__global__ void k_dot(long n, const T* __restrict__ a, const T* __restrict__ b, double* partial, const PcgScalars* S) {
+__global__ void k_dot(BoxRange bx, const T* __restrict__ a, const T* __restrict__ b, double* partial,
+                      const PcgScalars* S) {
     __shared__ double sh[16];
     if (S && S->done) return;
     double acc = 0.0;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    const unsigned stride = gridDim.x * blockDim.x;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
+    bw_init(bx, t, kq, off);
+    for (; t < bx.n; t += stride) {
+        const long c = bw_cell(bx, kq, off);
+        bw_next(bx, kq, off);
         acc += (double)a[c] * (double)b[c];
+    }
     acc = block_reduce<double, false>(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -412,7 +439,7 @@ __global__ void k_absmax(long n, const T* __restrict__ a, double* partial) {
 template <class T>
 __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __restrict__ z,
                            const T* __restrict__ s_old, T* __restrict__ s_new, T* __restrict__ y,
-                           const double* __restrict__ pin, int npin, double* partial, PcgScalars* S) {
+                           const double* __restrict__ pin, int npin, double* partial, PcgScalars* S, BoxRange bx) {
     __shared__ double sh[16];
     __shared__ double beta_sh;
     if (S->done) return;
@@ -425,26 +452,35 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
     const T beta = (T)beta_sh;
     auto sv = [=](long e) __attribute__((always_inline)) { return z[e] + beta * s_old[e]; };
     double acc = 0.0;
-    const long stride = (long)gridDim.x * blockDim.x;
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    for (; c + 3 * stride < g.N; c += 4 * stride) {
+    const unsigned stride = gridDim.x * blockDim.x;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
+    bw_init(bx, t, kq, off);
+    for (; t + 3 * stride < bx.n; t += 4 * stride) {
+        long cs[4];
         uint16_t cc[4];
         T yv[4], sc[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cc[u] = code[c + u * stride];
+        for (int u = 0; u < 4; ++u) {
+            cs[u] = bw_cell(bx, kq, off);
+            bw_next(bx, kq, off);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cc[u] = code[cs[u]];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            yv[u] = spmv_cell_g<T>(g, cc[u], c + u * stride, sv);
-            sc[u] = sv(c + u * stride);
+            yv[u] = spmv_cell_g<T>(g, cc[u], cs[u], sv);
+            sc[u] = sv(cs[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            s_new[c + u * stride] = sc[u];
-            y[c + u * stride] = yv[u];
+            s_new[cs[u]] = sc[u];
+            y[cs[u]] = yv[u];
             acc += (double)sc[u] * (double)yv[u];
         }
     }
-    for (; c < g.N; c += stride) {
+    for (; t < bx.n; t += stride) {
+        const long c = bw_cell(bx, kq, off);
+        bw_next(bx, kq, off);
         const T yv = spmv_cell_g<T>(g, code[c], c, sv), sc = sv(c);
         s_new[c] = sc;
         y[c] = yv;
@@ -526,10 +562,13 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     const int tm_start = tm ? tm->used : 0;
     if (tm) tm->k = 0;
     double* partial2 = partial + RED_BLOCKS;
+    // the iteration kernels walk only the rows of the active rows' bounding box
+    // (every vector is zero elsewhere: b has no rows there and x starts at zero)
+    const BoxRange bxr = sk ? box_range(g.ni, g.nj, g.nk, sk->box, RED_BLOCKS * XF_BLOCK) : full_range(g);
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
-    if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
-        k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
+    if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &bxr))
+        k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
     k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, RED_BLOCKS, S, tol_factor, 0);
     k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
     const int CH = 4;
@@ -548,20 +587,20 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
         for (; it < end; ++it) {
             // (the spmv and dot partials go to `partial`, the residual's to `partial2`)
             if (it == 0) {
-                k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, sc, q, partial, S);
+                k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, sc, q, partial, S, bxr);
             } else {
                 k_spmv_upd<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, z, sc, sn, q, partial, RED_BLOCKS,
-                                                               partial2, S);
+                                                               partial2, S, bxr);
                 T* t = sc;
                 sc = sn;
                 sn = t;
             }
-            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
+            k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
                                                             it == 0 ? partial2 : partial, S);
             k_finish<<<1, 1024, 0, st>>>(FIN_RES, it == 0 ? partial2 : partial, RED_BLOCKS, S, tol_factor, it,
                                          it == end - 1 ? slot : nullptr);
-            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial))
-                k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, r, partial, S);
+            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &bxr))
+                k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
         }
         hipEventRecord(chunk_ev[c & 1], st);
     };

@@ -596,14 +596,21 @@ __global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restr
 // z from the oct layout as k_from_oct, and the block partials of dot(z, r)
 // over the same grid-stride cells in the same order as k_dot (pcg_solver.h:
 // 273, rho = dot(z, r)), so the partials are bit-identical to k_dot's
+// over the rows of the box (BoxRange bx); z elsewhere is left as it is (the
+// PCG kernels never read it: they walk the same rows)
 template <class T>
 __global__ void k_from_oct_dot(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ z, const T* __restrict__ r,
-                               double* __restrict__ partial, const PcgScalars* S, Box8 b) {
+                               double* __restrict__ partial, const PcgScalars* S, Box8 b, BoxRange bx) {
     __shared__ double sh[16];
     if (S && S->done) return;
     double acc = 0.0;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    const unsigned stride = gridDim.x * blockDim.x;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
+    bw_init(bx, t, kq, off);
+    for (; t < bx.n; t += stride) {
+        const long c = bw_cell(bx, kq, off);
+        const int k = b.k0 + (int)kq, j = b.j0 + (int)(off / (unsigned)g.ni), i = (int)(off % (unsigned)g.ni);
+        bw_next(bx, kq, off);
         T v = (T)0;
         if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
             const long t = (long)(j >> 3) * ntk + (k >> 3);
@@ -1212,7 +1219,7 @@ static bool w8_fused_fits(int tiles) {
 template <class T>
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
-                      int dot_blocks) {
+                      int dot_blocks, const BoxRange* dot_range) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
@@ -1236,8 +1243,8 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     }
     if (t1) hipEventRecord(t1, st);
     const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
-    if (dot_partial)
-        k_from_oct_dot<T><<<dot_blocks, XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, r, dot_partial, S, bx);
+    if (dot_partial && dot_range)
+        k_from_oct_dot<T><<<dot_blocks, XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, r, dot_partial, S, bx, *dot_range);
     else
         k_from_oct<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, S, bx);
     return true;
@@ -1246,6 +1253,8 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
 template void sweep_prepare_w8<double>(const Geo&, const uint16_t*, const double*, const SweepSkew<double>&, hipStream_t);
 template void sweep_prepare_w8<float>(const Geo&, const uint16_t*, const float*, const SweepSkew<float>&, hipStream_t);
 template bool sweep_precond_w8<double>(const Geo&, const SweepSkew<double>&, const double*, double*, const PcgScalars*,
-                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int);
+                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int,
+                                       const BoxRange*);
 template bool sweep_precond_w8<float>(const Geo&, const SweepSkew<float>&, const float*, float*, const PcgScalars*,
-                                      int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int);
+                                      int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int,
+                                      const BoxRange*);
