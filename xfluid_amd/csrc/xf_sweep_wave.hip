@@ -39,6 +39,8 @@
 // zero.  Edge rows: per tile and step 32 words, 0-15 the tile's edge column
 // (forward jl = 15, backward jl = 0) by kl, 16-31 the edge row (forward
 // kl = 15, backward kl = 0) by jl.
+#include <hip/hip_ext.h>
+
 #include "xf_internal.h"
 
 #define WV_PAD 30
@@ -1234,14 +1236,17 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     const W8Planes<T> b{sk.oQ, sk.oZ, sk.oRec + 4 * n, sk.oEdge};
     const int ntjb = J1 - J0 + 1, ntkb = K1 - K0 + 1, nS = s1 - s0 + 1;
     const int tiles = ntjb * ntkb;
-    if (t0) hipEventRecord(t0, st);
+    // t0 / t1 ride on the dispatch packets themselves, so they time the kernels and not
+    // the marker packets (and their cache release) that hipEventRecord would add
     if (w8_fused_fits<T>(tiles)) {
-        k_sweep_w8_fb<T><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, b, S, tag1, tag2, flags);
+        hipExtLaunchKernelGGL(k_sweep_w8_fb<T>, dim3(tiles), dim3(192), 0, st, t0, t1, 0, g, ntk, J0, K0, ntjb,
+                              ntkb, s0, nS, f, b, S, tag1, tag2, flags);
     } else {
-        k_sweep_w8<T, true><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags);
-        k_sweep_w8<T, false><<<tiles, 192, 0, st>>>(g, ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags);
+        hipExtLaunchKernelGGL(k_sweep_w8<T, true>, dim3(tiles), dim3(192), 0, st, t0, (hipEvent_t) nullptr, 0, g,
+                              ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags, 0.0, 0.0);
+        hipExtLaunchKernelGGL(k_sweep_w8<T, false>, dim3(tiles), dim3(192), 0, st, (hipEvent_t) nullptr, t1, 0, g,
+                              ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags, 0.0, 0.0);
     }
-    if (t1) hipEventRecord(t1, st);
     const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
     if (dot_partial && dot_range)
         k_from_oct_dot<T><<<dot_blocks, XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, r, dot_partial, S, bx, *dot_range);
