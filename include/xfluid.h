@@ -142,6 +142,22 @@ int  xf_sim_add_sphere_fluid(xf_sim* s, double x, double y, double z, double r);
 int  xf_sim_add_cubic_fluid(xf_sim* s, const double lu[3], const double rd[3]);
 int  xf_sim_add_constant_force(xf_sim* s, double fx, double fy, double fz);
 
+/* ---- fluid sources: addFluidSource (FluidSimulator.h:53; FluidSimulator.cpp:152-166)
+ *      and CubicFluidSource (fluidsource.cpp:8-33, 75-92).  The source cells are
+ *      fixed when the source is added: the cells from (int)((c - w) / dx) to
+ *      (int)((c + w) / dx) per axis, i slowest, that are in range and not solid
+ *      at that moment.  After every substep's advection each source marks its
+ *      cells fluid and appends (int)(8 |v| dt / dx) particles per cell, with
+ *      velocity v, at the cell centre plus randomDouble(-0.4999 dx, 0.4999 dx)
+ *      per axis (xfluidutil.cpp:67-70) drawn in x, y, z order from a 31-bit
+ *      SplitMix64 stream (RAND_MAX = 2^31 - 1) in place of the reference's
+ *      unseeded rand().  Particle count and ids grow; ids continue in emission
+ *      order as the reference's push_back does. */
+int  xf_sim_add_fluid_source(xf_sim* s, const double center[3], double wx, double wy, double wz,
+                             const double velocity[3]);
+/* Seed of the source stream (default 0); also restarts the stream. */
+int  xf_sim_set_source_seed(xf_sim* s, uint64_t seed);
+
 /* ---- init: initParticles / init (FluidSimulator.h:61-66;
  *      FluidSimulator.cpp:168-222).  The reference seeds with unseeded rand();
  *      here the jitter comes from SplitMix64(seed) so runs are reproducible. */

@@ -349,6 +349,14 @@ struct Sim {
     std::vector<uint8_t> cube;   // their marching-cubes case
     xf_solve_stats last{};
     int frame = 0;
+    // fluid sources (fluidsource.h:10-53): cells fixed at creation, velocity
+    struct Source {
+        std::vector<long> cells;
+        double v[3];
+        double normv;
+    };
+    std::vector<Source> sources;
+    uint64_t src_state = 0;  // 31-bit SplitMix64 stream in place of rand()
 
     long N() const { return (long)ni * nj * nk; }
     long nu() const { return (long)(ni + 1) * nj * nk; }
@@ -822,6 +830,53 @@ struct Sim {
     }
 
     // FluidSimulator.cpp:652-723 (G2P blend, RK4 advection, collision)
+    // FluidSimulator::addFluidSource (FluidSimulator.cpp:152-166) +
+    // CubicFluidSource::setCubic (fluidsource.cpp:8-33) and setVelocity (:35-39):
+    // the cells between pos2Index(c - w) and pos2Index(c + w), i slowest, in range
+    // and not solid now
+    void add_fluid_source(const double* c, double wx, double wy, double wz, const double* vel) {
+        Source src;
+        const double lo[3] = {c[0] - wx, c[1] - wy, c[2] - wz}, hi[3] = {c[0] + wx, c[1] + wy, c[2] + wz};
+        const int gi0 = (int)(lo[0] / dx), gj0 = (int)(lo[1] / dx), gk0 = (int)(lo[2] / dx);
+        const int gi1 = (int)(hi[0] / dx), gj1 = (int)(hi[1] / dx), gk1 = (int)(hi[2] / dx);
+        for (int i = gi0; i <= gi1; ++i)
+            for (int j = gj0; j <= gj1; ++j)
+                for (int k = gk0; k <= gk1; ++k)
+                    if (mat_in(i, j, k) && M(i, j, k) != SOLID) src.cells.push_back(i + (long)ni * j + (long)ni * nj * k);
+        for (int a = 0; a < 3; ++a) src.v[a] = vel[a];
+        src.normv = std::sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+        sources.push_back(src);
+    }
+    // randomDouble (xfluidutil.cpp:67-70) over a 31-bit draw, RAND_MAX = 2^31 - 1
+    static double random_double(double lo, double hi, uint32_t r) {
+        return lo + (double)r / ((double)2147483647 / (hi - lo));
+    }
+    // _applyFluidSource (FluidSimulator.cpp:1087-1093) -> CubicFluidSource::update
+    // (fluidsource.cpp:75-92).  Draws x, y, z per particle from the stream, or from
+    // `draws` (three per particle, in that order) when given.
+    int apply_sources(double dt, const uint32_t* draws = nullptr) {
+        long d = 0;
+        for (const Source& src : sources) {
+            const double x = 8.0 * src.normv * dt / dx;  // _particlePerCell * _normv * dt / _dx
+            if (!(x < 2147483647.0)) return -1;
+            const int nnew = x > 0 ? (int)x : 0;
+            for (long c : src.cells) {
+                mat[c] = FLUID;
+                const int i = (int)(c % ni), j = (int)((c / ni) % nj), k = (int)(c / ((long)ni * nj));
+                for (int q = 0; q < nnew; ++q) {
+                    double p[3] = {(i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx};  // idx2Pos
+                    for (int a = 0; a < 3; ++a) {
+                        const uint32_t r = draws ? draws[d++] : (uint32_t)(splitmix64(src_state) >> 33);
+                        p[a] = p[a] + random_double(-0.4999 * dx, 0.4999 * dx, r);
+                    }
+                    px.push_back(p[0]); py.push_back(p[1]); pz.push_back(p[2]);
+                    vx.push_back(src.v[0]); vy.push_back(src.v[1]); vz.push_back(src.v[2]);
+                }
+            }
+        }
+        return 0;
+    }
+
     void advect(double dt) {
         const long n = (long)px.size();
         const long lim = prm.tail_skip ? (n / 6) * 6 : n;
@@ -1124,7 +1179,7 @@ struct Sim {
             case XF_STAGE_SOLVE: return solve();
             case XF_STAGE_APPLY: apply(); return 0;
             case XF_STAGE_EXTRAPOLATE: extrapolate(); return 0;
-            case XF_STAGE_ADVECT: advect(dt); return 0;
+            case XF_STAGE_ADVECT: advect(dt); return apply_sources(dt);
             case XF_STAGE_P2G: p2g(); return 0;
         }
         return -1;
@@ -1140,6 +1195,7 @@ struct Sim {
         apply();
         extrapolate();
         advect(dt);
+        if (apply_sources(dt) < 0) return -1;
         p2g();
         return st;
     }
@@ -1269,6 +1325,12 @@ int xo_get_field(void* h, int which, void* dst, size_t bytes) {
     return 0;
 }
 int xo_stage(void* h, int stage, double dt) { return ((Sim*)h)->stage(stage, dt); }
+void xo_add_fluid_source(void* h, const double* c, double wx, double wy, double wz, const double* v) {
+    ((Sim*)h)->add_fluid_source(c, wx, wy, wz, v);
+}
+void xo_set_source_seed(void* h, uint64_t seed) { ((Sim*)h)->src_state = seed; }
+// one _applyFluidSource call with the given 31-bit draws (3 per new particle)
+int xo_apply_sources(void* h, double dt, const uint32_t* draws) { return ((Sim*)h)->apply_sources(dt, draws); }
 int xo_time_step(void* h, double dt, int mesh) { return ((Sim*)h)->time_step(dt, mesh != 0); }
 int xo_frame_step(void* h, double dt) { return ((Sim*)h)->frame_step(dt); }
 void xo_last_solve(void* h, xf_solve_stats* st) { *st = ((Sim*)h)->last; }

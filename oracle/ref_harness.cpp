@@ -105,6 +105,22 @@ void xr_add_cubic_fluid(void* h, const double* lu, const double* rd) {
 void xr_add_sphere_fluid(void* h, double x, double y, double z, double rad) { ((Ref*)h)->sim.addSpereFluid(x, y, z, rad); }
 void xr_add_constant_force(void* h, double fx, double fy, double fz) { ((Ref*)h)->sim.addConstantForce(fx, fy, fz); }
 
+// FluidSimulator::addFluidSource (FluidSimulator.cpp:152-166)
+void xr_add_fluid_source(void* h, const double* c, double wx, double wy, double wz, const double* v) {
+    ((Ref*)h)->sim.addFluidSource(Vec3d(c[0], c[1], c[2]), wx, wy, wz, Vec3d(v[0], v[1], v[2]));
+}
+// _applyFluidSource (FluidSimulator.cpp:1087-1093) with rand() restarted at srand(seed)
+void xr_apply_sources(void* h, double dt, unsigned seed) {
+    srand(seed);
+    ((Ref*)h)->sim._applyFluidSource(dt);
+}
+// the first n values rand() returns after srand(seed), and RAND_MAX
+int xr_rand(unsigned seed, int64_t n, uint32_t* out) {
+    srand(seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = (uint32_t)rand();
+    return RAND_MAX;
+}
+
 // FluidSimulator::init (FluidSimulator.cpp:186-222) minus initParticles, whose
 // rand() jitter is replaced by injected particles (xr_set_particles).
 void xr_init(void* h) {

@@ -56,6 +56,9 @@ def _load_oracle():
     lib.xo_add_sphere_fluid.argtypes = [C.c_void_p] + [C.c_double] * 4
     lib.xo_add_cubic_fluid.argtypes = [C.c_void_p, _dp, _dp]
     lib.xo_add_constant_force.argtypes = [C.c_void_p] + [C.c_double] * 3
+    lib.xo_add_fluid_source.argtypes = [C.c_void_p, _dp] + [C.c_double] * 3 + [_dp]
+    lib.xo_set_source_seed.argtypes = [C.c_void_p, C.c_uint64]
+    lib.xo_apply_sources.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
     lib.xo_seed_particles.argtypes = [C.c_void_p, C.c_int, C.c_uint64]
     lib.xo_num_particles.restype = C.c_int64
     lib.xo_num_particles.argtypes = [C.c_void_p]
@@ -107,6 +110,9 @@ def _load_ref():
     lib.xr_add_cubic_fluid.argtypes = [C.c_void_p, _dp, _dp]
     lib.xr_add_sphere_fluid.argtypes = [C.c_void_p] + [C.c_double] * 4
     lib.xr_add_constant_force.argtypes = [C.c_void_p] + [C.c_double] * 3
+    lib.xr_add_fluid_source.argtypes = [C.c_void_p, _dp] + [C.c_double] * 3 + [_dp]
+    lib.xr_apply_sources.argtypes = [C.c_void_p, C.c_double, C.c_uint]
+    lib.xr_rand.argtypes = [C.c_uint, C.c_int64, C.c_void_p]
     lib.xr_init.argtypes = [C.c_void_p]
     lib.xr_set_particles.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6
     lib.xr_num_particles.restype = C.c_int64
@@ -225,6 +231,19 @@ class OracleSim:
     def stage(self, s, dt=0.0):
         return self.lib.xo_stage(self.h, s, dt)
 
+    def add_fluid_source(self, c, wx, wy, wz, v):
+        self.lib.xo_add_fluid_source(self.h, np.asarray(c, np.float64), wx, wy, wz, np.asarray(v, np.float64))
+
+    def set_source_seed(self, seed):
+        self.lib.xo_set_source_seed(self.h, seed)
+
+    def apply_sources(self, dt, draws=None):
+        """One _applyFluidSource; `draws` (uint32, 3 per new particle) replaces the stream."""
+        if draws is None:
+            return self.lib.xo_apply_sources(self.h, dt, None)
+        d = np.ascontiguousarray(draws, np.uint32)
+        return self.lib.xo_apply_sources(self.h, dt, d.ctypes.data)
+
     def time_step(self, dt, mesh=False):
         return self.lib.xo_time_step(self.h, dt, int(mesh))
 
@@ -301,6 +320,13 @@ class RefSim:
     def time_step(self, dt):
         return self.lib.xr_time_step(self.h, dt)
 
+    def add_fluid_source(self, c, wx, wy, wz, v):
+        self.lib.xr_add_fluid_source(self.h, np.asarray(c, np.float64), wx, wy, wz, np.asarray(v, np.float64))
+
+    def apply_sources(self, dt, seed):
+        """One _applyFluidSource of the reference, rand() restarted at srand(seed)."""
+        self.lib.xr_apply_sources(self.h, dt, seed)
+
     def frame_step(self, dt):
         return self.lib.xr_frame_step(self.h, dt)
 
@@ -324,6 +350,13 @@ class RefSim:
         cells = np.empty(max(n, 1), np.int32)
         self.lib.xr_surface_cells(self.h, cells, n)
         return cells[:n]
+
+
+def ref_rand(seed, n):
+    """The first n values of the reference build's rand() after srand(seed), and RAND_MAX."""
+    out = np.empty(max(n, 1), np.uint32)
+    rmax = ref().xr_rand(seed, n, out.ctypes.data)
+    return out[:n], rmax
 
 
 def dam_break_scene(ni, nj, nk, dx, fx, fy, fz=1.0):

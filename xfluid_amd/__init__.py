@@ -89,6 +89,8 @@ def lib():
     L.xf_sim_add_sphere_fluid.argtypes = [V] + [C.c_double] * 4
     L.xf_sim_add_cubic_fluid.argtypes = [V, _dp, _dp]
     L.xf_sim_add_constant_force.argtypes = [V] + [C.c_double] * 3
+    L.xf_sim_add_fluid_source.argtypes = [V, _dp] + [C.c_double] * 3 + [_dp]
+    L.xf_sim_set_source_seed.argtypes = [V, C.c_uint64]
     L.xf_sim_seed_particles.argtypes = [V, C.c_int, C.c_uint64]
     L.xf_sim_init.argtypes = [V]
     L.xf_sim_frame_step.argtypes = [V, C.c_double]
@@ -189,6 +191,16 @@ class FluidSimulator:
 
     def addConstantForce(self, fx, fy, fz):
         _chk(self.L.xf_sim_add_constant_force(self.h, fx, fy, fz), "addConstantForce")
+
+    def addFluidSource(self, center, wx, wy, wz, velocity):
+        """FluidSimulator::addFluidSource (FluidSimulator.cpp:152-166): a cubic source
+        of half-widths (wx, wy, wz) around `center`, emitting with `velocity`."""
+        _chk(self.L.xf_sim_add_fluid_source(self.h, np.asarray(center, np.float64), wx, wy, wz,
+                                            np.asarray(velocity, np.float64)), "addFluidSource")
+
+    def set_source_seed(self, seed):
+        """Seed (and restart) the fluid sources' jitter stream."""
+        _chk(self.L.xf_sim_set_source_seed(self.h, seed), "set_source_seed")
 
     def initParticles(self, mode=SEED_REFERENCE, seed=0):
         _chk(self.L.xf_sim_seed_particles(self.h, mode, seed), "initParticles")

@@ -81,6 +81,15 @@ struct xf_sim {
     double *ga_u = nullptr, *ga_v = nullptr, *ga_w = nullptr, *ga_p = nullptr;  // adjoint scratch
     bool mesh_out = false;
     std::string mesh_prefix;
+    // fluid sources (fluidsource.h:10-53): cells fixed at creation, velocity, |v|
+    struct Source {
+        int32_t* cells = nullptr;
+        int ncells = 0;
+        double v[3] = {0, 0, 0};
+        double normv = 0;
+    };
+    std::vector<Source> sources;
+    uint64_t source_seed = 0, source_emitted = 0;  // stream seed, particles emitted so far
 
     long nu() const { return (long)(g.ni + 1) * g.nj * g.nk; }
     long nv() const { return (long)g.ni * (g.nj + 1) * g.nk; }
@@ -114,6 +123,68 @@ int ensure_particles(xf_sim* s, int64_t n) {
     HIPCHK(dalloc(&P.pid, n)); HIPCHK(dalloc(&P.tpid, n)); HIPCHK(dalloc(&P.key, n)); HIPCHK(dalloc(&P.perm, n));
     P.cap = n;
     P.n = n;
+    return 0;
+}
+
+// grow the particle arrays to hold `need` particles, keeping the first P.n
+// (capacity grows by at least half, so a source appending every substep
+// reallocates rarely); the sort scratch is reallocated without copying
+int grow_particles(xf_sim* s, int64_t need) {
+    Particles& P = s->P;
+    if (need <= P.cap) return 0;
+    const int64_t cap = std::max<int64_t>(need, P.cap + P.cap / 2);
+    double** keep[] = {&P.px, &P.py, &P.pz, &P.vx, &P.vy, &P.vz};
+    double** scratch[] = {&P.tx, &P.ty, &P.tz, &P.tvx, &P.tvy, &P.tvz};
+    int32_t** iscratch[] = {&P.tpid, &P.key, &P.perm};
+    double* nk[6] = {};
+    int32_t* npid = nullptr;
+    bool ok = dalloc(&npid, cap) == hipSuccess;
+    for (int a = 0; a < 6; ++a) ok = ok && dalloc(&nk[a], cap) == hipSuccess;
+    if (!ok) {
+        for (auto& q : nk) dfree(q);
+        dfree(npid);
+        return set_err(-2, "particle allocation failed (%lld particles)", (long long)cap);
+    }
+    for (int a = 0; a < 6; ++a)
+        if (P.n) HIPCHK(hipMemcpyAsync(nk[a], *keep[a], P.n * 8, hipMemcpyDeviceToDevice, s->st));
+    if (P.n) HIPCHK(hipMemcpyAsync(npid, P.pid, P.n * 4, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (int a = 0; a < 6; ++a) { dfree(*keep[a]); *keep[a] = nk[a]; }
+    dfree(P.pid);
+    P.pid = npid;
+    for (auto* q : scratch) { dfree(*q); HIPCHK(dalloc(q, cap)); }
+    for (auto* q : iscratch) { dfree(*q); HIPCHK(dalloc(q, cap)); }
+    P.cap = cap;
+    return 0;
+}
+
+// _applyFluidSource (FluidSimulator.cpp:1087-1093) at the end of the advection
+// stage (FluidSimulator.cpp:725-726): sources in creation order, each appending
+// (int)(8 |v| dt / dx) particles per cell (fluidsource.cpp:77)
+int apply_sources(xf_sim* s, double dt) {
+    if (s->sources.empty()) return 0;
+    const Geo& g = s->g;
+    std::vector<int> per(s->sources.size());
+    int64_t add = 0;
+    for (size_t q = 0; q < s->sources.size(); ++q) {
+        const double x = 8.0 * s->sources[q].normv * dt / g.dx;  // _particlePerCell * _normv * dt / _dx
+        if (!(x < 2147483647.0)) return set_err(-1, "fluid source %d: %g particles per cell", (int)q, x);
+        per[q] = x > 0 ? (int)x : 0;
+        add += (int64_t)per[q] * s->sources[q].ncells;
+    }
+    int64_t n = s->P.n;
+    if (n + add > 2147483647LL) return set_err(-1, "fluid sources: particle count would pass 2^31 - 1");
+    if (grow_particles(s, n + add)) return -2;
+    for (size_t q = 0; q < s->sources.size(); ++q) {
+        const auto& src = s->sources[q];
+        launch_source(g, s->P, s->mat, src.cells, src.ncells, per[q], src.v, s->source_seed, s->source_emitted, n,
+                      s->st);
+        n += (int64_t)per[q] * src.ncells;
+        s->source_emitted += (uint64_t)per[q] * src.ncells;
+    }
+    HIPCHK(hipGetLastError());
+    s->P.n = n;
+    s->sorted = false;
     return 0;
 }
 
@@ -266,7 +337,7 @@ int st_advect(xf_sim* s, double dt) {
                       s->prm.integrator, limit, s->st);
     HIPCHK(hipGetLastError());
     s->sorted = false;
-    return 0;
+    return apply_sources(s, dt);
 }
 int st_p2g(xf_sim* s) {
     int rc = ensure_sorted(s);
@@ -438,6 +509,7 @@ void xf_sim_destroy(xf_sim* s) {
     dfree(s->pw.sk64.oRec); dfree(s->pw.sk64.oR); dfree(s->pw.sk64.oQ); dfree(s->pw.sk64.oZ); dfree(s->pw.sk64.oEdge);
     dfree(s->pw.sk32.oRec); dfree(s->pw.sk32.oR); dfree(s->pw.sk32.oQ); dfree(s->pw.sk32.oZ); dfree(s->pw.sk32.oEdge);
     mesh_free(s->mesh);
+    for (auto& src : s->sources) dfree(src.cells);
     dfree(s->ga_u); dfree(s->ga_v); dfree(s->ga_w); dfree(s->ga_p);
     if (s->pw.timer.ev) {
         for (int i = 0; i < s->pw.timer.cap; ++i) hipEventDestroy(s->pw.timer.ev[i]);
@@ -514,6 +586,45 @@ int xf_sim_add_cubic_fluid(xf_sim* s, const double lu[3], const double rd[3]) {
 int xf_sim_add_constant_force(xf_sim* s, double fx, double fy, double fz) {
     if (check(s)) return -1;
     s->forces.push_back(fx); s->forces.push_back(fy); s->forces.push_back(fz);
+    return 0;
+}
+
+// FluidSimulator::addFluidSource (FluidSimulator.cpp:152-166) with
+// CubicFluidSource::setCubic / setVelocity (fluidsource.cpp:8-39)
+int xf_sim_add_fluid_source(xf_sim* s, const double c[3], double wx, double wy, double wz, const double vel[3]) {
+    if (check(s) || !c || !vel) return set_err(-1, "null argument");
+    if (download_mat(s)) return -2;
+    const Geo& g = s->g;
+    const double lo[3] = {c[0] - wx, c[1] - wy, c[2] - wz}, hi[3] = {c[0] + wx, c[1] + wy, c[2] + wz};
+    int a[3], b[3];
+    for (int d = 0; d < 3; ++d) {
+        const double l = lo[d] / g.dx, h = hi[d] / g.dx;
+        if (!(l > -2147483648.0 && l < 2147483647.0 && h > -2147483648.0 && h < 2147483647.0))
+            return set_err(-1, "fluid source box outside the index range");
+        a[d] = (int)l;  // pos2Index truncates (xfluidutil.cpp:3-6)
+        b[d] = (int)h;
+    }
+    std::vector<int32_t> cells;
+    for (int i = std::max(a[0], 0); i <= b[0] && i < g.ni; ++i)
+        for (int j = std::max(a[1], 0); j <= b[1] && j < g.nj; ++j)
+            for (int k = std::max(a[2], 0); k <= b[2] && k < g.nk; ++k)
+                if (s->h_mat[cidx(g, i, j, k)] != XF_SOLID) cells.push_back((int32_t)cidx(g, i, j, k));
+    xf_sim::Source src;
+    src.v[0] = vel[0]; src.v[1] = vel[1]; src.v[2] = vel[2];
+    src.normv = std::sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);  // Vec3::norm
+    src.ncells = (int)cells.size();
+    if (src.ncells) {
+        HIPCHK(dalloc(&src.cells, cells.size()));
+        HIPCHK(hipMemcpyAsync(src.cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    }
+    s->sources.push_back(src);
+    return 0;
+}
+int xf_sim_set_source_seed(xf_sim* s, uint64_t seed) {
+    if (check(s)) return -1;
+    s->source_seed = seed;
+    s->source_emitted = 0;
     return 0;
 }
 

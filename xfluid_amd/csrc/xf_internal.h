@@ -152,6 +152,10 @@ void sort_particles(const Geo& g, Particles& P, CellIndex& ci, hipStream_t st);
 void launch_g2p_advect(const Geo& g, Particles& P, const uint8_t* mat, const double* u, const double* v,
                        const double* w, const double* us, const double* vs, const double* ws, double dt, double flip,
                        int integrator, int64_t limit_id, hipStream_t st);
+// one fluid source's update: mark its cells fluid, write ncells * per_cell
+// particles at n0.. (the caller has grown P)
+void launch_source(const Geo& g, Particles& P, uint8_t* mat, const int32_t* cells, int ncells, int per_cell,
+                   const double v[3], uint64_t seed, uint64_t m0, int64_t n0, hipStream_t st);
 void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f, int axis, int kernel,
                 hipStream_t st);
 void launch_density(const Geo& g, const Particles& P, const CellIndex& ci, double* phi, int d, double radius,

@@ -272,6 +272,58 @@ void launch_g2p_advect(const Geo& g, Particles& P, const uint8_t* mat, const dou
 }
 
 // ---------------------------------------------------------------------------
+// CubicFluidSource::update (fluidsource.cpp:75-92), applied at the end of
+// _advectParticles (FluidSimulator.cpp:725-726).  Thread t of one source's
+// launch writes particle n0 + t: cell cells[t / per_cell], position = cell
+// centre (idx2Pos, xfluidutil.cpp:18-21) + randomDouble(-0.4999 dx, 0.4999 dx)
+// per axis (xfluidutil.cpp:67-70), velocity = the source's, id = n0 + t.  The
+// draws come from a 31-bit SplitMix64 stream in place of rand() (RAND_MAX =
+// 2^31 - 1): draw d is mix(seed + (d + 1) * 0x9E3779B97F4A7C15), so thread t
+// takes draws 3 (m0 + t) + 0, 1, 2 (x, y, z) without walking the stream.
+__device__ inline uint32_t source_draw(uint64_t seed, uint64_t d) {
+    uint64_t z = seed + (d + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)((z ^ (z >> 31)) >> 33);
+}
+
+__global__ void k_source_emit(Geo g, const int32_t* __restrict__ cells, long total, int per_cell, double svx,
+                              double svy, double svz, uint64_t seed, uint64_t m0, long n0, double* px, double* py,
+                              double* pz, double* vx, double* vy, double* vz, int32_t* pid) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const long c = cells[t / per_cell];
+    const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    const double lo = -0.4999 * g.dx, hi = 0.4999 * g.dx;
+    const double scale = 2147483647.0 / (hi - lo);
+    const uint64_t d = 3 * (m0 + (uint64_t)t);
+    const double jx = lo + (double)source_draw(seed, d) / scale;
+    const double jy = lo + (double)source_draw(seed, d + 1) / scale;
+    const double jz = lo + (double)source_draw(seed, d + 2) / scale;
+    const long q = n0 + t;
+    px[q] = (i + 0.5) * g.dx + jx;
+    py[q] = (j + 0.5) * g.dx + jy;
+    pz[q] = (k + 0.5) * g.dx + jz;
+    vx[q] = svx; vy[q] = svy; vz[q] = svz;
+    pid[q] = (int32_t)q;
+}
+
+__global__ void k_source_mark(uint8_t* mat, const int32_t* __restrict__ cells, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) mat[cells[t]] = XF_FLUID;
+}
+
+void launch_source(const Geo& g, Particles& P, uint8_t* mat, const int32_t* cells, int ncells, int per_cell,
+                   const double v[3], uint64_t seed, uint64_t m0, int64_t n0, hipStream_t st) {
+    if (ncells <= 0) return;
+    k_source_mark<<<(ncells + XF_BLOCK - 1) / XF_BLOCK, XF_BLOCK, 0, st>>>(mat, cells, ncells);
+    const long total = (long)ncells * per_cell;
+    if (total <= 0) return;
+    k_source_emit<<<(unsigned)((total + XF_BLOCK - 1) / XF_BLOCK), XF_BLOCK, 0, st>>>(
+        g, cells, total, per_cell, v[0], v[1], v[2], seed, m0, n0, P.px, P.py, P.pz, P.vx, P.vy, P.vz, P.pid);
+}
+
+// ---------------------------------------------------------------------------
 // P2G, _updateVelocityByParticles (FluidSimulator.cpp:909-1085), as a gather:
 // face f = sum_p W(|q_p - c_f|^2) v_p / sum_p W, q_p = p + dx/2 e_axis,
 // c_f = idx2Pos(f) -- the same per-pair arithmetic as the reference; only the

@@ -55,9 +55,10 @@ public:
     }
     void addConstantForce(const Vec3d& f) { addConstantForce(f.x, f.y, f.z); }
     void addConstantForce(double fx, double fy, double fz) { report(xf_sim_add_constant_force(_sim, fx, fy, fz)); }
-    // Fluid sources (FluidSimulator.cpp:152-166) are not part of this path.
-    void addFluidSource(const Vec3d&, double, double, double, const Vec3d&) {
-        std::fprintf(stderr, "FluidSimulator::addFluidSource: fluid sources are not supported by this build\n");
+    // FluidSimulator.cpp:152-166 (jitter stream seeded by setSeed)
+    void addFluidSource(const Vec3d& c, double wx, double wy, double wz, const Vec3d& v) {
+        double cc[3] = {c.x, c.y, c.z}, vv[3] = {v.x, v.y, v.z};
+        report(xf_sim_add_fluid_source(_sim, cc, wx, wy, wz, vv));
     }
     void initParticles() { report(xf_sim_seed_particles(_sim, XF_SEED_REFERENCE, _seed)); }
     void init() { report(xf_sim_init(_sim)); }
@@ -77,7 +78,11 @@ public:
     // -- beyond the reference interface --
     xf_sim* handle() { return _sim; }
     void setParams(const xf_params& p) { report(xf_sim_set_params(_sim, &p)); }
-    void setSeed(uint64_t s) { _seed = s; }
+    // seed of initParticles' jitter and of the fluid sources' stream
+    void setSeed(uint64_t s) {
+        _seed = s;
+        report(xf_sim_set_source_seed(_sim, s));
+    }
 
 private:
     void report(int rc) {
