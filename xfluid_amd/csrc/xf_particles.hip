@@ -101,33 +101,31 @@ __global__ void k_keys(Geo g, long n, const double* px, const double* py, const 
     key[p] = kk;
     atomicAdd(&cnt[kk], 1);
 }
-__global__ void k_scatter(long n, const int32_t* key, const int32_t* start, int32_t* cursor, int32_t* perm) {
+// each particle's original index into a free slot of its cell's range
+__global__ void k_scatter_ids(long n, const int32_t* __restrict__ key, const int32_t* __restrict__ pid,
+                              const int32_t* __restrict__ start, int32_t* cursor, int32_t* ids) {
     long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (p >= n) return;
     int32_t kk = key[p];
-    perm[start[kk] + atomicAdd(&cursor[kk], 1)] = (int32_t)p;
+    ids[start[kk] + atomicAdd(&cursor[kk], 1)] = pid[p];
 }
-__global__ void k_sort_cells(long nbins, const int32_t* start, int32_t* perm, const int32_t* pid) {
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (c >= nbins) return;
-    int32_t a = start[c], b = start[c + 1];
-    for (int32_t x = a + 1; x < b; ++x) {
-        int32_t v = perm[x];
-        int32_t key = pid[v];
-        int32_t y = x - 1;
-        while (y >= a && pid[perm[y]] > key) { perm[y + 1] = perm[y]; --y; }
-        perm[y + 1] = v;
-    }
-}
-__global__ void k_gather(long n, const int32_t* perm, const double* px, const double* py, const double* pz,
-                         const double* vx, const double* vy, const double* vz, const int32_t* pid, double* tx,
-                         double* ty, double* tz, double* tvx, double* tvy, double* tvz, int32_t* tpid) {
-    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int32_t p = perm[s];
+// Particle p goes to slot start + (the number of its cell's particles with a
+// smaller original index): ascending original index within every cell, as a
+// stable sort gives, with no per-cell serial pass (a cell holding m particles
+// costs each of them m reads of one contiguous run of ids).
+__global__ void k_rank_gather(long n, const int32_t* __restrict__ key, const int32_t* __restrict__ start,
+                              const int32_t* __restrict__ ids, const double* px, const double* py, const double* pz,
+                              const double* vx, const double* vy, const double* vz, const int32_t* pid, double* tx,
+                              double* ty, double* tz, double* tvx, double* tvy, double* tvz, int32_t* tpid) {
+    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t kk = key[p], a = start[kk], b = start[kk + 1], me = pid[p];
+    int32_t r = 0;
+    for (int32_t x = a; x < b; ++x) r += ids[x] < me;
+    const int32_t s = a + r;
     tx[s] = px[p]; ty[s] = py[p]; tz[s] = pz[p];
     tvx[s] = vx[p]; tvy[s] = vy[p]; tvz[s] = vz[p];
-    tpid[s] = pid[p];
+    tpid[s] = me;
 }
 template <class T>
 __global__ void k_zero(long n, T* a) {
@@ -142,10 +140,9 @@ void sort_particles(const Geo& g, Particles& P, CellIndex& ci, hipStream_t st) {
     if (P.n > 0) k_keys<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(g, P.n, P.px, P.py, P.pz, P.key, ci.count);
     exclusive_scan_i32(ci.count, ci.start, nb, ci.scan_tmp, ci.scan_tmp_n, st);
     if (P.n == 0) return;
-    k_scatter<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.key, ci.start, ci.cursor, P.perm);
-    k_sort_cells<<<xf_blocks(nb), XF_BLOCK, 0, st>>>(nb, ci.start, P.perm, P.pid);
-    k_gather<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.perm, P.px, P.py, P.pz, P.vx, P.vy, P.vz, P.pid, P.tx,
-                                                   P.ty, P.tz, P.tvx, P.tvy, P.tvz, P.tpid);
+    k_scatter_ids<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.key, P.pid, ci.start, ci.cursor, P.perm);
+    k_rank_gather<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.key, ci.start, P.perm, P.px, P.py, P.pz, P.vx, P.vy,
+                                                       P.vz, P.pid, P.tx, P.ty, P.tz, P.tvx, P.tvy, P.tvz, P.tpid);
     std::swap(P.px, P.tx); std::swap(P.py, P.ty); std::swap(P.pz, P.tz);
     std::swap(P.vx, P.tvx); std::swap(P.vy, P.tvy); std::swap(P.vz, P.tvz);
     std::swap(P.pid, P.tpid);
