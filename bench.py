@@ -158,6 +158,65 @@ def run_projection(xf, n, reps):
                       "solves": out}), flush=True)
 
 
+SOURCE_DEMO = {
+    # the reference's demo_source (main.cpp:5-26): 64^3, dx 0.125, gravity -20,
+    # a cubic source at (4, 30, 30) dx of half-widths (2, 5, 10) dx emitting at
+    # (10, 0, 0), no initial fluid, 1/30 s frames (CFL 1, FLIP 0.95, PCG 1e-5)
+    "grid": (64, 64, 64), "dx": 0.125, "force": (0.0, -20.0, 0.0),
+    "center": (4.0, 30.0, 30.0), "half": (2.0, 5.0, 10.0), "velocity": (10.0, 0.0, 0.0), "frame": 1.0 / 30,
+}
+
+
+def run_source_demo(xf, frames, cpu_frames):
+    """The reference's fluid-source demo: frames/s of frameStep(1/30) on the
+    GPU (mesh built on each frame's last substep, as the reference does; no
+    file output), and the oracle's time for its first `cpu_frames` frames."""
+    d = SOURCE_DEMO
+    dx = d["dx"]
+    c = [x * dx for x in d["center"]]
+    w = [x * dx for x in d["half"]]
+    s = xf.FluidSimulator(*d["grid"], xf.default_params())
+    s.setCellWidth(dx)
+    s.addConstantForce(*d["force"])
+    s.addFluidSource(c, *w, d["velocity"])
+    s.init()
+    s.synchronize()
+    per_frame = []
+    for f in range(frames):
+        t0 = time.perf_counter()
+        rc = s.frameStep(d["frame"])
+        s.synchronize()
+        per_frame.append(time.perf_counter() - t0)
+        if rc < 0:
+            raise SystemExit(f"bench.py: source demo frame {f}: rc {rc}")
+    n_end = s.num_particles()
+    s.close()
+    total = sum(per_frame)
+    cpu = None
+    if cpu_frames:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import OracleSim, default_params
+        o = OracleSim(*d["grid"], dx, default_params())
+        o.lib.xo_add_constant_force(o.h, *d["force"])
+        o.add_fluid_source(c, *w, d["velocity"])
+        o.lib.xo_set_solid_boundary(o.h)
+        t0 = time.perf_counter()
+        for _ in range(cpu_frames):
+            o.frame_step(d["frame"])
+        cpu_s = time.perf_counter() - t0
+        gpu_s = sum(per_frame[:cpu_frames])
+        cpu = {"frames": cpu_frames, "seconds": round(cpu_s, 3), "gpu_seconds_same_frames": round(gpu_s, 4),
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port"}
+    print(json.dumps({"metric": "frames/s (64^3 fluid-source demo, frameStep(1/30) with mesh)",
+                      "value": round(frames / total, 3), "unit": "frames/s", "higher_is_better": True,
+                      "n_gpus": 1, "data": "synthetic", "dtype": "f64",
+                      "config": {"workload": "demo_source: 64^3, dx 0.125, source (4,30,30) dx +- (2,5,10) dx "
+                                             "at (10,0,0), gravity -20, no initial fluid",
+                                 "frames": frames},
+                      "particles_end": n_end, "ms_first_frame": round(per_frame[0] * 1e3, 2),
+                      "ms_last_frame": round(per_frame[-1] * 1e3, 2), "cpu_baseline": cpu}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +228,8 @@ def main():
     ap.add_argument("--sweep", type=int, default=0)
     ap.add_argument("--projection", type=int, default=0, metavar="N",
                     help="only run the N^3 sphere-scene projection benchmark (N = 256 for the full size) and exit")
+    ap.add_argument("--source-demo", type=int, default=0, metavar="FRAMES",
+                    help="only run the reference's fluid-source demo for FRAMES frames and exit")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -184,6 +245,9 @@ def main():
         raise SystemExit("bench.py: no HIP device")
     if args.projection:
         run_projection(xf, args.projection, 2)
+        return
+    if args.source_demo:
+        run_source_demo(xf, args.source_demo, 0 if args.no_cpu else 2)
         return
     dt = 1.0 / 240
     ni, nj, nk = CONFIGS[args.config][:3]
