@@ -32,10 +32,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_sim(xf, cfg, precision, sweep_kernel=0):
+def make_sim(xf, cfg, precision, sweep_kernel=0, pressure=0):
     ni, nj, nk, dx, frac, seed, maxit = CONFIGS[cfg]
     p = xf.default_params(flip_ratio=0.95, pcg_tol=1e-6, pcg_max_iter=maxit, precision=precision,
-                          sweep_kernel=sweep_kernel, mixed_inner_tol=1e-4)
+                          sweep_kernel=sweep_kernel, mixed_inner_tol=1e-4, pressure_solver=pressure)
     s = xf.FluidSimulator(ni, nj, nk, p)
     s.setCellWidth(dx)
     s.setSolidBoundary()
@@ -46,8 +46,8 @@ def make_sim(xf, cfg, precision, sweep_kernel=0):
     return s
 
 
-def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0):
-    s = make_sim(xf, cfg, precision, sweep_kernel)
+def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0, pressure=0):
+    s = make_sim(xf, cfg, precision, sweep_kernel, pressure)
     for _ in range(warmup):
         s.timeStep(dt)
     s.synchronize()
@@ -226,6 +226,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--sweep", type=int, default=0)
+    ap.add_argument("--pressure", default="variational", choices=["variational", "normal"],
+                    help="pressure operator: the reference's VariationalPressureSolver (default) or "
+                         "NormalPressureSolver")
     ap.add_argument("--projection", type=int, default=0, metavar="N",
                     help="only run the N^3 sphere-scene projection benchmark (N = 256 for the full size) and exit")
     ap.add_argument("--source-demo", type=int, default=0, metavar="FRAMES",
@@ -253,12 +256,13 @@ def main():
     ni, nj, nk = CONFIGS[args.config][:3]
     N = ni * nj * nk
     log(f"[bench] mixed precision, {args.config}, warmup {args.warmup}, steps {args.steps}")
-    mixed = run_gpu(xf, args.config, xf.PREC_MIXED, args.steps, args.warmup, dt, args.sweep)
+    op = xf.PRESSURE_NORMAL if args.pressure == "normal" else xf.PRESSURE_VARIATIONAL
+    mixed = run_gpu(xf, args.config, xf.PREC_MIXED, args.steps, args.warmup, dt, args.sweep, op)
     log(f"[bench] mixed: {mixed['elapsed']:.3f} s, solve {mixed['solve_ms']:.1f} ms, iters {mixed['iters']}")
     fp64 = None
     if not args.no_fp64:
         log("[bench] fp64 PCG")
-        fp64 = run_gpu(xf, args.config, xf.PREC_FP64, args.steps, args.warmup, dt, args.sweep)
+        fp64 = run_gpu(xf, args.config, xf.PREC_FP64, args.steps, args.warmup, dt, args.sweep, op)
         log(f"[bench] fp64: {fp64['elapsed']:.3f} s, solve {fp64['solve_ms']:.1f} ms, iters {fp64['iters']}")
     K = args.steps
     value = K / mixed["elapsed"]
@@ -290,7 +294,7 @@ def main():
         "dtype": "f64 (PCG bulk f32 + f64 dots/refinement)",
         "data": "synthetic",
         "config": {"workload": f"{args.config} dam break, {mixed['particles']} particles, dt=1/240, FLIP 0.95",
-                   "grid": [ni, nj, nk], "particles": mixed["particles"]},
+                   "grid": [ni, nj, nk], "particles": mixed["particles"], "pressure_solver": args.pressure},
         "pcg_ms_per_solve_mixed": round(mixed["solve_ms"] / K, 3),
         "pcg_iters_per_solve_mixed": round(mixed["iters"] / K, 2),
         "pcg_outer_per_solve_mixed": round(mixed["outer"] / K, 2),

@@ -61,8 +61,20 @@ typedef struct xf_params {
     double mixed_inner_tol;   /* mixed mode: relative reduction asked of an FP32 inner solve; later
                                  inner solves stop at half of what is left to the outer tolerance */
     int    mixed_max_outer;   /* mixed mode: cap on FP64 refinement steps           */
-    int    reserved[5];
+    int    pressure_solver;   /* XF_PRESSURE_VARIATIONAL (the reference's, FluidSimulator.h:215) or
+                                 XF_PRESSURE_NORMAL (NormalPressureSolver, FluidSimulator.h:216) */
+    int    reserved[4];
 } xf_params;
+
+/* Pressure operators of the projection seam (FluidSimulator.h:214-216). */
+enum {
+    XF_PRESSURE_VARIATIONAL = 0, /* VariationalPressureSolver: fluid + fluid-adjacent solid rows,
+                                    weights fluid 1, air 2, solid 0.5 (variational_pressuresolver.cpp:231-326) */
+    XF_PRESSURE_NORMAL      = 1  /* NormalPressureSolver: fluid rows, diagonal = non-solid neighbours,
+                                    -1 per fluid neighbour; solid faces dropped from the divergence;
+                                    apply with air ghost 0 and solid faces set to zero normal
+                                    velocity (pressuresolver.cpp:109-212, 220-347) */
+};
 
 /* Particle seeding modes of xf_sim_seed_particles. */
 enum {

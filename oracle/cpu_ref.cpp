@@ -103,6 +103,55 @@ void assemble(int ni, int nj, int nk, const uint8_t* mat, const double* u, const
             }
 }
 
+// NormalPressureSolver::_calculate_coefficient (pressuresolver.cpp:220-347):
+// rows for fluid cells only (i slowest), neighbour order +i,-i,+j,-j,+k,-k
+// (xfluidutil.cpp:33-41); an in-range non-solid neighbour adds 1 to the
+// diagonal, a fluid one also -1 off the diagonal; then b, per fluid cell,
+// +u(i) -u(i+1) +v(j) -v(j+1) +w(k) -w(k+1) for the sides whose cell is in
+// range and not solid.
+void assemble_normal(int ni, int nj, int nk, const uint8_t* mat, const double* u, const double* v,
+                     const double* w, double* A, double* b) {
+    const long N = (long)ni * nj * nk, nn = (long)ni * nj;
+    std::fill(A, A + 7 * N, 0.0);
+    std::fill(b, b + N, 0.0);
+    auto in = [&](int i, int j, int k) { return i >= 0 && j >= 0 && k >= 0 && i < ni && j < nj && k < nk; };
+    auto Mt = [&](int i, int j, int k) -> uint8_t { return in(i, j, k) ? mat[i + (long)ni * j + nn * k] : SOLID; };
+    auto U = [&](int i, int j, int k) { return u[i + (long)(ni + 1) * (j + (long)nj * k)]; };
+    auto V = [&](int i, int j, int k) { return v[i + (long)ni * (j + (long)(nj + 1) * k)]; };
+    auto W = [&](int i, int j, int k) { return w[i + (long)ni * (j + (long)nj * k)]; };
+    const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+    const int slot_cm[6] = {A_XP, A_XM, A_YP, A_YM, A_ZP, A_ZM};
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j)
+            for (int k = 0; k < nk; ++k) {
+                long c = i + (long)ni * j + nn * k;
+                if (mat[c] != FLUID) continue;
+                int count = 0;
+                for (int e = 0; e < 6; ++e) {
+                    int gi = i + di[e], gj = j + dj[e], gk = k + dk[e];
+                    if (!in(gi, gj, gk)) continue;
+                    uint8_t mm = Mt(gi, gj, gk);
+                    if (mm != SOLID) {
+                        ++count;
+                        if (mm == FLUID) A[slot_cm[e] * N + c] += -1.0;
+                    }
+                }
+                A[A_D * N + c] += (double)count * 1.0;
+            }
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j)
+            for (int k = 0; k < nk; ++k) {
+                long c = i + (long)ni * j + nn * k;
+                if (mat[c] != FLUID) continue;
+                if (Mt(i - 1, j, k) != SOLID) b[c] += U(i, j, k);
+                if (Mt(i + 1, j, k) != SOLID) b[c] -= U(i + 1, j, k);
+                if (Mt(i, j - 1, k) != SOLID) b[c] += V(i, j, k);
+                if (Mt(i, j + 1, k) != SOLID) b[c] -= V(i, j + 1, k);
+                if (Mt(i, j, k - 1) != SOLID) b[c] += W(i, j, k);
+                if (Mt(i, j, k + 1) != SOLID) b[c] -= W(i, j, k + 1);
+            }
+}
+
 // sparse_matrix.h:266-276 (multiply): present entries in column order.
 void spmv(int ni, int nj, int nk, const double* A, const double* x, double* y) {
     const long N = (long)ni * nj * nk, nn = (long)ni * nj;
@@ -262,6 +311,42 @@ void apply_pressure(int ni, int nj, int nk, const uint8_t* mat, const double* p,
                     }
                     long fi = i + (long)si * (j + (long)sj * k);
                     f[fi] = f[fi] - 1.0 * (p2 - p1);
+                }
+    }
+}
+
+// NormalPressureSolver::applyPressure (pressuresolver.cpp:109-195), factor
+// 1.0: outside cells count as solid; an air side has pressure 0, a solid side
+// the fluid pressure plus (minus) the face velocity.
+void apply_pressure_normal(int ni, int nj, int nk, const uint8_t* mat, const double* p, double* u, double* v,
+                           double* w) {
+    const long nn = (long)ni * nj;
+    auto M = [&](int i, int j, int k) -> uint8_t {
+        if (i < 0 || j < 0 || k < 0 || i >= ni || j >= nj || k >= nk) return SOLID;
+        return mat[i + (long)ni * j + nn * k];
+    };
+    auto P = [&](int i, int j, int k) { return p[i + (long)ni * j + nn * k]; };
+    const double factor = 1.0;
+    for (int axis = 0; axis < 3; ++axis) {
+        int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
+        int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
+        double* f = axis == 0 ? u : axis == 1 ? v : w;
+        for (int i = 0; i < si; ++i)
+            for (int j = 0; j < sj; ++j)
+                for (int k = 0; k < sk; ++k) {
+                    uint8_t ml = M(i + di, j + dj, k + dk), mr = M(i, j, k);
+                    if (ml != FLUID && mr != FLUID) continue;
+                    long fi = i + (long)si * (j + (long)sj * k);
+                    double p1 = 0.0, p2 = 0.0;
+                    if (ml == FLUID && mr == FLUID) { p2 = P(i, j, k); p1 = P(i + di, j + dj, k + dk); }
+                    else if (ml == FLUID) {
+                        p1 = P(i + di, j + dj, k + dk);
+                        p2 = (mr == SOLID) ? p1 + f[fi] / factor : 0.0;
+                    } else {
+                        p2 = P(i, j, k);
+                        p1 = (ml == SOLID) ? p2 - f[fi] / factor : 0.0;
+                    }
+                    f[fi] = f[fi] - factor * (p2 - p1);
                 }
     }
 }
@@ -645,7 +730,10 @@ struct Sim {
     int solve() {
         const long n = N();
         std::vector<double> A(7 * n);
-        assemble(ni, nj, nk, mat.data(), u.data(), v.data(), w.data(), A.data(), rhs.data());
+        if (prm.pressure_solver == XF_PRESSURE_NORMAL)
+            assemble_normal(ni, nj, nk, mat.data(), u.data(), v.data(), w.data(), A.data(), rhs.data());
+        else
+            assemble(ni, nj, nk, mat.data(), u.data(), v.data(), w.data(), A.data(), rhs.data());
         int it = 0;
         double res = 0;
         int st = pcg(ni, nj, nk, A.data(), rhs.data(), pressure.data(), prm.pcg_tol, prm.pcg_max_iter, prm.mic_tau,
@@ -654,7 +742,12 @@ struct Sim {
         last.rhs_norm = absmax(rhs.data(), n);
         return st;
     }
-    void apply() { apply_pressure(ni, nj, nk, mat.data(), pressure.data(), u.data(), v.data(), w.data()); }
+    void apply() {
+        if (prm.pressure_solver == XF_PRESSURE_NORMAL)
+            apply_pressure_normal(ni, nj, nk, mat.data(), pressure.data(), u.data(), v.data(), w.data());
+        else
+            apply_pressure(ni, nj, nk, mat.data(), pressure.data(), u.data(), v.data(), w.data());
+    }
 
     // ---- a15: FluidSimulator.cpp:511-650 ----
     void extrapolate_axis(int layers, int axis) {
@@ -1257,6 +1350,14 @@ int xo_pcg(int ni, int nj, int nk, const double* A7, const double* b, double* x,
 }
 void xo_apply_pressure(int ni, int nj, int nk, const uint8_t* mat, const double* p, double* u, double* v, double* w) {
     apply_pressure(ni, nj, nk, mat, p, u, v, w);
+}
+void xo_assemble_normal(int ni, int nj, int nk, const uint8_t* mat, const double* u, const double* v,
+                        const double* w, double* A7, double* b) {
+    assemble_normal(ni, nj, nk, mat, u, v, w, A7, b);
+}
+void xo_apply_pressure_normal(int ni, int nj, int nk, const uint8_t* mat, const double* p, double* u, double* v,
+                              double* w) {
+    apply_pressure_normal(ni, nj, nk, mat, p, u, v, w);
 }
 
 // -- simulator handle --

@@ -341,4 +341,41 @@ int xr_pressure_pieces(void* h, double* A7, double* b_out, double* inv_out, cons
     return 0;
 }
 
+// NormalPressureSolver (pressuresolver.cpp:50-347) on the simulator's material
+// and velocities: its matrix in 7 slots, its right-hand side, the pressure of
+// its solve (default PCGSolver parameters, pcg_solver.h:228-231) and, when the
+// solve succeeds, the velocities after its applyPressure (left in the
+// simulator's u, v, w).  Returns 0 on success, 1 when the solve fails.
+int xr_normal_pieces(void* h, double* A7, double* b_out, double* p_out) {
+    FluidSimulator& s = ((Ref*)h)->sim;
+    NormalPressureSolver ns;
+    ns.setSize(s._ni, s._nj, s._nk);
+    ns.setVelocityField(&s._u, &s._v, &s._w);
+    ns.setMaterial(&s._material);
+    ns.setDx(s._dx);
+    SparseMatrix<double> A;
+    std::vector<double> b;
+    ns._calculate_coefficient(A, b);
+    const long N = (long)s._ni * s._nj * s._nk, nn = (long)s._ni * s._nj;
+    const long off[7] = {-nn, -(long)s._ni, -1, 0, 1, (long)s._ni, nn};
+    std::fill(A7, A7 + 7 * N, 0.0);
+    for (long c = 0; c < N; ++c)
+        for (size_t e = 0; e < A.index[c].size(); ++e) {
+            long col = A.index[c][e];
+            int slot = -1;
+            for (int t = 0; t < 7; ++t) if (c + off[t] == col) slot = t;
+            if (slot < 0) return -2;
+            A7[slot * N + c] = A.value[c][e];
+        }
+    std::copy(b.begin(), b.end(), b_out);
+    Quiet q;
+    Array3d<double> pressure;
+    if (!ns.solve(pressure)) return 1;
+    for (int k = 0; k < s._nk; ++k)
+        for (int j = 0; j < s._nj; ++j)
+            for (int i = 0; i < s._ni; ++i) p_out[i + s._ni * (j + (long)s._nj * k)] = pressure(i, j, k);
+    ns.applyPressure(pressure);
+    return 0;
+}
+
 }  // extern "C"

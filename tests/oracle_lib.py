@@ -33,7 +33,8 @@ class XfParams(C.Structure):
         ("blob_radius", C.c_double), ("iso", C.c_double), ("precision", C.c_int), ("integrator", C.c_int),
         ("p2g_kernel", C.c_int), ("tail_skip", C.c_int), ("density_every_substep", C.c_int),
         ("sweep_kernel", C.c_int), ("mixed_inner_tol", C.c_double), ("mixed_max_outer", C.c_int),
-        ("reserved", C.c_int * 5),
+        ("pressure_solver", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 
@@ -86,6 +87,8 @@ def _load_oracle():
     lib.xo_pcg.argtypes = g + [_dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double,
                                C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.xo_apply_pressure.argtypes = g + [_u8, _dp, _dp, _dp, _dp]
+    lib.xo_assemble_normal.argtypes = g + [_u8, _dp, _dp, _dp, _dp, _dp]
+    lib.xo_apply_pressure_normal.argtypes = g + [_u8, _dp, _dp, _dp, _dp]
     P = C.c_void_p
     lib.xo_sample_points.argtypes = [P, C.c_int64, _dp, _dp, C.c_int]
     lib.xo_sample_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
@@ -127,6 +130,7 @@ def _load_ref():
     lib.xr_surface_cells.restype = C.c_int64
     lib.xr_surface_cells.argtypes = [C.c_void_p, _i32, C.c_int64]
     lib.xr_pressure_pieces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    lib.xr_normal_pieces.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.xr_mesh.restype = C.c_int64
     lib.xr_mesh.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
     lib.xr_get_mesh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]

@@ -18,6 +18,7 @@ FIELD = dict(U=0, V=1, W=2, USAVED=3, VSAVED=4, WSAVED=5, MATERIAL=6, PHI=7, PRE
 STAGE = dict(DENSITY=0, MATERIAL=1, SAVE=2, FORCES=3, SOLVE=4, APPLY=5, EXTRAPOLATE=6, ADVECT=7, P2G=8)
 SEED_REFERENCE, SEED_SUBCELL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+PRESSURE_VARIATIONAL, PRESSURE_NORMAL = 0, 1
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
@@ -32,7 +33,8 @@ class XfParams(C.Structure):
         ("blob_radius", C.c_double), ("iso", C.c_double), ("precision", C.c_int), ("integrator", C.c_int),
         ("p2g_kernel", C.c_int), ("tail_skip", C.c_int), ("density_every_substep", C.c_int),
         ("sweep_kernel", C.c_int), ("mixed_inner_tol", C.c_double), ("mixed_max_outer", C.c_int),
-        ("reserved", C.c_int * 5),
+        ("pressure_solver", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 

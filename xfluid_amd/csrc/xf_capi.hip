@@ -10,6 +10,8 @@
 #include <algorithm>
 #include "../../include/xfluid.h"
 #include "xf_internal.h"
+static_assert(OP_VARIATIONAL == XF_PRESSURE_VARIATIONAL && OP_NORMAL == XF_PRESSURE_NORMAL,
+              "pressure operator ids");
 
 namespace {
 thread_local std::string g_err;
@@ -261,7 +263,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s->st));
-    launch_stencil_rhs(s->g, s->mat, fu, fv, fw, s->code, s->pw.b, s->st);
+    launch_stencil_rhs(s->g, s->mat, fu, fv, fw, s->code, s->pw.b, s->st, s->prm.pressure_solver);
     if (rhs_override)
         HIPCHK(hipMemcpyAsync(s->pw.b, rhs_override, s->g.N * 8, hipMemcpyDeviceToDevice, s->st));
     s->last.reserved = 0;
@@ -318,7 +320,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
 int st_solve(xf_sim* s) { return st_solve_from(s, s->u, s->v, s->w, nullptr); }
 int st_apply(xf_sim* s) {
     double* fa[3] = {s->u, s->v, s->w};
-    for (int a = 0; a < 3; ++a) launch_apply_pressure(s->g, s->mat, s->pw.x, fa[a], a, s->st);
+    for (int a = 0; a < 3; ++a) launch_apply_pressure(s->g, s->mat, s->pw.x, fa[a], a, s->st, s->prm.pressure_solver);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -534,6 +536,8 @@ int xf_sim_set_params(xf_sim* s, const xf_params* p) {
     if (check(s) || !p) return set_err(-1, "null argument");
     if (p->sub_division < 1 || p->sub_division > 4) return set_err(-1, "sub_division must be 1..4");
     if (p->pcg_max_iter < 0) return set_err(-1, "pcg_max_iter < 0");
+    if (p->pressure_solver != XF_PRESSURE_VARIATIONAL && p->pressure_solver != XF_PRESSURE_NORMAL)
+        return set_err(-1, "unknown pressure_solver %d", p->pressure_solver);
     s->prm = *p;
     return ensure_phi(s);
 }
@@ -1019,6 +1023,8 @@ int xf_sim_project_fields(xf_sim* s, double* u, double* v, double* w, int device
 }
 int xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs) {
     if (check(s) || !gu || !gv || !gw) return set_err(-1, "null argument");
+    if (s->prm.pressure_solver != XF_PRESSURE_VARIATIONAL)
+        return set_err(-1, "the projection adjoint is built for the variational pressure operator");
     if (!s->ga_u) {
         if (dalloc(&s->ga_u, s->nu()) != hipSuccess || dalloc(&s->ga_v, s->nv()) != hipSuccess ||
             dalloc(&s->ga_w, s->nw()) != hipSuccess || dalloc(&s->ga_p, s->g.N) != hipSuccess)
@@ -1189,9 +1195,9 @@ int xf_project(int ni, int nj, int nk, const uint8_t* material, double* u, doubl
     if (p) prm = *p; else xf_params_default(&prm);
     rc = xf_sim_set_params(s, &prm);
     const long N = (long)ni * nj * nk;
-    // the reference's assembly has no range checks (variational_pressuresolver.cpp:252-292):
-    // fluid must not touch the grid boundary
-    for (int k = 0; k < nk && rc == 0; ++k)
+    // the reference's variational assembly has no range checks
+    // (variational_pressuresolver.cpp:252-292): fluid must not touch the grid boundary
+    for (int k = 0; k < nk && rc == 0 && prm.pressure_solver == XF_PRESSURE_VARIATIONAL; ++k)
         for (int j = 0; j < nj && rc == 0; ++j)
             for (int i = 0; i < ni; ++i)
                 if ((i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1) &&

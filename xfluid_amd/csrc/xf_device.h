@@ -9,6 +9,9 @@
 
 #define XF_AIR 0
 #define XF_FLUID 1
+// pressure operators (the values of XF_PRESSURE_VARIATIONAL / XF_PRESSURE_NORMAL)
+#define OP_VARIATIONAL 0
+#define OP_NORMAL 1
 #define XF_SOLID 2
 
 // Grid geometry passed by value to every kernel.
@@ -70,12 +73,18 @@ __host__ __device__ inline double code_coef(int cls) { return cls == 2 ? -1.0 : 
 // All terms are exact halves, so the order of the sum does not matter.
 // Counted per class with bit operations on the six 2-bit fields (class 2 in
 // n2 fields, class 1 in n1; classes 0 and 3 count 2.0 on a fluid row).
+// A fluid row of the NormalPressureSolver operator carries CODE_NORMAL_ROW in
+// the material field: its diagonal counts the in-range non-solid neighbours,
+// fluid (class 2, off-diagonal -1) and air (class 0); solid and outside
+// neighbours are class 3 (pressuresolver.cpp:241-272).
+#define CODE_NORMAL_ROW 3
 __host__ __device__ inline double code_diag(uint16_t c) {
     const int m = code_mat(c);
     const unsigned lo = c & 0x555u, hi = (c >> 1) & 0x555u;
     const int n2 = __builtin_popcount(hi & ~lo), n1 = __builtin_popcount(lo & ~hi);
     if (m == XF_FLUID) return n2 * 1.0 + n1 * 0.5 + (6 - n2 - n1) * 2.0;
     if (m == XF_SOLID) return n1 * 0.5;
+    if (m == CODE_NORMAL_ROW) return (double)(n2 + __builtin_popcount(~(lo | hi) & 0x555u));
     return 0.0;
 }
 

@@ -113,9 +113,49 @@ __global__ void k_stencil_rhs(Geo g, const uint8_t* mat, const double* u, const 
     code[c] = cd;
     b[c] = bb;
 }
+// NormalPressureSolver::_calculate_coefficient (pressuresolver.cpp:220-347):
+// rows for fluid cells only; diagonal = in-range non-solid neighbours, -1 per
+// fluid neighbour; b = +u(i) - u(i+1) + v(j) - v(j+1) + w(k) - w(k+1), in that
+// order, each term only when the cell on that side is in range and not solid.
+__global__ void k_stencil_rhs_normal(Geo g, const uint8_t* mat, const double* u, const double* v, const double* w,
+                                     uint16_t* code, double* b) {
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    const uint8_t m = mat[c];
+    if (m != XF_FLUID) {  // an empty row
+        code[c] = (uint16_t)((m & 3) << 12);
+        b[c] = 0.0;
+        return;
+    }
+    uint8_t nb[6];
+    nb[D_ZM] = mat_at(g, mat, i, j, k - 1, XF_SOLID);
+    nb[D_YM] = mat_at(g, mat, i, j - 1, k, XF_SOLID);
+    nb[D_XM] = mat_at(g, mat, i - 1, j, k, XF_SOLID);
+    nb[D_XP] = mat_at(g, mat, i + 1, j, k, XF_SOLID);
+    nb[D_YP] = mat_at(g, mat, i, j + 1, k, XF_SOLID);
+    nb[D_ZP] = mat_at(g, mat, i, j, k + 1, XF_SOLID);
+    uint16_t cd = (uint16_t)(CODE_NORMAL_ROW << 12);
+    for (int d = 0; d < 6; ++d) {
+        const int cl = nb[d] == XF_FLUID ? 2 : nb[d] == XF_SOLID ? 3 : 0;
+        cd |= (uint16_t)(cl << (2 * d));
+    }
+    double bb = 0.0;
+    if (nb[D_XM] != XF_SOLID) bb += face_u(g, u, i, j, k);
+    if (nb[D_XP] != XF_SOLID) bb -= face_u(g, u, i + 1, j, k);
+    if (nb[D_YM] != XF_SOLID) bb += face_v(g, v, i, j, k);
+    if (nb[D_YP] != XF_SOLID) bb -= face_v(g, v, i, j + 1, k);
+    if (nb[D_ZM] != XF_SOLID) bb += face_w(g, w, i, j, k);
+    if (nb[D_ZP] != XF_SOLID) bb -= face_w(g, w, i, j, k + 1);
+    code[c] = cd;
+    b[c] = bb;
+}
 void launch_stencil_rhs(const Geo& g, const uint8_t* mat, const double* u, const double* v, const double* w,
-                        uint16_t* code, double* b, hipStream_t st) {
-    k_stencil_rhs<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, mat, u, v, w, code, b);
+                        uint16_t* code, double* b, hipStream_t st, int op) {
+    if (op == OP_NORMAL)
+        k_stencil_rhs_normal<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, mat, u, v, w, code, b);
+    else
+        k_stencil_rhs<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, mat, u, v, w, code, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -143,9 +183,40 @@ __global__ void k_apply_pressure(Geo g, const uint8_t* mat, const double* p, dou
     }
     f[t] = f[t] - 1.0 * (p2 - p1);
 }
-void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st) {
+// NormalPressureSolver::applyPressure (pressuresolver.cpp:109-195), factor 1.0:
+// outside cells count as solid; an air side has pressure 0; a solid side takes
+// the fluid pressure plus (minus) the face velocity, so the update leaves the
+// face at u - ((p + u) - p).
+__global__ void k_apply_pressure_normal(Geo g, const uint8_t* mat, const double* p, double* f, int axis) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nf = (long)si * sj * sk;
+    if (t >= nf) return;
+    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
+    uint8_t ml = mat_at(g, mat, i + di, j + dj, k + dk, XF_SOLID), mr = mat_at(g, mat, i, j, k, XF_SOLID);
+    if (ml != XF_FLUID && mr != XF_FLUID) return;
+    const double fv = f[t];
+    double p1, p2;
+    if (ml == XF_FLUID && mr == XF_FLUID) {
+        p2 = p[cidx(g, i, j, k)];
+        p1 = p[cidx(g, i + di, j + dj, k + dk)];
+    } else if (ml == XF_FLUID) {
+        p1 = p[cidx(g, i + di, j + dj, k + dk)];
+        p2 = (mr == XF_SOLID) ? p1 + fv / 1.0 : 0.0;
+    } else {
+        p2 = p[cidx(g, i, j, k)];
+        p1 = (ml == XF_SOLID) ? p2 - fv / 1.0 : 0.0;
+    }
+    f[t] = fv - 1.0 * (p2 - p1);
+}
+void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st,
+                           int op) {
     long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
-    k_apply_pressure<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, p, f, axis);
+    if (op == OP_NORMAL)
+        k_apply_pressure_normal<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, p, f, axis);
+    else
+        k_apply_pressure<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, p, f, axis);
 }
 
 // ---------------------------------------------------------------------------

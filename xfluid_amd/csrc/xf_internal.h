@@ -19,9 +19,11 @@ struct PcgScalars {
 // ---------------- grid stage kernels (xf_grid.hip) ----------------
 void launch_material(const Geo& g, uint8_t* mat, const int32_t* cell_count, hipStream_t st);
 void launch_forces(const Geo& g, const uint8_t* mat, double* f, int axis, double dv, hipStream_t st);
+// op: OP_VARIATIONAL or OP_NORMAL
 void launch_stencil_rhs(const Geo& g, const uint8_t* mat, const double* u, const double* v, const double* w,
-                        uint16_t* code, double* b, hipStream_t st);
-void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st);
+                        uint16_t* code, double* b, hipStream_t st, int op = OP_VARIATIONAL);
+void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st,
+                           int op = OP_VARIATIONAL);
 void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
                         hipStream_t st);
 
