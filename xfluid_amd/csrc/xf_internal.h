@@ -15,6 +15,18 @@ struct PcgScalars {
     double rho_next;  // rho of the next iteration, moved into rho by the next spmv
     int done, iters, status, it;
 };
+// The residual test of PCG iteration `it` (pcg_solver.h:266-270): max of the n
+// partials of |r|, then S->res, it, done and iters, and S copied to `mirror`
+// when given.  Handed to the preconditioner that follows it, which runs it
+// (inside the fused sweep kernel, or as k_finish before its sweeps) and sets `ran`.
+struct FinRes {
+    const double* partial;
+    int n, it;
+    PcgScalars* S;
+    PcgScalars* mirror;
+    bool ran;
+};
+void launch_finish_res(const FinRes& f, hipStream_t st);
 
 // ---------------- grid stage kernels (xf_grid.hip) ----------------
 void launch_material(const Geo& g, uint8_t* mat, const int32_t* cell_count, hipStream_t st);
@@ -129,7 +141,8 @@ template <class T>
 // XF_BLOCK threads (k_dot's cells and order)
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st,
-                      double* dot_partial = nullptr, int dot_blocks = 0, const BoxRange* dot_range = nullptr);
+                      double* dot_partial = nullptr, int dot_blocks = 0, const BoxRange* dot_range = nullptr,
+                      FinRes* fin = nullptr);
 template <class T>
 bool sweep_precond_pipelined(const Geo& g, int kind, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S,
                              int* flags, unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);

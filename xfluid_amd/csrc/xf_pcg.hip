@@ -208,7 +208,13 @@ template <class T>
 static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
                       int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
                       SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr,
-                      const BoxRange* dot_range = nullptr) {
+                      const BoxRange* dot_range = nullptr, FinRes* fin = nullptr) {
+    // a pending residual test runs first: inside the 8x8-tile schedule's fused
+    // kernel, else as k_finish here
+    if (fin && !(sweep_kernel == 9 && sk)) {
+        launch_finish_res(*fin, st);
+        fin->ran = true;
+    }
     // pipelined schedule (xf_sweep.hip) on the skewed copies prepared by the caller
     if (sweep_kernel >= 2 && sk) {
         hipEvent_t a = nullptr, b = nullptr;
@@ -217,13 +223,17 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
             b = tm->ev[tm->used + 1];
         }
         const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st,
-                                                                 dot_partial, RED_BLOCKS, dot_range)
+                                                                 dot_partial, RED_BLOCKS, dot_range, fin)
                          : sweep_kernel >= 8 ? sweep_precond_wave<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
                                            : sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st);
         if (ran) {
             if (a) tm->used += 2;
             return sweep_kernel == 9 && dot_partial && dot_range;
         }
+    }
+    if (fin && !fin->ran) {
+        launch_finish_res(*fin, st);
+        fin->ran = true;
     }
     const int planes = g.ni + g.nj + g.nk - 2;
     for (int t = 0; t < planes; ++t) {
@@ -357,8 +367,20 @@ __global__ void k_residual(Geo g, const uint16_t* __restrict__ code, const doubl
 
 template <bool MAX>
 __device__ inline double reduce_partials(const double* p, int n, double* sh) {
+    // a thread's first four partials are loaded together (they were just written by
+    // another kernel: each load is a round trip to memory), then added in index order
+    constexpr int U = 4;
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = threadIdx.x + u * blockDim.x;
+        a[u] = i < n ? p[i] : 0.0;
+    }
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = MAX ? (p[i] > v ? p[i] : v) : v + p[i];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (threadIdx.x + u * blockDim.x < n) v = MAX ? (a[u] > v ? a[u] : v) : v + a[u];
+    for (int i = threadIdx.x + U * blockDim.x; i < n; i += blockDim.x) v = MAX ? (p[i] > v ? p[i] : v) : v + p[i];
     return block_reduce<double, MAX>(v, sh);
 }
 
@@ -550,6 +572,10 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
     if (mirror) *mirror = *S;
 }
 
+void launch_finish_res(const FinRes& f, hipStream_t st) {
+    k_finish<<<1, 1024, 0, st>>>(FIN_RES, f.partial, f.n, f.S, 0.0, f.it, f.mirror);
+}
+
 // ---------------------------------------------------------------------------
 // One PCG solve (pcg_solver.h:243-285) of A x = rhs with T-precision vectors.
 // x must be zeroed by the caller; rhs is consumed as the initial residual r.
@@ -597,9 +623,8 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             }
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
                                                             it == 0 ? partial2 : partial, S);
-            k_finish<<<1, 1024, 0, st>>>(FIN_RES, it == 0 ? partial2 : partial, RED_BLOCKS, S, tol_factor, it,
-                                         it == end - 1 ? slot : nullptr);
-            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &bxr))
+            FinRes fin{it == 0 ? partial2 : partial, RED_BLOCKS, it, S, it == end - 1 ? slot : nullptr, false};
+            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &bxr, &fin))
                 k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
         }
         hipEventRecord(chunk_ev[c & 1], st);

@@ -1105,10 +1105,46 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
 template <class T>
 __global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0,
                                                      int nS, W8Planes<T> Pf, W8Planes<T> Pb, const PcgScalars* S,
-                                                     uint32_t tag1, uint32_t tag2, int* err) {
+                                                     uint32_t tag1, uint32_t tag2, int* err, const double* fin_partial,
+                                                     int fin_n, int fin_it, PcgScalars* Sw, PcgScalars* mirror) {
     __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, false>::R * W8Ring<T, false>::SLOT];
     __shared__ int ctr_[2];
-    if (S && S->done) return;
+    if (fin_partial) {
+        // the iteration's residual test (k_finish FIN_RES): every workgroup reduces the
+        // partials in the same order and agrees on `stop`; workgroup 0 records it
+        __shared__ double sh[16];
+        __shared__ int stop;
+        // all of a thread's loads in flight at once (the partials were just written
+        // by another kernel, so each load is a round trip to memory)
+        constexpr int FU = 6;
+        double v[FU];
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            const int q = threadIdx.x + u * 192;
+            v[u] = q < fin_n ? fin_partial[q] : 0.0;
+        }
+        double m = 0.0;
+#pragma unroll
+        for (int u = 0; u < FU; ++u) m = v[u] > m ? v[u] : m;
+        for (int q = threadIdx.x + FU * 192; q < fin_n; q += 192) m = fin_partial[q] > m ? fin_partial[q] : m;
+        m = block_reduce<double, true>(m, sh);
+        if (threadIdx.x == 0) {
+            const bool was = Sw->done != 0;
+            stop = was || m <= Sw->tol;
+            if (blockIdx.x == 0) {
+                if (!was) {
+                    Sw->res = m;
+                    Sw->it = fin_it + 1;
+                    if (m <= Sw->tol) { Sw->done = 1; Sw->iters = fin_it + 1; }
+                }
+                if (mirror) *mirror = *Sw;
+            }
+        }
+        __syncthreads();
+        if (stop) return;
+    } else if (S && S->done) {
+        return;
+    }
     int J, K;
     wv_tile(blockIdx.x, ntjb, ntkb, true, &J, &K);
     w8_sweep<T, true, false>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_, ctr_);
@@ -1221,7 +1257,7 @@ static bool w8_fused_fits(int tiles) {
 template <class T>
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
-                      int dot_blocks, const BoxRange* dot_range) {
+                      int dot_blocks, const BoxRange* dot_range, FinRes* fin) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
@@ -1239,9 +1275,16 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     // t0 / t1 ride on the dispatch packets themselves, so they time the kernels and not
     // the marker packets (and their cache release) that hipEventRecord would add
     if (w8_fused_fits<T>(tiles)) {
+        const double* fp = fin ? fin->partial : nullptr;
         hipExtLaunchKernelGGL(k_sweep_w8_fb<T>, dim3(tiles), dim3(192), 0, st, t0, t1, 0, g, ntk, J0, K0, ntjb,
-                              ntkb, s0, nS, f, b, S, tag1, tag2, flags);
+                              ntkb, s0, nS, f, b, S, tag1, tag2, flags, fp, fin ? fin->n : 0, fin ? fin->it : 0,
+                              fin ? fin->S : (PcgScalars*)nullptr, fin ? fin->mirror : (PcgScalars*)nullptr);
+        if (fin) fin->ran = true;
     } else {
+        if (fin) {
+            launch_finish_res(*fin, st);
+            fin->ran = true;
+        }
         hipExtLaunchKernelGGL(k_sweep_w8<T, true>, dim3(tiles), dim3(192), 0, st, t0, (hipEvent_t) nullptr, 0, g,
                               ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags, 0.0, 0.0);
         hipExtLaunchKernelGGL(k_sweep_w8<T, false>, dim3(tiles), dim3(192), 0, st, (hipEvent_t) nullptr, t1, 0, g,
@@ -1259,7 +1302,7 @@ template void sweep_prepare_w8<double>(const Geo&, const uint16_t*, const double
 template void sweep_prepare_w8<float>(const Geo&, const uint16_t*, const float*, const SweepSkew<float>&, hipStream_t);
 template bool sweep_precond_w8<double>(const Geo&, const SweepSkew<double>&, const double*, double*, const PcgScalars*,
                                        int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int,
-                                       const BoxRange*);
+                                       const BoxRange*, FinRes*);
 template bool sweep_precond_w8<float>(const Geo&, const SweepSkew<float>&, const float*, float*, const PcgScalars*,
                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int,
-                                      const BoxRange*);
+                                      const BoxRange*, FinRes*);
