@@ -92,22 +92,56 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, long n, int32_t* tmp, i
 // ---------------------------------------------------------------------------
 // Counting sort by cell: key = pos2Index cell (xfluidutil.cpp:3-6) or N when
 // outside the grid; within a cell, ascending original index.
+// Runs of equal keys over a wavefront's consecutive lanes: particles stored in
+// cell order by the previous sort mostly keep their cell, so a run's first lane
+// does one atomic for the whole run.  Returns the lane's offset in its run and
+// sets the run's first lane and length (for live lanes; the live lanes of a
+// wavefront are a prefix).  Every lane of the wavefront must call it.
+__device__ inline int key_run(int32_t kk, bool live, int* head, int* len) {
+    const int lane = threadIdx.x & 63;
+    const int32_t prev = __shfl_up(kk, 1, 64);
+    const uint64_t heads = __ballot(live && (lane == 0 || prev != kk));
+    const uint64_t alive = __ballot(live);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    if (!live) {
+        *head = lane;
+        *len = 0;
+        return 0;
+    }
+    const int hd = 63 - __builtin_clzll(heads & upto);
+    const uint64_t stop = (heads | ~alive) & ~upto;
+    const int end = stop ? __builtin_ctzll(stop) : 64;
+    *head = hd;
+    *len = end - hd;
+    return lane - hd;
+}
+
 __global__ void k_keys(Geo g, long n, const double* px, const double* py, const double* pz, int32_t* key,
                        int32_t* cnt) {
-    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int i = pos2idx(px[p], g.dx), j = pos2idx(py[p], g.dx), k = pos2idx(pz[p], g.dx);
-    int32_t kk = cin(g, i, j, k) ? (int32_t)cidx(g, i, j, k) : (int32_t)g.N;
-    key[p] = kk;
-    atomicAdd(&cnt[kk], 1);
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const bool live = p < n;
+    int32_t kk = -1;
+    if (live) {
+        int i = pos2idx(px[p], g.dx), j = pos2idx(py[p], g.dx), k = pos2idx(pz[p], g.dx);
+        kk = cin(g, i, j, k) ? (int32_t)cidx(g, i, j, k) : (int32_t)g.N;
+        key[p] = kk;
+    }
+    int hd, len;
+    const int off = key_run(kk, live, &hd, &len);
+    if (live && off == 0) atomicAdd(&cnt[kk], len);
 }
 // each particle's original index into a free slot of its cell's range
 __global__ void k_scatter_ids(long n, const int32_t* __restrict__ key, const int32_t* __restrict__ pid,
                               const int32_t* __restrict__ start, int32_t* cursor, int32_t* ids) {
-    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int32_t kk = key[p];
-    ids[start[kk] + atomicAdd(&cursor[kk], 1)] = pid[p];
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const bool live = p < n;
+    const int32_t kk = live ? key[p] : -1;
+    int hd, len;
+    const int off = key_run(kk, live, &hd, &len);
+    int base = 0;
+    if (live && off == 0) base = atomicAdd(&cursor[kk], len);
+    base = __shfl(base, hd, 64);
+    if (live) ids[start[kk] + base + off] = pid[p];
 }
 // Particle p goes to slot start + (the number of its cell's particles with a
 // smaller original index): ascending original index within every cell, as a
