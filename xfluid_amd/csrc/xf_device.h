@@ -128,6 +128,15 @@ __device__ inline T block_reduce(T v, T* sh) {
     return v;  // valid in thread 0
 }
 
+// (i, j, k) of flat index t of an si x sj x * array, in 32-bit arithmetic
+// (grids hold fewer than 2^30 cells, so every cell and face index fits)
+__device__ inline void unflat3(long t, int si, int sj, int* i, int* j, int* k) {
+    const unsigned u = (unsigned)t, r = u / (unsigned)si;
+    *i = (int)(u - r * (unsigned)si);
+    *j = (int)(r % (unsigned)sj);
+    *k = (int)(r / (unsigned)sj);
+}
+
 // material of cell (i,j,k), `outside` for cells beyond the grid
 __device__ inline uint8_t mat_at(const Geo& g, const uint8_t* mat, int i, int j, int k, uint8_t outside) {
     return cin(g, i, j, k) ? mat[cidx(g, i, j, k)] : outside;

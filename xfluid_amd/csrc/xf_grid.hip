@@ -27,7 +27,8 @@ __global__ void k_forces(Geo g, const uint8_t* mat, double* f, int axis, double 
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long nf = (long)si * sj * sk;
     if (t >= nf) return;
-    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int i, j, k;
+    unflat3(t, si, sj, &i, &j, &k);
     int oi = axis == 0, oj = axis == 1, ok = axis == 2;
     bool a = mat_at(g, mat, i, j, k, XF_AIR) == XF_FLUID;
     bool b = mat_at(g, mat, i - oi, j - oj, k - ok, XF_AIR) == XF_FLUID;
@@ -62,7 +63,8 @@ __global__ void k_stencil_rhs(Geo g, const uint8_t* mat, const double* u, const 
                               uint16_t* code, double* b) {
     long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (c >= g.N) return;
-    int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    int i, j, k;
+    unflat3(c, g.ni, g.nj, &i, &j, &k);
     uint8_t m = mat[c];
     const uint8_t OUT = 3;
     // neighbour materials in code direction order: -k,-j,-i,+i,+j,+k
@@ -121,7 +123,8 @@ __global__ void k_stencil_rhs_normal(Geo g, const uint8_t* mat, const double* u,
                                      uint16_t* code, double* b) {
     long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (c >= g.N) return;
-    int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    int i, j, k;
+    unflat3(c, g.ni, g.nj, &i, &j, &k);
     const uint8_t m = mat[c];
     if (m != XF_FLUID) {  // an empty row
         code[c] = (uint16_t)((m & 3) << 12);
@@ -166,7 +169,8 @@ __global__ void k_apply_pressure(Geo g, const uint8_t* mat, const double* p, dou
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long nf = (long)si * sj * sk;
     if (t >= nf) return;
-    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int i, j, k;
+    unflat3(t, si, sj, &i, &j, &k);
     int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
     uint8_t ml = mat_at(g, mat, i + di, j + dj, k + dk, XF_SOLID), mr = mat_at(g, mat, i, j, k, XF_SOLID);
     if (ml != XF_FLUID && mr != XF_FLUID) return;
@@ -192,7 +196,8 @@ __global__ void k_apply_pressure_normal(Geo g, const uint8_t* mat, const double*
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long nf = (long)si * sj * sk;
     if (t >= nf) return;
-    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int i, j, k;
+    unflat3(t, si, sj, &i, &j, &k);
     int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
     uint8_t ml = mat_at(g, mat, i + di, j + dj, k + dk, XF_SOLID), mr = mat_at(g, mat, i, j, k, XF_SOLID);
     if (ml != XF_FLUID && mr != XF_FLUID) return;
@@ -232,7 +237,8 @@ __global__ void k_extrap_init(Geo g, const uint8_t* mat, int8_t* mark, int axis)
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long nf = (long)si * sj * sk;
     if (t >= nf) return;
-    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int i, j, k;
+    unflat3(t, si, sj, &i, &j, &k);
     int oi = axis == 0, oj = axis == 1, ok = axis == 2;
     bool a = mat_at(g, mat, i, j, k, XF_AIR) == XF_FLUID;
     bool b = mat_at(g, mat, i - oi, j - oj, k - ok, XF_AIR) == XF_FLUID;
@@ -245,7 +251,8 @@ __global__ void k_extrap_layer(Geo g, double* f, int8_t* mark, int axis, int L) 
     long nf = (long)si * sj * sk;
     if (t >= nf) return;
     if (mark[t] != -1) return;
-    int i = (int)(t % si), j = (int)((t / si) % sj), k = (int)(t / ((long)si * sj));
+    int i, j, k;
+    unflat3(t, si, sj, &i, &j, &k);
     const long sij = (long)si * sj;
     double sum = 0.0;
     int count = 0;

@@ -324,7 +324,8 @@ __global__ void k_source_emit(Geo g, const int32_t* __restrict__ cells, long tot
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= total) return;
     const long c = cells[t / per_cell];
-    const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+    int i, j, k;
+    unflat3(c, g.ni, g.nj, &i, &j, &k);
     const double lo = -0.4999 * g.dx, hi = 0.4999 * g.dx;
     const double scale = 2147483647.0 / (hi - lo);
     const uint64_t d = 3 * (m0 + (uint64_t)t);

@@ -100,8 +100,9 @@ __global__ void k_active_box(Geo g, const uint16_t* __restrict__ code, int* box)
         const uint16_t cc = code[c];
         bool act = code_diag(cc) != 0.0;
         for (int d = 0; d < 6; ++d) act = act || present(code_dir(cc, d));
-        if (act) {
-            const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
+        if (act) {  // (32-bit index arithmetic: N < 2^30)
+            const unsigned cu = (unsigned)c, row = cu / (unsigned)g.ni;
+            const int i = (int)(cu - row * (unsigned)g.ni), j = (int)(row % (unsigned)g.nj), k = (int)(row / (unsigned)g.nj);
             v[0] = v[1] = i; v[2] = v[3] = j; v[4] = v[5] = k;
         }
     }
@@ -118,7 +119,10 @@ __global__ void k_active_box(Geo g, const uint16_t* __restrict__ code, int* box)
         const int d = threadIdx.x;
         int m = red[d][0];
         for (int q = 1; q < XF_BLOCK / 64; ++q) m = (d & 1) ? (red[d][q] > m ? red[d][q] : m) : (red[d][q] < m ? red[d][q] : m);
-        if ((d & 1) ? m >= 0 : m != INT_MAX) {
+        // the bounds only tighten, so a block whose value does not improve on one
+        // already seen skips its atomic (most blocks, once a few have landed)
+        const int cur = __hip_atomic_load(box + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((d & 1) ? (m >= 0 && m > cur) : (m != INT_MAX && m < cur)) {
             if (d & 1) atomicMax(box + d, m);
             else atomicMin(box + d, m);
         }
