@@ -129,3 +129,21 @@ def test_source_matches_reference(walls_first):
         raise AssertionError(f"oracle source particles differ from the reference's (max {diff})")
     assert np.array_equal(Vo, Vr)
     np.testing.assert_array_equal(o_try.get(FIELD["MATERIAL"]), r.get(FIELD["MATERIAL"]))
+
+
+def test_source_golden():
+    """The committed fixture of the reference's update (tests/golden/make_golden.py
+    `sources`): the oracle fed the same raw draws reproduces it bit for bit."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sources16x14x12.npz")
+    g = np.load(path, allow_pickle=False)
+    dims = tuple(int(x) for x in g["dims"])
+    o = OracleSim(*dims, float(g["dx"]), default_params())
+    o.lib.xo_set_solid_boundary(o.h)
+    for c, w, v in zip(g["centers"], g["halves"], g["vels"]):
+        o.add_fluid_source(c, *w, v)
+    draws = np.ascontiguousarray(g["raw"].reshape(-1, 3)[:, tuple(int(x) for x in g["order"])]).ravel()
+    assert o.apply_sources(float(g["dt"]), draws) == 0
+    P, V = o.particles()
+    assert np.array_equal(P, g["P"]) and np.array_equal(V, g["V"])
+    assert np.array_equal(o.get(FIELD["MATERIAL"]), g["material"])

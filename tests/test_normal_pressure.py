@@ -118,3 +118,30 @@ def test_normal_projection_divergence_free():
     A, b = _assemble(o)  # b of the projected field = its divergence
     scale = np.abs(o.get(FIELD["U"])).max()
     assert np.abs(b).max() <= 1e-9 * max(scale, 1.0)
+
+
+def test_normal_golden():
+    """The committed fixture of the reference's NormalPressureSolver
+    (tests/golden/make_golden.py `normal`): matrix, RHS, pressure and applied
+    velocities, reproduced by the oracle bit for bit."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "normal14x11x9.npz")
+    g = np.load(path, allow_pickle=False)
+    O = oracle()
+    ni, nj, nk = (int(x) for x in g["dims"])
+    N = ni * nj * nk
+    mat = np.ascontiguousarray(g["material"]).ravel()
+    A, b = np.empty(7 * N), np.empty(N)
+    O.xo_assemble_normal(ni, nj, nk, mat, np.ascontiguousarray(g["u"]).ravel(), np.ascontiguousarray(g["v"]).ravel(),
+                         np.ascontiguousarray(g["w"]).ravel(), A, b)
+    _same(A, g["A7"], "golden matrix")
+    _same(b, g["b"], "golden rhs")
+    x = np.zeros(N)
+    it, res = C.c_int(), C.c_double()
+    assert O.xo_pcg(ni, nj, nk, A, b, x, 1e-5, 100, 0.97, 0.25, C.byref(it), C.byref(res)) == 0
+    _same(x, g["pressure"], "golden pressure")
+    u, v, w = (np.ascontiguousarray(g[f]).ravel().copy() for f in ("u", "v", "w"))
+    O.xo_apply_pressure_normal(ni, nj, nk, mat, np.ascontiguousarray(g["pressure"]), u, v, w)
+    _same(u, g["u_out"].ravel(), "golden apply u")
+    _same(v, g["v_out"].ravel(), "golden apply v")
+    _same(w, g["w_out"].ravel(), "golden apply w")
