@@ -240,8 +240,9 @@ def main():
         copy_calibration(args.copy_calib)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs the substep on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py: the substep runs on one GPU; multi-GPU runs are not part of this library "
+              "(the path would shard as z-slabs, with the MIC(0) sweeps pipelined across slabs).",
+              file=sys.stderr)
         sys.exit(2)
     import xfluid_amd as xf
     if xf.device_count() < 1:
