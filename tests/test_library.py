@@ -30,3 +30,28 @@ def test_params_default_without_gpu():
     import xfluid_amd as xf
     p = xf.default_params()
     assert p.flip_ratio == 0.95 and p.pcg_max_iter == 100 and p.mic_tau == 0.97 and p.iso == 0.4
+    assert p.pressure_solver == xf.PRESSURE_VARIATIONAL
+
+
+def test_no_device_fails_loudly():
+    """Without a HIP device the library refuses to create a simulator (no CPU fallback)."""
+    import pytest
+    import xfluid_amd as xf
+    if xf.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    lib = ctypes.CDLL(os.path.join(ROOT, "xfluid_amd", "libxfluid_hip.so"))
+    lib.xf_last_error.restype = ctypes.c_char_p
+    h = ctypes.c_void_p()
+    assert lib.xf_sim_create(8, 8, 8, ctypes.byref(h)) == -3
+    assert b"no HIP device" in lib.xf_last_error()
+    with pytest.raises(Exception):
+        xf.FluidSimulator(8, 8, 8)
+
+
+def test_bench_refuses_several_gpus():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
