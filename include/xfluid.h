@@ -122,7 +122,7 @@ typedef struct xf_solve_stats {
     double rhs_norm;          /* inf-norm of the right-hand side                   */
     double ms;                /* wall time of the solve on the device, ms          */
     double sweep_ms;          /* timing on, pipelined sweeps: summed device time of
-                                 the MIC(0) sweep launches of every 8th
+                                 the MIC(0) sweep launches of every 5th
                                  preconditioner application, ms                    */
     int    sweeps;            /* number of sweep launches summed in sweep_ms       */
     int    reserved2;

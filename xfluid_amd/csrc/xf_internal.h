@@ -63,12 +63,13 @@ struct SweepSkew {
     // bounding box of the active rows (i0, i1, j0, j1, k0, k1; host copy), see pcg_active_box
     int box[6];
 };
-// Optional device timing of the sweep launches: pairs of events around the
-// forward + backward launches of every SWEEP_TIMER_STRIDE-th preconditioner
-// application of a solve (application k of the solve, k % stride == 0; each
-// timestamp event drains the stream, so timing every application would slow
-// the solve it measures).
-#define SWEEP_TIMER_STRIDE 8
+// Optional device timing of the sweep launches: events on the forward +
+// backward launches of every SWEEP_TIMER_STRIDE-th preconditioner application
+// of a solve (application k of the solve, k % stride == 0; timing every
+// application would slow the solve it measures).  The stride is prime to the
+// 4-iteration chunks, so the samples cover every position in a chunk alike,
+// the chunk's last launch (which also writes the host status slot) included.
+#define SWEEP_TIMER_STRIDE 5
 struct SweepTimer {
     hipEvent_t* ev;
     int cap, used;
