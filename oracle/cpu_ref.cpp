@@ -493,8 +493,11 @@ struct Sim {
                 M(i, j, k) = FLUID;
         }
     }
-    // FluidSimulator.cpp:168-184, 347-370 with a seeded generator
-    void seed(int mode, uint64_t s) {
+    // FluidSimulator.cpp:168-184, 347-370 with a seeded generator, or (draws
+    // given, reference mode) with the jitter randomDouble(-0.01, 0.01) of the
+    // given 31-bit values, three per particle in x, y, z order
+    void seed(int mode, uint64_t s, const uint32_t* draws = nullptr) {
+        long d = 0;
         px.clear(); py.clear(); pz.clear(); vx.clear(); vy.clear(); vz.clear();
         const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1},
                               {-1, 1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, 1, 1}};
@@ -505,6 +508,11 @@ struct Sim {
             for (int t = 0; t < 8; ++t) {
                 double p[3];
                 for (int a = 0; a < 3; ++a) {
+                    if (draws) {
+                        double pt = sg[t][a] < 0 ? c[a] - q : c[a] + q;
+                        p[a] = pt + random_double(-0.01, 0.01, draws[d++]);
+                        continue;
+                    }
                     double r = u01(s);
                     if (mode == XF_SEED_REFERENCE) {
                         double pt = sg[t][a] < 0 ? c[a] - q : c[a] + q;
@@ -1017,41 +1025,21 @@ struct Sim {
     }
 
     // ---- a18: FluidSimulator.cpp:909-1085 (Wyvill r = 1.5dx, normalised) ----
+    // Sums in particle order over each particle's face box (the reference's
+    // loops), the weight from p2g_visit.
     void p2g_axis(int axis) {
         const int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
         const long nf = (long)si * sj * sk;
         std::vector<double> wt(nf, 0.0), val(nf, 0.0);
         const double err = 0.0000001;
-        const bool tri = prm.p2g_kernel == XF_P2G_TRILINEAR;
-        const double r = tri ? dx : 1.5 * dx;
-        const double r2 = r * r;
-        const double c0 = (4.0 / 9.0) * (1.0 / (r2 * r2 * r2));
-        const double c1 = (17.0 / 9.0) * (1.0 / (r2 * r2));
-        const double c2 = (22.0 / 9.0) * (1.0 / (r2));
         const double* vc = axis == 0 ? vx.data() : axis == 1 ? vy.data() : vz.data();
         for (size_t p = 0; p < px.size(); ++p) {
-            double q[3] = {px[p] + (axis == 0 ? 0.5 * dx : 0.0), py[p] + (axis == 1 ? 0.5 * dx : 0.0),
-                           pz[p] + (axis == 2 ? 0.5 * dx : 0.0)};
-            double vv = vc[p];
-            int g0[3], g1[3];
-            for (int a = 0; a < 3; ++a) { g0[a] = pos2idx(-r + q[a], dx); g1[a] = pos2idx(q[a] + r, dx); }
-            for (int i = g0[0]; i <= g1[0]; ++i)
-                for (int j = g0[1]; j <= g1[1]; ++j)
-                    for (int k = g0[2]; k <= g1[2]; ++k) {
-                        if (i < 0 || j < 0 || k < 0 || i >= si || j >= sj || k >= sk) continue;
-                        double ex = q[0] - idx2pos(i, dx), ey = q[1] - idx2pos(j, dx), ez = q[2] - idx2pos(k, dx);
-                        double cw;
-                        if (tri) {
-                            double ax = std::fabs(ex) / dx, ay = std::fabs(ey) / dx, az = std::fabs(ez) / dx;
-                            cw = (ax < 1.0 && ay < 1.0 && az < 1.0) ? (1.0 - ax) * (1.0 - ay) * (1.0 - az) : 0.0;
-                        } else {
-                            double rs = ex * ex + ey * ey + ez * ez;
-                            cw = rs < r2 ? 1.0 - c0 * rs * rs * rs + c1 * rs * rs - c2 * rs : 0.0;
-                        }
-                        long f = i + (long)si * (j + (long)sj * k);
-                        wt[f] += cw;
-                        val[f] += vv * cw;
-                    }
+            const double pp[3] = {px[p], py[p], pz[p]};
+            const double vv = vc[p];
+            p2g_visit<false>(axis, pp, [&](long f, double cw, const double*) {
+                wt[f] += cw;
+                val[f] += vv * cw;
+            });
         }
         double* out = axis == 0 ? u.data() : axis == 1 ? v.data() : w.data();
         for (long f = 0; f < nf; ++f) out[f] = wt[f] > err ? val[f] / wt[f] : 0.0;
@@ -1068,15 +1056,15 @@ struct Sim {
         std::vector<double> val(nf, 0.0);
         std::fill(wt, wt + nf, 0.0);
         for (long p = 0; p < n; ++p)
-            p2g_visit(axis, pos + 3 * p, [&](long f, double cw, const double*) {
+            p2g_visit<false>(axis, pos + 3 * p, [&](long f, double cw, const double*) {
                 wt[f] += cw;
                 val[f] += vel[3 * p + axis] * cw;
             });
         for (long f = 0; f < nf; ++f) out[f] = wt[f] > 0.0000001 ? val[f] / wt[f] : 0.0;
     }
-    // visits the faces of one axis in a particle's support with the weight and
-    // its gradient with respect to the particle position
-    template <class F>
+    // visits the faces of one axis in a particle's support with the weight
+    // (GRAD: also the weight's gradient with respect to the particle position)
+    template <bool GRAD, class F>
     void p2g_visit(int axis, const double* pp, F&& fn) const {
         const int si = ni + (axis == 0), sj = nj + (axis == 1), sk = nk + (axis == 2);
         const bool tri = prm.p2g_kernel == XF_P2G_TRILINEAR;
@@ -1099,6 +1087,7 @@ struct Sim {
                         double a[3] = {std::fabs(e[0]) / dx, std::fabs(e[1]) / dx, std::fabs(e[2]) / dx};
                         if (a[0] < 1.0 && a[1] < 1.0 && a[2] < 1.0) {
                             cw = (1.0 - a[0]) * (1.0 - a[1]) * (1.0 - a[2]);
+                            if (GRAD)
                             for (int t = 0; t < 3; ++t) {
                                 double sg = e[t] > 0 ? 1.0 : e[t] < 0 ? -1.0 : 0.0;
                                 gw[t] = -sg / dx * (1.0 - a[(t + 1) % 3]) * (1.0 - a[(t + 2) % 3]);
@@ -1108,8 +1097,10 @@ struct Sim {
                         double rs = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
                         if (rs < r2) {
                             cw = 1.0 - c0 * rs * rs * rs + c1 * rs * rs - c2 * rs;
-                            const double dw = -3.0 * c0 * rs * rs + 2.0 * c1 * rs - c2;
-                            for (int t = 0; t < 3; ++t) gw[t] = dw * 2.0 * e[t];
+                            if (GRAD) {
+                                const double dw = -3.0 * c0 * rs * rs + 2.0 * c1 * rs - c2;
+                                for (int t = 0; t < 3; ++t) gw[t] = dw * 2.0 * e[t];
+                            }
                         }
                     }
                     fn(i + (long)si * (j + (long)sj * k), cw, gw);
@@ -1162,7 +1153,7 @@ struct Sim {
             for (long f = 0; f < nf; ++f) a[f] = wt[f] > 0.0000001 ? g[axis][f] / wt[f] : 0.0;
             for (long p = 0; p < n; ++p) {
                 const double vp = vel[3 * p + axis];
-                p2g_visit(axis, pos + 3 * p, [&](long f, double cw, const double* gw) {
+                p2g_visit<true>(axis, pos + 3 * p, [&](long f, double cw, const double* gw) {
                     if (a[f] == 0.0) return;
                     gvel[3 * p + axis] += a[f] * cw;
                     for (int t = 0; t < 3; ++t) gpos[3 * p + t] += a[f] * (vp - uf[f]) * gw[t];
@@ -1381,6 +1372,8 @@ void xo_add_constant_force(void* h, double fx, double fy, double fz) {
     s->forces.push_back(fx); s->forces.push_back(fy); s->forces.push_back(fz);
 }
 void xo_seed_particles(void* h, int mode, uint64_t seed) { ((Sim*)h)->seed(mode, seed); }
+// reference-mode seeding with injected 31-bit draws (3 per particle, x, y, z)
+void xo_seed_particles_draws(void* h, const uint32_t* draws) { ((Sim*)h)->seed(XF_SEED_REFERENCE, 0, draws); }
 int64_t xo_num_particles(void* h) { return (int64_t)((Sim*)h)->px.size(); }
 void xo_set_particles(void* h, int64_t n, const double* px, const double* py, const double* pz, const double* vx,
                       const double* vy, const double* vz) {
@@ -1502,6 +1495,11 @@ void xo_p2g_points(void* h, int64_t n, const double* pos, const double* vel, dou
         std::vector<double> wt((size_t)(s->ni + (axis == 0)) * (s->nj + (axis == 1)) * (s->nk + (axis == 2)));
         s->p2g_points(axis, n, pos, vel, out[axis], wt.data());
     }
+}
+// P2G of one axis for given particles with the weight sums of every face
+// (wt) beside the normalised values (out)
+void xo_p2g_weights(void* h, int axis, int64_t n, const double* pos, const double* vel, double* out, double* wt) {
+    ((Sim*)h)->p2g_points(axis, n, pos, vel, out, wt);
 }
 void xo_density_adjoint(void* h, int64_t n, const double* pos, const double* gphi, double* gpos) {
     ((Sim*)h)->density_adjoint(n, pos, gphi, gpos);

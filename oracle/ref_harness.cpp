@@ -114,6 +114,12 @@ void xr_apply_sources(void* h, double dt, unsigned seed) {
     srand(seed);
     ((Ref*)h)->sim._applyFluidSource(dt);
 }
+// FluidSimulator::initParticles (FluidSimulator.cpp:168-184, 347-370) with
+// rand() restarted at srand(seed)
+void xr_init_particles(void* h, unsigned seed) {
+    srand(seed);
+    ((Ref*)h)->sim.initParticles();
+}
 // the first n values rand() returns after srand(seed), and RAND_MAX
 int xr_rand(unsigned seed, int64_t n, uint32_t* out) {
     srand(seed);

@@ -61,6 +61,7 @@ def _load_oracle():
     lib.xo_set_source_seed.argtypes = [C.c_void_p, C.c_uint64]
     lib.xo_apply_sources.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
     lib.xo_seed_particles.argtypes = [C.c_void_p, C.c_int, C.c_uint64]
+    lib.xo_seed_particles_draws.argtypes = [C.c_void_p, C.c_void_p]
     lib.xo_num_particles.restype = C.c_int64
     lib.xo_num_particles.argtypes = [C.c_void_p]
     lib.xo_set_particles.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6
@@ -96,6 +97,7 @@ def _load_oracle():
     lib.xo_advect_adjoint.argtypes = [P, C.c_int64, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]
     lib.xo_p2g_points.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp]
     lib.xo_p2g_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.xo_p2g_weights.argtypes = [P, C.c_int, C.c_int64, _dp, _dp, _dp, _dp]
     lib.xo_density_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp]
     lib.xo_project.argtypes = g + [_u8, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double]
     lib.xo_project_adjoint.argtypes = g + [_u8, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double]
@@ -116,6 +118,7 @@ def _load_ref():
     lib.xr_add_fluid_source.argtypes = [C.c_void_p, _dp] + [C.c_double] * 3 + [_dp]
     lib.xr_apply_sources.argtypes = [C.c_void_p, C.c_double, C.c_uint]
     lib.xr_rand.argtypes = [C.c_uint, C.c_int64, C.c_void_p]
+    lib.xr_init_particles.argtypes = [C.c_void_p, C.c_uint]
     lib.xr_init.argtypes = [C.c_void_p]
     lib.xr_set_particles.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6
     lib.xr_num_particles.restype = C.c_int64
