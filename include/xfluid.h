@@ -253,6 +253,18 @@ int  xf_sim_advect_adjoint(xf_sim* s, int64_t n, const double* pos, double dt, c
  * (gradient on the face arrays) -> gvel (written) and gpos (added). */
 int  xf_sim_p2g_adjoint(xf_sim* s, int64_t n, const double* pos, const double* vel, const double* gu,
                         const double* gv, const double* gw, double* gvel, double* gpos, int device_ptrs);
+/* Allocation-free variant of xf_sim_p2g_adjoint.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as xf_sim_p2g_adjoint with device_ptrs = 1 (all
+ * arrays are device pointers).  It calls no hipMalloc/hipFree: the weight and
+ * value sums live in a static device array of 2 * XF_P2G_ADJOINT_NOALLOC_MAX_FACES
+ * doubles (34.6 MB) that each GPU the library runs on holds until the process
+ * ends.  Grids whose largest face array exceeds the maximum are rejected (<0).
+ * Every call on one GPU shares that array: calls that overlap on one GPU race. */
+#define XF_P2G_ADJOINT_NOALLOC_MAX_FACES 2162688
+int  xf_sim_p2g_adjoint_noalloc(xf_sim* s, int64_t n, const double* pos, const double* vel, const double* gu,
+                                const double* gv, const double* gw, double* gvel, double* gpos);
 /* VJP of the particle-blob density (DirectedDistanceField::updateFeild,
  * directeddistancefield.cpp:122-169, 300-328) for the given particles: gphi
  * (gradient on the (d*ni+1)(d*nj+1)(d*nk+1) density lattice, d =

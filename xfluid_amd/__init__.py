@@ -119,6 +119,7 @@ def lib():
     L.xf_sim_advect_adjoint.argtypes = [V, C.c_int64, V, C.c_double, V, V, V, V, V, C.c_int]
     L.xf_sim_p2g_adjoint.argtypes = [V, C.c_int64, V, V, V, V, V, V, V, C.c_int]
     L.xf_sim_density_adjoint.argtypes = [V, C.c_int64, V, V, V, C.c_int]
+    L.xf_sim_p2g_adjoint_noalloc.argtypes = [V, C.c_int64, V, V, V, V, V, V, V]
     L.xf_sim_generate_mesh.argtypes = [V, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xf_sim_get_mesh.argtypes = [V, V, V, V]
     L.xf_sim_write_mesh_ply.argtypes = [V, C.c_char_p]
@@ -339,6 +340,14 @@ class FluidSimulator:
         """VJP of P2G; gvel is written, gpos is added to."""
         p, dev = self._ptrs((pos, vel, gu, gv, gw, gvel, gpos), "p2g_adjoint")
         return _chk(self.L.xf_sim_p2g_adjoint(self.h, len(pos), *p, dev), "p2g_adjoint")
+
+    def p2g_adjoint_noalloc(self, pos, vel, gu, gv, gw, gvel, gpos):
+        """VJP of P2G with no device allocation (device tensors only; the sums use
+        the library's per-GPU static scratch); gvel is written, gpos is added to."""
+        p, dev = self._ptrs((pos, vel, gu, gv, gw, gvel, gpos), "p2g_adjoint_noalloc")
+        if not dev:
+            raise ValueError("p2g_adjoint_noalloc: arrays must be device tensors")
+        return _chk(self.L.xf_sim_p2g_adjoint_noalloc(self.h, len(pos), *p), "p2g_adjoint_noalloc")
 
     def density_adjoint(self, pos, gphi, gpos):
         """VJP of the particle-blob density on the sub-vertex lattice; gpos is added to."""

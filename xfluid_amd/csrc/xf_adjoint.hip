@@ -308,15 +308,24 @@ void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const
     k_advect_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, Faces{F[0], F[1], F[2]}, FaceGrads{G[0], G[1], G[2]}, n, pos,
                                                       dt, integrator, gnew, gpos);
 }
-// scratch: two arrays of the largest face count
+// clears the weight and value sums of one face array (a kernel, so a caller
+// may record the launch into a graph)
+__global__ void k_zero2(long nf, double* __restrict__ a, double* __restrict__ b) {
+    const long f = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    a[f] = 0.0;
+    b[f] = 0.0;
+}
+
+// scratch: two arrays of the largest face count (their contents on entry do
+// not matter: each axis clears what it uses)
 void launch_p2g_adjoint(const Geo& g, int kernel, long n, const double* pos, const double* vel,
                         const double* const gf[3], double* gvel, double* gpos, double* s_wt, double* s_val,
                         hipStream_t st) {
     const bool tri = kernel == XF_P2G_TRILINEAR;
     for (int axis = 0; axis < 3; ++axis) {
         const long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
-        (void)hipMemsetAsync(s_wt, 0, nf * 8, st);
-        (void)hipMemsetAsync(s_val, 0, nf * 8, st);
+        k_zero2<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(nf, s_wt, s_val);
         if (n > 0) k_p2g_sums<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, axis, tri, n, pos, vel, s_wt, s_val);
         k_p2g_coef<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(nf, gf[axis], s_wt, s_val);
         if (n > 0) k_p2g_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, axis, tri, n, pos, vel, s_wt, s_val, gvel, gpos);

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "../../include/xfluid.h"
 #include "xf_internal.h"
 static_assert(OP_VARIATIONAL == XF_PRESSURE_VARIATIONAL && OP_NORMAL == XF_PRESSURE_NORMAL,
@@ -49,6 +50,38 @@ inline uint64_t splitmix64(uint64_t& s) {
     return z ^ (z >> 31);
 }
 inline double u01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+}  // namespace
+
+// Scratch of the allocation-free P2G adjoint (xf_sim_p2g_adjoint_noalloc): the
+// weight sums and value sums of one face array, XF_P2G_ADJOINT_NOALLOC_MAX_FACES
+// doubles each.  One copy per GPU, held until the process ends.  Host code never
+// passes this name to a kernel: it uses the per-device address that
+// hipGetSymbolAddress returns (scratch_setup, at simulator creation).
+static __device__ __align__(256) double scratch[2 * (size_t)XF_P2G_ADJOINT_NOALLOC_MAX_FACES];
+
+namespace {
+const int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device dev, which must be current.
+// Looked up once per device; 0 on success.
+int scratch_setup(int dev) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return set_err(-1, "device %d out of range for the scratch table", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_err(-2, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)ptr;
+    return 0;
+}
+double* scratch_ptr(int dev) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
 }  // namespace
 
 struct xf_sim {
@@ -404,6 +437,7 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     xf_params_default(&s->prm);
     hipGetDevice(&s->dev);
     auto fail = [&](int rc) { xf_sim_destroy(s); return rc; };
+    if (scratch_setup(s->dev)) return fail(-2);
     if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(set_err(-2, "stream"));
     const long N = s->g.N;
     bool ok = dalloc(&s->u, s->nu()) == hipSuccess && dalloc(&s->v, s->nv()) == hipSuccess &&
@@ -1146,6 +1180,27 @@ int xf_sim_p2g_adjoint(xf_sim* s, int64_t n, const double* pos, const double* ve
     if (e == hipSuccess) e = hipStreamSynchronize(s->st);
     dfree(wt); dfree(val);
     if (e != hipSuccess) return set_err(-2, "p2g adjoint: %s", hipGetErrorString(e));
+    return 0;
+}
+
+// The same VJP with no allocation: device pointers only, the weight and value
+// sums in the per-GPU static scratch (cleared per face array by a kernel).
+int xf_sim_p2g_adjoint_noalloc(xf_sim* s, int64_t n, const double* pos, const double* vel, const double* gu,
+                               const double* gv, const double* gw, double* gvel, double* gpos) {
+    if (check(s) || n < 0 || (n > 0 && (!pos || !vel || !gvel || !gpos)) || !gu || !gv || !gw)
+        return set_err(-1, "null argument");
+    const long mf = s->maxface();
+    if (mf > XF_P2G_ADJOINT_NOALLOC_MAX_FACES)
+        return set_err(-1, "p2g adjoint (no allocation): the grid has %ld faces per axis, the static scratch holds %d",
+                       mf, XF_P2G_ADJOINT_NOALLOC_MAX_FACES);
+    double* base = scratch_ptr(s->dev);
+    if (!base) return set_err(-2, "p2g adjoint (no allocation): no scratch address for device %d", s->dev);
+    const double* gf[3] = {gu, gv, gw};
+    launch_p2g_adjoint(s->g, s->prm.p2g_kernel, n, pos, vel, gf, gvel, gpos, base,
+                       base + XF_P2G_ADJOINT_NOALLOC_MAX_FACES, s->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) return set_err(-2, "p2g adjoint (no allocation): %s", hipGetErrorString(e));
     return 0;
 }
 
