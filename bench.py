@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -54,7 +56,7 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0, pressure=0):
     s.set_timing(True)
     stage_ms = [0.0] * 9
     solve_ms, iters, outer, worst = 0.0, 0, 0, 0.0
-    sweep_ms, sweeps = 0.0, 0
+    sweep_ms, sweeps, fallbacks, box_cells = 0.0, 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(steps):
         rc = s.timeStep(dt)
@@ -62,6 +64,8 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0, pressure=0):
         solve_ms += st.ms
         sweep_ms += st.sweep_ms
         sweeps += st.sweeps
+        fallbacks += st.fallback
+        box_cells += st.box_cells * st.sweeps // 2  # weighted by the timed preconditioner applications
         iters += st.iterations
         outer += st.outer
         worst = max(worst, st.residual / max(st.rhs_norm, 1e-300))
@@ -75,7 +79,8 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0, pressure=0):
     n = s.num_particles()
     s.close()
     return dict(elapsed=elapsed, solve_ms=solve_ms, iters=iters, outer=outer, worst_rel_res=worst,
-                stage_ms=stage_ms, particles=n, sweep_ms=sweep_ms, sweeps=sweeps)
+                stage_ms=stage_ms, particles=n, sweep_ms=sweep_ms, sweeps=sweeps, fallbacks=fallbacks,
+                box_cells=box_cells / max(sweeps // 2, 1))
 
 
 def measured_traffic(cfg):
@@ -90,22 +95,59 @@ def measured_traffic(cfg):
     return None
 
 
-def cpu_baseline(cfg, dt):
-    """The oracle (FP64 C++ restatement, reference-faithful threading) on one substep."""
+def host_cores():
+    """CPU threads this process may run on (the box's share, not the machine's)."""
+    return len(os.sched_getaffinity(0))
+
+
+def oracle_run(cfg, dt, warmup):
+    """The oracle (FP64 C++ restatement, reference-faithful threading: serial
+    assembly, MIC(0), PCG and P2G; 6-chunk OpenMP density and G2P) on the same
+    scene: `warmup` untimed substeps, then one timed substep.  Returns the
+    baseline record and the oracle (its state after warmup + 1 substeps)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from scenes import oracle_dam_break
     from oracle_lib import default_params
     ni, nj, nk, dx, frac, seed, maxit = CONFIGS[cfg]
     p = default_params(flip_ratio=0.95, pcg_tol=1e-6, pcg_max_iter=maxit)
     o = oracle_dam_break(ni, nj, nk, dx, frac, seed, p)
+    for _ in range(warmup):
+        o.time_step(dt)
     t0 = time.perf_counter()
     o.time_step(dt)
     el = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return dict(value=1.0 / el, unit="substeps/s", cores=min(cores, 6), kind="port",
-                sample=f"1 substep of {cfg} from rest ({o.lib.xo_num_particles(o.h)} particles), "
-                       f"oracle in reference-faithful mode (serial assembly/MIC(0)/PCG/P2G, 6-chunk OpenMP "
-                       f"density and G2P), {el:.1f} s")
+    st = o.last_solve()
+    omp = int(os.environ.get("OMP_NUM_THREADS", host_cores()))
+    rec = dict(value=1.0 / el, unit="substeps/s", cores=min(omp, 6), kind="port",
+               host_cores=host_cores(), omp_threads=omp, pcg_iterations=int(st.iterations),
+               sample=f"substep {warmup + 1} of {cfg} ({o.lib.xo_num_particles(o.h)} particles, the GPU run's "
+                      f"scene and state), timed alone after {warmup} untimed substeps; the oracle in "
+                      f"reference-faithful mode: serial assembly/MIC(0)/PCG/P2G, 6-chunk OpenMP density and "
+                      f"G2P (so at most 6 of the {host_cores()} host threads work); FP64 PCG to 1e-6, "
+                      f"{st.iterations} iterations; {el:.2f} s")
+    return rec, o
+
+
+def per_field_error(xf, cfg, o, substeps, dt):
+    """Relative max-norm error of the GPU's pressure, u, v, w and particle
+    positions against the FP64 oracle after the same `substeps` substeps from
+    the same initial state, for the FP64 and the mixed-precision PCG."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import FIELD
+    ref = {f: o.get(FIELD[f]) for f in ("PRESSURE", "U", "V", "W")}
+    ref["positions"] = o.particles()[0]
+    out = {}
+    for name, prec in (("fp64", xf.PREC_FP64), ("mixed", xf.PREC_MIXED)):
+        s = make_sim(xf, cfg, prec)
+        for _ in range(substeps):
+            s.timeStep(dt)
+        got = {f: s.get_field(FIELD[f]) for f in ("PRESSURE", "U", "V", "W")}
+        got["positions"] = s.particles()[0]
+        s.close()
+        out[name] = {k.lower(): float(np.abs(got[k] - ref[k]).max() / max(np.abs(ref[k]).max(), 1e-300))
+                     for k in ref}
+    out["substeps"] = substeps
+    return out
 
 
 def copy_calibration(mib):
@@ -267,20 +309,24 @@ def main():
         log(f"[bench] fp64: {fp64['elapsed']:.3f} s, solve {fp64['solve_ms']:.1f} ms, iters {fp64['iters']}")
     K = args.steps
     value = K / mixed["elapsed"]
-    # Roofline of the dominant kernel, the pipelined MIC(0) sweep (k_sweep_w8):
-    # One launch of k_sweep_w8_fb = the forward and the backward sweep of one
-    # preconditioner application.  Algorithmic bytes per sweep over the N-cell
-    # grid = read input + pivots + 2-byte stencil code, write output: FP32
-    # (mixed run) 4+4+2+4 = 14 B/cell, FP64 26 B/cell; 28 B/cell per launch.
-    # Average launch duration from device events around the launches of every
-    # 8th application of the timed substeps (the library counts 2 sweeps each).
+    # Roofline of the dominant kernel, the pipelined MIC(0) sweep: one launch of
+    # k_sweep_w8_fb = the forward and the backward sweep of one preconditioner
+    # application.  Algorithmic bytes per cell and sweep = read input + pivot +
+    # 2-byte stencil code, write output: FP32 (mixed run) 4+4+2+4 = 14 B, FP64
+    # 26 B; 28 B per cell and launch in the mixed run.  The cells are those of
+    # the active rows' bounding box (the library's box_cells: the only cells the
+    # sweep and the PCG vector kernels visit); the full-grid figure (N cells)
+    # is reported beside it.  Average launch duration from device events on the
+    # library's stream around the launches of every 5th application.
     apps = mixed["sweeps"] // 2
     sweep_us = 1e3 * mixed["sweep_ms"] / max(apps, 1)
-    ach_sweep = 28.0 * N / (sweep_us * 1e-6) / 1e9 if apps else 0.0
+    box = mixed["box_cells"]
+    ach_sweep = 28.0 * box / (sweep_us * 1e-6) / 1e9 if apps else 0.0
+    ach_sweep_full = 28.0 * N / (sweep_us * 1e-6) / 1e9 if apps else 0.0
     # whole-solve rate (algorithmic bytes per PCG iteration: FP64 131 B/cell,
-    # FP32 bulk 67 B/cell over the full N-cell vectors)
+    # FP32 bulk 67 B/cell over the active box)
     it_m = max(mixed["iters"], 1)
-    ach_m = 67.0 * N * it_m / (mixed["solve_ms"] * 1e-3) / 1e9
+    ach_m = 67.0 * box * it_m / (mixed["solve_ms"] * 1e-3) / 1e9
     out = {
         "metric": "substeps/s (128^3 dam break, mixed-precision PCG to 1e-6 rel. residual)",
         "value": round(value, 4),
@@ -307,8 +353,10 @@ def main():
                      "achieved": round(ach_sweep, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_sweep / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(args.config) if args.sweep in (0, 9) else None,
-                     "bytes_per_launch": 28 * N, "avg_launch_us": round(sweep_us, 2),
-                     "launches_timed": apps},
+                     "bytes_per_launch": int(28 * box), "box_cells": int(box), "avg_launch_us": round(sweep_us, 2),
+                     "launches_timed": apps,
+                     "achieved_full_grid": round(ach_sweep_full, 2), "bytes_per_launch_full_grid": 28 * N},
+        "sweep_fallbacks_mixed": mixed["fallbacks"],
         "pcg_mixed_achieved_GBs": round(ach_m, 2),
     }
     if fp64:
@@ -317,12 +365,15 @@ def main():
         out["pcg_ms_per_solve_fp64"] = round(fp64["solve_ms"] / K, 3)
         out["pcg_iters_per_solve_fp64"] = round(fp64["iters"] / K, 2)
         out["mixed_vs_fp64_solve_rate"] = round(fp64["solve_ms"] / max(mixed["solve_ms"], 1e-9), 3)
-        out["pcg_fp64_achieved_GBs"] = round(131.0 * N * it_f / (fp64["solve_ms"] * 1e-3) / 1e9, 2)
+        out["pcg_fp64_achieved_GBs"] = round(131.0 * fp64["box_cells"] * it_f / (fp64["solve_ms"] * 1e-3) / 1e9, 2)
+        out["sweep_fallbacks_fp64"] = fp64["fallbacks"]
         if fp64["sweeps"]:
             out["sweep_fp64_avg_launch_us"] = round(1e3 * fp64["sweep_ms"] / (fp64["sweeps"] // 2), 2)
     if not args.no_cpu:
-        log("[bench] cpu baseline (oracle, 1 substep)")
-        out["cpu_baseline"] = cpu_baseline(args.config, dt)
+        log(f"[bench] cpu baseline (oracle: {args.warmup} untimed substeps + 1 timed)")
+        out["cpu_baseline"], o = oracle_run(args.config, dt, args.warmup)
+        log("[bench] per-field error of the GPU runs against the oracle after the same substeps")
+        out["per_field_error_vs_fp64_oracle"] = per_field_error(xf, args.config, o, args.warmup + 1, dt)
     print(json.dumps(out), flush=True)
 
 

@@ -55,9 +55,7 @@ typedef struct xf_params {
     int    tail_skip;         /* 1: the last n%6 particles are not advected (reference chunking) */
     int    density_every_substep; /* 1: density each substep (reference); 0: only when a mesh is output */
     int    sweep_kernel;      /* MIC(0) sweep schedule: 0 auto (= 9), 1 per-hyperplane launches,
-                                 2..5 pipelined 16x16 line tiles, chunks of 8, 4, 2, 1 steps,
-                                 6, 7 pipelined 16x16 line tiles with load/store waves,
-                                 8 one wavefront per 16x16 tile, 9 one wavefront per 8x8 tile */
+                                 9 pipelined, one wavefront per 8x8 tile of i-lines */
     double mixed_inner_tol;   /* mixed mode: relative reduction asked of an FP32 inner solve; later
                                  inner solves stop at half of what is left to the outer tolerance */
     int    mixed_max_outer;   /* mixed mode: cap on FP64 refinement steps           */
@@ -117,7 +115,8 @@ typedef struct xf_solve_stats {
     int    status;            /* 0 converged, 1 not converged, 2 rho invalid       */
     int    iterations;        /* PCG iterations (mixed: total inner iterations)    */
     int    outer;             /* mixed: FP64 refinement steps; FP64: 1             */
-    int    reserved;
+    int    fallback;          /* 1: a pipelined sweep wait timed out and the solve was
+                                 redone with per-hyperplane sweeps                  */
     double residual;          /* final inf-norm residual (FP64)                    */
     double rhs_norm;          /* inf-norm of the right-hand side                   */
     double ms;                /* wall time of the solve on the device, ms          */
@@ -126,6 +125,9 @@ typedef struct xf_solve_stats {
                                  preconditioner application, ms                    */
     int    sweeps;            /* number of sweep launches summed in sweep_ms       */
     int    reserved2;
+    int64_t box_cells;        /* cells in the bounding box of the active rows: the
+                                 cells the PCG iteration kernels and the pipelined
+                                 sweeps visit (the other rows are empty)           */
 } xf_solve_stats;
 
 /* Per-substep stage timings (device events, ms), filled when timing is on. */

@@ -22,7 +22,7 @@ CASES = {
     "c1_small": (24, 16, 12, 0.125, (0.35, 0.6, 1.0), 0),
     "c1_quarter": (32, 16, 16, 0.125, (0.35, 0.6, 1.0), 0),
     "odd": (19, 13, 11, 0.1, (0.5, 0.55, 1.0), 7),
-    # several 16x16 (j,k) tiles: exercises the tile-to-tile hand-off of the pipelined sweeps
+    # several 8x8 (j,k) line tiles in both directions: exercises the tile-to-tile hand-off of the pipelined sweeps
     "tiles": (21, 40, 35, 0.05, (0.6, 0.7, 1.0), 9),
 }
 
@@ -304,8 +304,7 @@ def test_frame_step_runs(xf):
     _close(Pg, Po, 1e-8, "positions after one frame")
 
 
-@pytest.mark.parametrize("case,kind", [("odd", 2), ("c1_quarter", 3)] + [("tiles", k) for k in (2, 3, 4, 5, 6, 7, 8, 9)]
-                         + [("odd", 6), ("c1_quarter", 7), ("odd", 8), ("c1_quarter", 8), ("odd", 9), ("c1_quarter", 9)])
+@pytest.mark.parametrize("case,kind", [("tiles", 9), ("odd", 9), ("c1_quarter", 9), ("c1_small", 9)])
 def test_pipelined_sweeps_bitexact(xf, case, kind):
     """The pipelined line-tile sweeps give the same bits as the serial sweeps,
     and a whole solve with them equals the per-hyperplane solve exactly."""
@@ -340,12 +339,12 @@ def test_pipelined_sweeps_bitexact(xf, case, kind):
     copy_state(o, g)
     assert g.stage(STAGE["SOLVE"]) == 0
     st = g.last_solve()
-    assert st.reserved == 0, "pipelined wait timed out"
+    assert st.fallback == 0, "pipelined wait timed out"
     assert st.iterations == it1
     _exact(g.get_field(FIELD["PRESSURE"]), p1, "pressure, pipelined vs per-plane sweeps")
 
 
-@pytest.mark.parametrize("case,kind", [("tiles", 6), ("tiles", 8), ("odd", 8), ("tiles", 9), ("odd", 9)])
+@pytest.mark.parametrize("case,kind", [("tiles", 9), ("odd", 9), ("c1_quarter", 9)])
 def test_pipelined_sweeps_fp32_bitexact(xf, case, kind):
     """FP32 sweeps (mixed-precision solve): a pipelined schedule gives the same
     bits as the per-hyperplane FP32 sweeps, so the whole mixed solve (iteration
@@ -365,7 +364,7 @@ def test_pipelined_sweeps_fp32_bitexact(xf, case, kind):
         g.set_params(p)
         assert g.stage(STAGE["SOLVE"]) == 0
         st = g.last_solve()
-        assert st.reserved == 0, "pipelined wait timed out"
+        assert st.fallback == 0, "pipelined wait timed out"
         res.append((st.iterations, st.outer, g.get_field(FIELD["PRESSURE"])))
     assert res[0][:2] == res[1][:2]
     _exact(res[1][2], res[0][2], "mixed pressure, pipelined vs per-plane FP32 sweeps")

@@ -299,7 +299,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
     launch_stencil_rhs(s->g, s->mat, fu, fv, fw, s->code, s->pw.b, s->st, s->prm.pressure_solver);
     if (rhs_override)
         HIPCHK(hipMemcpyAsync(s->pw.b, rhs_override, s->g.N * 8, hipMemcpyDeviceToDevice, s->st));
-    s->last.reserved = 0;
+    s->last.fallback = 0;
     const xf_params& p = s->prm;
     SweepTimer& tm = s->pw.timer;
     tm.on = s->timing;
@@ -322,7 +322,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
             tm.used = 0;
             r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                           p.mixed_inner_tol, p.mixed_max_outer, 1, s->st);
-            s->last.reserved = 1;
+            s->last.fallback = 1;
         }
     }
     HIPCHK(hipEventRecord(e1, s->st));
@@ -337,6 +337,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
     s->last.outer = r.outer;
     s->last.residual = r.residual;
     s->last.rhs_norm = r.rhs_norm;
+    s->last.box_cells = r.box_cells;
     s->last.ms = ms;
     s->last.sweep_ms = 0.0;
     s->last.sweeps = 0;
@@ -462,35 +463,18 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     if (dalloc(&s->pw.sweep_flags, s->pw.nflags) != hipSuccess) return fail(set_err(-2, "alloc flags"));
     if (dalloc(&s->pw.box_dev, 8) != hipSuccess || hipHostMalloc((void**)&s->pw.box_host, 8 * sizeof(int)) != hipSuccess)
         return fail(set_err(-2, "alloc box"));
-    s->pw.nskew = sweep_skewed_size(s->g);
     {
-        const long M = s->pw.nskew;
         SweepSkew<double>& a = s->pw.sk64;
         SweepSkew<float>& b = s->pw.sk32;
-        if (dalloc(&a.code, M) != hipSuccess || dalloc(&a.inv, M) != hipSuccess || dalloc(&a.r, M) != hipSuccess ||
-            dalloc(&a.q, M) != hipSuccess || dalloc(&a.z, M) != hipSuccess || dalloc(&b.inv, M) != hipSuccess ||
-            dalloc(&b.r, M) != hipSuccess || dalloc(&b.q, M) != hipSuccess || dalloc(&b.z, M) != hipSuccess ||
-            dalloc(&a.eb, M / 4 + 64) != hipSuccess || dalloc(&b.eb, M / 4 + 64) != hipSuccess)
-            return fail(set_err(-2, "alloc sweep buffers"));
-        b.code = a.code;
-        // single-wave schedule buffers
-        const long W = sweep_wave_size(s->g), E = sweep_wave_edge_words(s->g);
-        if (dalloc(&a.wL, 7 * W) != hipSuccess || dalloc(&a.wm, 6 * W) != hipSuccess || dalloc(&a.wr, W) != hipSuccess ||
-            dalloc(&a.wq, W) != hipSuccess || dalloc(&a.wz, W) != hipSuccess || dalloc(&a.wedge, E) != hipSuccess ||
-            dalloc(&b.wL, 7 * W) != hipSuccess || dalloc(&b.wm, 6 * W) != hipSuccess || dalloc(&b.wr, W) != hipSuccess ||
-            dalloc(&b.wq, W) != hipSuccess || dalloc(&b.wz, W) != hipSuccess || dalloc(&b.wedge, E) != hipSuccess)
-            return fail(set_err(-2, "alloc sweep buffers"));
         const long W8 = sweep_w8_size(s->g), E8 = sweep_w8_edge_words(s->g);
         if (dalloc(&a.oRec, 8 * W8) != hipSuccess || dalloc(&a.oR, W8) != hipSuccess || dalloc(&a.oQ, W8) != hipSuccess ||
             dalloc(&a.oZ, W8) != hipSuccess || dalloc(&a.oEdge, E8) != hipSuccess || dalloc(&b.oRec, 8 * W8) != hipSuccess ||
             dalloc(&b.oR, W8) != hipSuccess || dalloc(&b.oQ, W8) != hipSuccess || dalloc(&b.oZ, W8) != hipSuccess ||
             dalloc(&b.oEdge, E8) != hipSuccess)
             return fail(set_err(-2, "alloc sweep buffers"));
+        // tags are never zero: zeroed edge words read as not yet written
         hipMemsetAsync(a.oEdge, 0, E8 * sizeof(uint32_t), s->st);
         hipMemsetAsync(b.oEdge, 0, E8 * sizeof(uint32_t), s->st);
-        // tags are never zero: zeroed edge words read as not yet written
-        hipMemsetAsync(a.wedge, 0, E * sizeof(uint32_t), s->st);
-        hipMemsetAsync(b.wedge, 0, E * sizeof(uint32_t), s->st);
     }
     hipMemsetAsync(s->pw.sweep_flags, 0, s->pw.nflags * sizeof(int), s->st);
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
@@ -536,12 +520,6 @@ void xf_sim_destroy(xf_sim* s) {
     dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
     dfree(s->pw.box_dev);
     if (s->pw.box_host) { hipHostFree(s->pw.box_host); s->pw.box_host = nullptr; }
-    dfree(s->pw.sk64.code); dfree(s->pw.sk64.inv); dfree(s->pw.sk64.r); dfree(s->pw.sk64.q); dfree(s->pw.sk64.z);
-    dfree(s->pw.sk32.inv); dfree(s->pw.sk32.r); dfree(s->pw.sk32.q); dfree(s->pw.sk32.z);
-    dfree(s->pw.sk64.eb); dfree(s->pw.sk32.eb);
-    dfree(s->pw.sk64.wL); dfree(s->pw.sk64.wm); dfree(s->pw.sk64.wr); dfree(s->pw.sk64.wq); dfree(s->pw.sk64.wz);
-    dfree(s->pw.sk64.wedge); dfree(s->pw.sk32.wL); dfree(s->pw.sk32.wm); dfree(s->pw.sk32.wr); dfree(s->pw.sk32.wq);
-    dfree(s->pw.sk32.wz); dfree(s->pw.sk32.wedge);
     dfree(s->pw.sk64.oRec); dfree(s->pw.sk64.oR); dfree(s->pw.sk64.oQ); dfree(s->pw.sk64.oZ); dfree(s->pw.sk64.oEdge);
     dfree(s->pw.sk32.oRec); dfree(s->pw.sk32.oR); dfree(s->pw.sk32.oQ); dfree(s->pw.sk32.oZ); dfree(s->pw.sk32.oEdge);
     mesh_free(s->mesh);
@@ -570,6 +548,9 @@ int xf_sim_set_params(xf_sim* s, const xf_params* p) {
     if (check(s) || !p) return set_err(-1, "null argument");
     if (p->sub_division < 1 || p->sub_division > 4) return set_err(-1, "sub_division must be 1..4");
     if (p->pcg_max_iter < 0) return set_err(-1, "pcg_max_iter < 0");
+    if (p->sweep_kernel != 0 && p->sweep_kernel != 1 && p->sweep_kernel != 9)
+        return set_err(-1, "sweep_kernel must be 0 (auto), 1 (per-hyperplane) or 9 (8x8 line tiles), got %d",
+                       p->sweep_kernel);
     if (p->pressure_solver != XF_PRESSURE_VARIATIONAL && p->pressure_solver != XF_PRESSURE_NORMAL)
         return set_err(-1, "unknown pressure_solver %d", p->pressure_solver);
     s->prm = *p;

@@ -40,22 +40,11 @@ void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* mar
                         hipStream_t st);
 
 // ---------------- pressure solve kernels (xf_pcg.hip) ----------------
-// Inputs/outputs of the pipelined sweeps in their skewed tile layout
-// (xf_sweep.hip): stencil codes, pivots, the residual and the forward result.
+// Buffers of the pipelined 8x8-tile sweeps (xf_sweep_wave.hip), oct layout:
+// forward and backward records, input / forward / backward vectors, tagged edge
+// words.
 template <class T>
 struct SweepSkew {
-    uint16_t* code;
-    T *inv, *r, *q, *z;
-    T* eb;  // tile-edge hand-off buffer of the memory-wave schedule
-    // single-wave schedule (xf_sweep_wave.hip), quad tile layout: 7 coefficient
-    // planes, 6 presence-mask planes, input / forward / backward vectors, and
-    // the tagged edge words
-    T* wL;
-    uint32_t* wm;
-    T *wr, *wq, *wz;
-    uint32_t* wedge;
-    // 8x8-tile schedule (xf_sweep_wave.hip), oct layout: forward and backward
-    // records, input / forward / backward vectors, tagged edge words
     T* oRec;
     T *oR, *oQ, *oZ;
     uint32_t* oEdge;
@@ -90,9 +79,8 @@ struct PcgWork {
     hipEvent_t chunk_ev[2];  // ends of the PCG's iteration chunks in flight
     int* sweep_flags;    // error word + progress counters of the pipelined sweep
     int nflags;
-    SweepSkew<double> sk64;  // skewed copies for the pipelined sweeps (FP64 solve)
-    SweepSkew<float> sk32;   // (FP32 inner solve; shares sk64.code)
-    long nskew;
+    SweepSkew<double> sk64;  // pipelined sweep buffers (FP64 solve)
+    SweepSkew<float> sk32;   // (FP32 inner solve)
     SweepTimer timer;
     unsigned long long epoch;  // sweep launch counter (tags progress counters)
     int* box_dev;   // 6 ints: bounding box of the active rows
@@ -101,6 +89,7 @@ struct PcgWork {
 struct PcgResult {
     int status, iterations, outer;
     double residual, rhs_norm;
+    long box_cells;  // cells of the active rows' bounding box
 };
 // Assemble the stencil codes and right-hand side, factor, and solve A x = b.
 PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,
@@ -115,21 +104,8 @@ bool pcg_active_box(const Geo& g, const uint16_t* code, PcgWork& w, int box[6], 
 void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const double* r, double* q, double* z,
                  int sweep_kernel, PcgWork& w, hipStream_t st);
 void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hipStream_t st);
-// ints needed for the pipelined sweep's error word + progress counters
+// ints of the pipelined sweep's flags buffer (flags[0]: error word)
 int sweep_flag_words(const Geo& g);
-// elements of one skewed-layout array
-long sweep_skewed_size(const Geo& g);
-// skewed copies of the stencil codes (if code != null) and pivots
-template <class T>
-void sweep_prepare(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
-// single-wave schedule: elements per quad-layout plane, edge-buffer words
-long sweep_wave_size(const Geo& g);
-long sweep_wave_edge_words(const Geo& g);
-template <class T>
-void sweep_prepare_wave(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
-template <class T>
-bool sweep_precond_wave(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
-                        unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
 // MIC(0) pivots through the 8x8-tile pipeline over sk.box (false: not run)
 bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const SweepSkew<double>& sk,
              int* flags, unsigned long long* epoch, hipStream_t st);
@@ -144,9 +120,6 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st,
                       double* dot_partial = nullptr, int dot_blocks = 0, const BoxRange* dot_range = nullptr,
                       FinRes* fin = nullptr);
-template <class T>
-bool sweep_precond_pipelined(const Geo& g, int kind, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S,
-                             int* flags, unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st);
 
 // ---------------- particle kernels (xf_particles.hip) ----------------
 struct Particles {

@@ -219,20 +219,16 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
         launch_finish_res(*fin, st);
         fin->ran = true;
     }
-    // pipelined schedule (xf_sweep.hip) on the skewed copies prepared by the caller
-    if (sweep_kernel >= 2 && sk) {
+    // pipelined 8x8-tile schedule (xf_sweep_wave.hip) on the records prepared by the caller
+    if (sweep_kernel == 9 && sk) {
         hipEvent_t a = nullptr, b = nullptr;
         if (tm && tm->on && tm->k++ % SWEEP_TIMER_STRIDE == 0 && tm->used + 2 <= tm->cap) {
             a = tm->ev[tm->used];
             b = tm->ev[tm->used + 1];
         }
-        const bool ran = sweep_kernel == 9 ? sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st,
-                                                                 dot_partial, RED_BLOCKS, dot_range, fin)
-                         : sweep_kernel >= 8 ? sweep_precond_wave<T>(g, *sk, r, z, S, flags, epoch, a, b, st)
-                                           : sweep_precond_pipelined<T>(g, sweep_kernel, *sk, r, z, S, flags, epoch, a, b, st);
-        if (ran) {
+        if (sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st, dot_partial, RED_BLOCKS, dot_range, fin)) {
             if (a) tm->used += 2;
-            return sweep_kernel == 9 && dot_partial && dot_range;
+            return dot_partial && dot_range;
         }
     }
     if (fin && !fin->ran) {
@@ -262,8 +258,6 @@ void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const do
         w.sk64.box[1] = g.ni - 1; w.sk64.box[3] = g.nj - 1; w.sk64.box[5] = g.nk - 1;
     }
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, inv, w.sk64, st);
-    else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, inv, w.sk64, st);
-    else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, inv, w.sk64, st);
     precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, nullptr, st);
 }
 
@@ -635,20 +629,24 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     };
     int chunk = 0;
     launch_chunk(0);
+    // the host reads each chunk's slot into its own copy: the chunk enqueued
+    // next may already be writing the other slot
+    PcgScalars hs{};
     for (;;) {
         if (it < max_iter) launch_chunk(chunk + 1);
         hipEventSynchronize(chunk_ev[chunk & 1]);
-        const PcgScalars* got = S_host + ((chunk & 1) ? 2 : 0);
-        if (got != S_host) *S_host = *got;
-        if (S_host->done || chunk * CH + CH >= max_iter) break;
+        hs = S_host[(chunk & 1) ? 2 : 0];
+        if (hs.done || chunk * CH + CH >= max_iter) break;
         ++chunk;
     }
     hipStreamSynchronize(st);
-    if (!S_host->done) {
+    if (!hs.done) {
         hipMemcpyAsync(S_host, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
-        if (!S_host->done) { S_host->status = 1; S_host->iters = max_iter; }
+        hs = *S_host;
+        if (!hs.done) { hs.status = 1; hs.iters = max_iter; }
     }
+    *S_host = hs;  // (the stream is idle: no kernel writes the slots now)
     if (tm && tm->used > tm_start) {
         // keep the timings of the applications that ran: the initial one and one per
         // iteration before the converged one (later launches returned at once);
@@ -670,11 +668,10 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         box[1] = g.ni - 1; box[3] = g.nj - 1; box[5] = g.nk - 1;
     }
     for (int d = 0; d < 6; ++d) w.sk64.box[d] = w.sk32.box[d] = box[d];
+    res.box_cells = (long)(box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (box[5] - box[4] + 1);
     if (!(sweep_kernel == 9 && mic0_w8(g, code, w.inv, tau, sigma, w.sk64, w.sweep_flags, &w.epoch, st)))
         pcg_mic0(g, code, w.inv, tau, sigma, box, st);
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
-    else if (sweep_kernel >= 8) sweep_prepare_wave<double>(g, code, w.inv, w.sk64, st);
-    else if (sweep_kernel >= 2) sweep_prepare<double>(g, code, w.inv, w.sk64, st);
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
@@ -695,8 +692,6 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     PcgScalars* S_outer_h = w.S_host + 1;
     k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.inv32, w.inv, nullptr);
     if (sweep_kernel == 9) sweep_prepare_w8<float>(g, code, w.inv32, w.sk32, st);
-    else if (sweep_kernel >= 8) sweep_prepare_wave<float>(g, code, w.inv32, w.sk32, st);
-    else if (sweep_kernel >= 2) sweep_prepare<float>(g, nullptr, w.inv32, w.sk32, st);
     k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
     k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
     k_absmax<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.partial);

@@ -1,73 +1,40 @@
-// xf_sweep_wave.hip -- MIC(0) triangular sweeps with one wavefront per line tile.
+// xf_sweep_wave.hip -- the MIC(0) triangular sweeps (pcg_solver.h:192-220) and
+// the MIC(0) factorisation (pcg_solver.h:85-186) as pipelined line-tile
+// schedules, one wavefront per 8x8 tile of i-lines.
 //
-// The recurrences are those of xf_sweep.hip (pcg_solver.h:192-220); this is a
-// second schedule for them.  The (j,k) plane is cut into 16x16 tiles of
-// i-lines and a single compute wavefront owns a tile: lane = jl + 16*g holds
-// the four lines kl = 4g..4g+3 and walks the tile's skewed hyperplanes
-// (forward step s: the cells with i + jl + kl = s; backward: the same slots in
-// reverse order).  At step s the -i neighbour of a cell is the same register
-// one step earlier, the -k neighbour the register of the line below it (for
-// kl = 4g: the top line of lane-16, one ds_bpermute), the -j neighbour lane-1's
-// register (a DPP row shift; a 16-lane DPP row is exactly one line group).  A
-// step needs no barrier and no LDS round trip for the recurrence itself.
-// Backward sweeps mirror every direction.
+// A compute wavefront owns a tile: lane jl + 8*kl holds line (jl, kl) and walks
+// the tile's skewed hyperplanes (forward step s: the cells with i + jl + kl = s;
+// backward: the same slots in reverse order).  At step s the -i neighbour of a
+// cell is the same register one step earlier, the -j neighbour lane-1's register
+// (a DPP row shift), the -k neighbour lane-8's (ds_bpermute).  A step needs no
+// barrier and no LDS round trip for the recurrence itself.  Backward sweeps
+// mirror every direction.
 //
 // The per-cell coefficients L_cm = A_cm * inv_m (forward: the neighbour's
-// pivot, pcg_solver.h:198-201; backward: the own pivot, :214-216) and the
-// presence of each stencil entry are formed once per solve ("records").  A
-// missing entry keeps the running value bit for bit (the serial sweep does not
-// visit it), so each term is evaluated as w = v - L*q and kept through a
-// bitwise select with an all-ones mask for absent entries.
-//
-// Two load waves copy each step's input and records straight from global
-// memory into an LDS ring (global_load_lds, 16 bytes per lane) WV_LEAD steps
-// ahead of the compute wave, and hand slots over through LDS counters.
+// pivot, pcg_solver.h:198-201; backward: the own pivot, :214-216) are formed
+// once per solve ("records").  A missing entry keeps the running value bit for
+// bit (the serial sweep does not visit it), so each term is evaluated as
+// w = v - L*q and kept through a bitwise select for absent entries.
 //
 // Tiles pass edge values to their downstream neighbours as tagged words: a
-// value is stored together with the 32-bit tag of the sweep in one 8-byte word
-// (FP64: two words, one per half), with device-scope stores.  A reader that
-// sees the tag has the value, so no progress counter and no wait for store
-// completion is involved.  The load wave checks the tags of the edge words it
-// copied and re-reads stale ones until the upstream tile has written them.
-// Waits are bounded: on timeout an error bit is set and the kernel still runs
-// to completion, and the host redoes the solve with per-hyperplane sweeps.
-//
-// Quad layout (inputs, outputs, records): element (slot, lane, c) of tile
-// t = J*ntk + K is stored at ((t*SL + slot)*64 + lane)*4 + c, SL = ni + 30;
-// it is cell (i, 16J + jl, 16K + 4g + c) with i = slot - jl - 4g - c.  Cells
-// outside the grid hold zeros, pivot 0 and all entries absent, and evaluate to
-// zero.  Edge rows: per tile and step 32 words, 0-15 the tile's edge column
-// (forward jl = 15, backward jl = 0) by kl, 16-31 the edge row (forward
-// kl = 15, backward kl = 0) by jl.
+// value is stored together with a 32-bit tag in one 8-byte word (FP64: two
+// words, one per half), with device-scope stores.  A reader that sees the tag
+// has the value, so no progress counter and no wait for store completion is
+// involved.  Waits are bounded: on timeout an error bit is set in flags[0] and
+// the kernel still runs to completion, and the host redoes the solve with
+// per-hyperplane sweeps.
 #include <hip/hip_ext.h>
 
 #include "xf_internal.h"
 
-#define WV_PAD 30
-#define WV_R 8      // LDS ring depth in steps
-#define WV_LEAD 6   // steps of loads each load wave keeps in flight
 #define WV_SPIN (1 << 20)       // LDS polls before a wait inside the workgroup is declared stuck
 #define WV_SPIN_EDGE (1 << 16)  // re-reads of an edge word (each a device-scope round trip)
 
 typedef __attribute__((address_space(3))) unsigned char lds_u8;
 typedef __attribute__((address_space(3))) int lds_int;
 
-static inline void wv_tiles(const Geo& g, int* ntj, int* ntk) {
-    *ntj = (g.nj + 15) / 16;
-    *ntk = (g.nk + 15) / 16;
-}
-__host__ __device__ inline int wv_slots(const Geo& g) { return g.ni + WV_PAD; }
-
-long sweep_wave_size(const Geo& g) {
-    int ntj, ntk;
-    wv_tiles(g, &ntj, &ntk);
-    return (long)ntj * ntk * wv_slots(g) * 256;
-}
-long sweep_wave_edge_words(const Geo& g) {
-    int ntj, ntk;
-    wv_tiles(g, &ntj, &ntk);
-    return (long)ntj * ntk * wv_slots(g) * 32 * 4;  // u32 units, sized for FP64 words
-}
+// error word of the sweeps (flags[0]) and a spare
+int sweep_flag_words(const Geo&) { return 2; }
 
 // tiles by diagonal J+K, upstream tiles first (a tile only ever waits on tiles
 // that were dispatched before it)
@@ -82,89 +49,6 @@ __device__ inline void wv_tile(int b, int ntj, int ntk, bool fwd, int* J, int* K
     }
     if (!fwd) { *J = ntj - 1 - *J; *K = ntk - 1 - *K; }
 }
-
-__device__ inline void wv_cell(const Geo& g, int ntk, long e, int* i, int* j, int* k) {
-    const int SL = wv_slots(g);
-    const int c = (int)(e & 3), lane = (int)((e >> 2) & 63);
-    const long row = e >> 8;
-    const long t = row / SL;
-    const int slot = (int)(row - t * SL);
-    const int jl = lane & 15, kl = 4 * (lane >> 4) + c;
-    *j = (int)(t / ntk) * 16 + jl;
-    *k = (int)(t % ntk) * 16 + kl;
-    *i = slot - jl - kl;
-}
-
-template <class D, class Sx>
-__global__ void k_to_quad(Geo g, int ntk, long n, const Sx* __restrict__ src, D* __restrict__ dst,
-                          const PcgScalars* S) {
-    if (S && S->done) return;
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    int i, j, k;
-    wv_cell(g, ntk, e, &i, &j, &k);
-    D v = (D)0;
-    if (j < g.nj && k < g.nk && i >= 0 && i < g.ni) v = (D)src[cidx(g, i, j, k)];
-    dst[e] = v;
-}
-
-template <class T>
-__global__ void k_from_quad(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S) {
-    if (S && S->done) return;
-    const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (c >= g.N) return;
-    const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
-    const long t = (long)(j >> 4) * ntk + (k >> 4);
-    const int jl = j & 15, kl = k & 15;
-    const int slot = i + jl + kl, lane = jl + 16 * (kl >> 2);
-    dst[c] = src[(((t * wv_slots(g) + slot) * 64 + lane) << 2) + (kl & 3)];
-}
-
-// Records: L planes 0-2 forward (-k, -j, -i), 3-5 backward (+i, +j, +k), 6 the
-// pivots; mask planes 0-2 forward, 3-5 backward (0: entry present, ~0: absent).
-template <class T>
-__global__ void k_wv_records(Geo g, int ntk, long n, const uint16_t* __restrict__ code, const T* __restrict__ inv,
-                             T* __restrict__ L, uint32_t* __restrict__ M) {
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    int i, j, k;
-    wv_cell(g, ntk, e, &i, &j, &k);
-    T lf[3] = {0, 0, 0}, lb[3] = {0, 0, 0}, ic = 0;
-    uint32_t mf[3] = {~0u, ~0u, ~0u}, mb[3] = {~0u, ~0u, ~0u};
-    if (j < g.nj && k < g.nk && i >= 0 && i < g.ni) {
-        const long c = cidx(g, i, j, k);
-        const uint16_t cd = code[c];
-        ic = inv[c];
-        const int dF[3] = {D_ZM, D_YM, D_XM}, dB[3] = {D_XP, D_YP, D_ZP};
-        const long oF[3] = {-g.nn, -(long)g.ni, -1};
-        const bool inF[3] = {k > 0, j > 0, i > 0};  // (the stencil has no entry across the grid boundary)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int cf = code_dir(cd, dF[d]), cb = code_dir(cd, dB[d]);
-            if ((cf == 1 || cf == 2) && inF[d]) { lf[d] = (T)code_coef(cf) * inv[c + oF[d]]; mf[d] = 0u; }
-            if (cb == 1 || cb == 2) { lb[d] = (T)code_coef(cb) * ic; mb[d] = 0u; }
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        L[d * n + e] = lf[d];
-        L[(3 + d) * n + e] = lb[d];
-        M[d * n + e] = mf[d];
-        M[(3 + d) * n + e] = mb[d];
-    }
-    L[6 * n + e] = ic;
-}
-
-template <class T>
-struct WvPlanes {
-    const T* in;          // input (forward: residual, backward: forward result)
-    T* out;               // output
-    const T* L;           // the sweep's three coefficient planes (stride n)
-    const uint32_t* M;    // the sweep's three mask planes (stride n)
-    const T* ic;          // pivots
-    uint32_t* edge;       // tagged edge words
-    long n;               // elements per plane
-};
 
 template <int N>
 __device__ inline void vm_wait() {
@@ -235,282 +119,6 @@ __device__ inline double wv_perm(int a, double x) {
 typedef uint32_t wv_u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t wv_u32x2 __attribute__((ext_vector_type(2)));
 #define WV_SC1 16  // cache policy: device-coherent access
-
-// value of a tagged edge word in LDS
-__device__ inline float wv_edge_val(const lds_u8* p, float*) { return *(const __attribute__((address_space(3))) float*)p; }
-__device__ inline double wv_edge_val(const lds_u8* p, double*) {
-    const uint32_t lo = *(const __attribute__((address_space(3))) uint32_t*)p;
-    const uint32_t hi = *(const __attribute__((address_space(3))) uint32_t*)(p + 8);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
-template <class T, bool FWD>
-__global__ __launch_bounds__(192) void k_sweep_wave(Geo g, int ntj, int ntk, WvPlanes<T> P, const PcgScalars* S,
-                                                    uint32_t tag, int* err) {
-    constexpr int NS = sizeof(T) / 4;           // 16-byte chunks per lane of one plane (4 cells)
-    constexpr int NC = 5 * NS + 3;              // chunks per step: input, 3 L, pivots, 3 masks
-    constexpr int EWB = 8 * NS;                 // bytes per tagged edge word
-    constexpr int ELN = 32 * EWB / 16;          // lanes copying the 32 edge words
-    constexpr int EOFF = NC * 1024;             // edge words within a slot
-    constexpr int SLOT = EOFF + 32 * EWB;       // bytes per ring slot
-    __shared__ __attribute__((aligned(16))) unsigned char ring_[WV_R * SLOT];
-    __shared__ int ctr_[4];  // [0], [1] steps published by the load waves, [2] steps consumed
-    if (S && S->done) return;
-    int J, K;
-    wv_tile(blockIdx.x, ntj, ntk, FWD, &J, &K);
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int SL = wv_slots(g);
-    const long myTile = (long)J * ntk + K;
-    const int uJ = FWD ? J - 1 : J + 1, uK = FWD ? K - 1 : K + 1;
-    const bool hasJ = uJ >= 0 && uJ < ntj, hasK = uK >= 0 && uK < ntk;
-    const long tJ = (long)uJ * ntk + K, tK = (long)J * ntk + uK;
-    lds_u8* ring = (lds_u8*)ring_;
-    lds_int* ctr = (lds_int*)ctr_;
-    if (threadIdx.x < 4) ctr[threadIdx.x] = 0;  // before any wave starts (barrier below)
-    __syncthreads();
-
-    const uint32_t ctrA = (uint32_t)(uintptr_t)ctr, ringA = (uint32_t)(uintptr_t)ring;
-    if (wid > 0) {
-        // ------------------------------- load waves -------------------------------
-        const int w = __builtin_amdgcn_readfirstlane(wid - 1);
-        int cons_seen = 0;
-        auto issue = [&](int s) __attribute__((always_inline)) {
-            if (s >= WV_R) {  // slot s % R is free once step s - R has been consumed
-                int spins = 0;
-                while (cons_seen < s - WV_R + 1) {
-                    cons_seen = lds_ld32(ctrA + 8);
-                    if (cons_seen >= s - WV_R + 1) break;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > WV_SPIN) { atomicOr(err, 4); cons_seen = s; }
-                }
-            }
-            const int slot = FWD ? s : SL - 1 - s;
-            const long e0 = ((myTile * SL + slot) * 64 + lane) * 4;
-            lds_u8* dst = ring + (s % WV_R) * SLOT;
-#pragma unroll
-            for (int ci = 0; ci < NC; ++ci) {
-                if ((ci & 1) != w) continue;
-                const unsigned char* src;
-                if (ci < NS) src = (const unsigned char*)(P.in + e0) + 16 * ci;
-                else if (ci < 4 * NS) src = (const unsigned char*)(P.L + ((ci - NS) / NS) * P.n + e0) + 16 * ((ci - NS) % NS);
-                else if (ci < 5 * NS) src = (const unsigned char*)(P.ic + e0) + 16 * (ci - 4 * NS);
-                else src = (const unsigned char*)(P.M + (ci - 5 * NS) * P.n + e0);
-                __builtin_amdgcn_global_load_lds(src, dst + ci * 1024, 16, 0, 0);
-            }
-            if (w == 0 && lane < ELN) {
-                // edge words of the upstream tiles' step s + 15 (a harmless own-row
-                // address where no word is needed, so the instruction count is fixed)
-                const int us = s + 15;
-                const bool jside = lane < ELN / 2;
-                const bool need = (jside ? hasJ : hasK) && us < SL;
-                const long urow = need ? (jside ? tJ : tK) * SL + us : myTile * SL + s;
-                const unsigned char* src = (const unsigned char*)P.edge + urow * (32 * EWB) + lane * 16;
-                __builtin_amdgcn_global_load_lds(src, dst + EOFF, 16, 0, WV_SC1);
-            }
-        };
-        // edge words the compute wave uses at step x: stale tags are re-read
-        auto validate = [&](int x) __attribute__((always_inline)) {
-            if (lane >= 32) return;
-            const bool jside = lane < 16;
-            const int us = x + 15;
-            if (!((jside ? hasJ : hasK) && us < SL)) return;
-            const uint32_t ea = ringA + (x % WV_R) * SLOT + EOFF + lane * EWB;
-            bool ok = (uint32_t)(lds_ld64(ea) >> 32) == tag;
-            if (NS == 2) ok = ok && (uint32_t)(lds_ld64(ea + 8) >> 32) == tag;
-            if (ok) return;
-            const uint64_t* src = (const uint64_t*)((const unsigned char*)P.edge + ((jside ? tJ : tK) * SL + us) * (32 * EWB) +
-                                                    lane * EWB);
-            int spins = 0;
-            for (;;) {
-                // another tile already gave up: stop waiting (the host redoes the solve)
-                if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                const uint64_t a0 = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t a1 = NS == 2 ? __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a0;
-                if ((uint32_t)(a0 >> 32) == tag && (uint32_t)(a1 >> 32) == tag) {
-                    lds_st64(ea, a0);
-                    if (NS == 2) lds_st64(ea + 8, a1);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > WV_SPIN_EDGE) { atomicOr(err, 2); break; }
-            }
-        };
-        constexpr int PER0 = (NC + 1) / 2 + 1, PER1 = NC / 2;  // VMEM instructions per step
-        int issued = 0;
-        for (int x = 0; x < SL; ++x) {
-            const int lim = x + WV_LEAD < SL ? x + WV_LEAD : SL - 1;
-            while (issued <= lim) { issue(issued); ++issued; }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (issued - 1 - x == WV_LEAD) {
-                if (w == 0) vm_wait<WV_LEAD * PER0>(); else vm_wait<WV_LEAD * PER1>();
-            } else {
-                vm_wait<0>();
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (w == 0) validate(x);
-            lds_st32(ctrA + 4 * w, x + 1);
-        }
-        return;
-    }
-
-    // ------------------------------- compute wave -------------------------------
-    const int jl = lane & 15, gq = lane >> 4;
-    const bool eJ = FWD ? jl == 0 : jl == 15;   // j-neighbour lies in the upstream tile
-    const bool eK = FWD ? gq == 0 : gq == 3;    // cell c = 0 (3) has its k-neighbour there
-    const bool oJ = FWD ? jl == 15 : jl == 0;   // cells read by the j-downstream tile
-    const bool oK = FWD ? gq == 3 : gq == 0;
-    const int bp = ((lane + (FWD ? 48 : 16)) & 63) * 4;  // lane -16 / +16 (edge lanes are overridden)
-    const __amdgpu_buffer_rsrc_t erc = __builtin_amdgcn_make_buffer_rsrc(P.edge, (short)0, 0x7fffffff, 0x00020000);
-    T qp[4] = {0, 0, 0, 0};
-    int seen = 0;
-    for (int s = 0; s < SL; ++s) {
-        if (seen < s + 1) {
-            int spins = 0;
-            for (;;) {
-                const int r0 = lds_ld32(ctrA), r1 = lds_ld32(ctrA + 4);
-                seen = __builtin_amdgcn_readfirstlane(r0 < r1 ? r0 : r1);
-                if (seen >= s + 1) break;
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > 4 * WV_SPIN) { atomicOr(err, 8); seen = SL; break; }
-            }
-        }
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        const lds_u8* sl = ring + (s % WV_R) * SLOT;
-        typedef __attribute__((address_space(3))) const T ldsT;
-        typedef __attribute__((address_space(3))) const uint32_t ldsU;
-        T rr[4], L0[4], L1[4], L2[4], ic[4];
-        uint32_t m0[4], m1[4], m2[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            // cell c of this lane: chunk c / (4 / NS), offset within the 16-byte chunk
-            const int h = c / (4 / NS), o = (c % (4 / NS)) * sizeof(T);
-            const int lo = lane * 16 + o;
-            rr[c] = *(ldsT*)(sl + h * 1024 + lo);
-            L0[c] = *(ldsT*)(sl + (NS + h) * 1024 + lo);
-            L1[c] = *(ldsT*)(sl + (2 * NS + h) * 1024 + lo);
-            L2[c] = *(ldsT*)(sl + (3 * NS + h) * 1024 + lo);
-            ic[c] = *(ldsT*)(sl + (4 * NS + h) * 1024 + lo);
-            m0[c] = *(ldsU*)(sl + (5 * NS) * 1024 + lane * 16 + 4 * c);
-            m1[c] = *(ldsU*)(sl + (5 * NS + 1) * 1024 + lane * 16 + 4 * c);
-            m2[c] = *(ldsU*)(sl + (5 * NS + 2) * 1024 + lane * 16 + 4 * c);
-        }
-        T ej[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ej[c] = wv_edge_val(sl + EOFF + (4 * gq + c) * EWB, (T*)nullptr);
-        const T ek = wv_edge_val(sl + EOFF + (16 + jl) * EWB, (T*)nullptr);
-        // neighbours' results of step s-1
-        T qj[4], qk[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const T x = wv_shift<FWD>(qp[c]);
-            qj[c] = eJ ? ej[c] : x;
-        }
-        if (FWD) {
-            const T x = wv_perm(bp, qp[3]);
-            qk[0] = eK ? ek : x;
-            qk[1] = qp[0]; qk[2] = qp[1]; qk[3] = qp[2];
-        } else {
-            const T x = wv_perm(bp, qp[0]);
-            qk[3] = eK ? ek : x;
-            qk[0] = qp[1]; qk[1] = qp[2]; qk[2] = qp[3];
-        }
-        T q[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            T v = rr[c], w;
-            if (FWD) {
-                // column order c-nn, c-ni, c-1
-                w = v - L0[c] * qk[c]; v = wv_sel<T>(m0[c], v, w);
-                w = v - L1[c] * qj[c]; v = wv_sel<T>(m1[c], v, w);
-                w = v - L2[c] * qp[c]; v = wv_sel<T>(m2[c], v, w);
-            } else {
-                // row order c+1, c+ni, c+nn
-                w = v - L0[c] * qp[c]; v = wv_sel<T>(m0[c], v, w);
-                w = v - L1[c] * qj[c]; v = wv_sel<T>(m1[c], v, w);
-                w = v - L2[c] * qk[c]; v = wv_sel<T>(m2[c], v, w);
-            }
-            q[c] = v * ic[c];
-        }
-        // outputs: the quad row, and the tile's edge words of this step
-        const int slot = FWD ? s : SL - 1 - s;
-        T* op = P.out + ((myTile * SL + slot) * 64 + lane) * 4;
-        if constexpr (NS == 1) {
-            *(HIP_vector_type<T, 4>*)op = HIP_vector_type<T, 4>(q[0], q[1], q[2], q[3]);
-        } else {
-            *(HIP_vector_type<T, 2>*)op = HIP_vector_type<T, 2>(q[0], q[1]);
-            *(HIP_vector_type<T, 2>*)(op + 2) = HIP_vector_type<T, 2>(q[2], q[3]);
-        }
-        const int row = (int)((myTile * SL + s) * (32 * EWB));
-        if (oJ) {
-            if constexpr (NS == 1) {
-                const wv_u32x4 a = {__float_as_uint((float)q[0]), tag, __float_as_uint((float)q[1]), tag};
-                const wv_u32x4 b = {__float_as_uint((float)q[2]), tag, __float_as_uint((float)q[3]), tag};
-                __builtin_amdgcn_raw_buffer_store_b128(a, erc, row + 4 * gq * EWB, 0, WV_SC1);
-                __builtin_amdgcn_raw_buffer_store_b128(b, erc, row + 4 * gq * EWB + 16, 0, WV_SC1);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const uint64_t u = (uint64_t)__double_as_longlong((double)q[c]);
-                    const wv_u32x4 a = {(uint32_t)u, tag, (uint32_t)(u >> 32), tag};
-                    __builtin_amdgcn_raw_buffer_store_b128(a, erc, row + (4 * gq + c) * EWB, 0, WV_SC1);
-                }
-            }
-        }
-        if (oK) {
-            const T v = FWD ? q[3] : q[0];
-            if constexpr (NS == 1) {
-                const wv_u32x2 a = {__float_as_uint((float)v), tag};
-                __builtin_amdgcn_raw_buffer_store_b64(a, erc, row + (16 + jl) * EWB, 0, WV_SC1);
-            } else {
-                const uint64_t u = (uint64_t)__double_as_longlong((double)v);
-                const wv_u32x4 a = {(uint32_t)u, tag, (uint32_t)(u >> 32), tag};
-                __builtin_amdgcn_raw_buffer_store_b128(a, erc, row + (16 + jl) * EWB, 0, WV_SC1);
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) qp[c] = q[c];
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        lds_st32_nowait(ctrA + 8, s + 1);
-    }
-}
-
-static inline uint32_t wv_tag(unsigned long long e) { return (uint32_t)(e & 0x7fffffffull) | 0x80000000u; }
-
-template <class T>
-void sweep_prepare_wave(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st) {
-    int ntj, ntk;
-    wv_tiles(g, &ntj, &ntk);
-    const long n = sweep_wave_size(g);
-    k_wv_records<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, inv, sk.wL, sk.wm);
-}
-
-template <class T>
-bool sweep_precond_wave(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
-                        unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st) {
-    int ntj, ntk;
-    wv_tiles(g, &ntj, &ntk);
-    const long n = sweep_wave_size(g);
-    const long ebytes = (long)ntj * ntk * wv_slots(g) * 32 * 8 * (long)(sizeof(T) / 4);
-    if (!flags || !epoch || !sk.wL || ebytes >= 0x7fffffffL) return false;
-    k_to_quad<T, T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, r, sk.wr, S);
-    const uint32_t tag1 = wv_tag((*epoch)++), tag2 = wv_tag((*epoch)++);
-    const WvPlanes<T> f{sk.wr, sk.wq, sk.wL, sk.wm, sk.wL + 6 * n, sk.wedge, n};
-    const WvPlanes<T> b{sk.wq, sk.wz, sk.wL + 3 * n, sk.wm + 3 * n, sk.wL + 6 * n, sk.wedge, n};
-    const int tiles = ntj * ntk;
-    if (t0) hipEventRecord(t0, st);
-    k_sweep_wave<T, true><<<tiles, 192, 0, st>>>(g, ntj, ntk, f, S, tag1, flags);
-    k_sweep_wave<T, false><<<tiles, 192, 0, st>>>(g, ntj, ntk, b, S, tag2, flags);
-    if (t1) hipEventRecord(t1, st);
-    k_from_quad<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.wz, z, S);
-    return true;
-}
-
-template void sweep_prepare_wave<double>(const Geo&, const uint16_t*, const double*, const SweepSkew<double>&, hipStream_t);
-template void sweep_prepare_wave<float>(const Geo&, const uint16_t*, const float*, const SweepSkew<float>&, hipStream_t);
-template bool sweep_precond_wave<double>(const Geo&, const SweepSkew<double>&, const double*, double*, const PcgScalars*,
-                                         int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t);
-template bool sweep_precond_wave<float>(const Geo&, const SweepSkew<float>&, const float*, float*, const PcgScalars*,
-                                        int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // 8x8 tiles.  A sweep step moves every input of a tile through one CU, and one
