@@ -124,7 +124,9 @@ typedef struct xf_solve_stats {
                                  the MIC(0) sweep launches of every 5th
                                  preconditioner application, ms                    */
     int    sweeps;            /* number of sweep launches summed in sweep_ms       */
-    int    reserved2;
+    int    sweep_fused;       /* pipelined sweeps: 1 forward + backward in one launch
+                                 (all tiles resident at once), 0 two launches;
+                                 -1 per-hyperplane sweeps                           */
     int64_t box_cells;        /* cells in the bounding box of the active rows: the
                                  cells the PCG iteration kernels and the pipelined
                                  sweeps visit (the other rows are empty)           */

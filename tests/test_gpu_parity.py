@@ -175,6 +175,10 @@ def test_operators_bitexact(xf, case):
     z = np.empty(N)
     O.xo_sweep_fwd(ni, nj, nk, A7, inv, x, q)
     O.xo_sweep_bwd(ni, nj, nk, A7, inv, q, z)
+    _exact(g.apply_precond(x), z, "precond (8x8-tile pipeline)")
+    p = g.params
+    p.sweep_kernel = 1
+    g.set_params(p)
     _exact(g.apply_precond(x), z, "precond (per-plane sweeps)")
 
 

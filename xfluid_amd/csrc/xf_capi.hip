@@ -310,6 +310,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
         for (int i = 0; i < tm.cap; ++i) HIPCHK(hipEventCreate(&tm.ev[i]));
     }
     int sweep = p.sweep_kernel == 0 ? 9 : p.sweep_kernel;
+    s->pw.sk64.fused = s->pw.sk32.fused = -1;
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
     if (sweep >= 2) {
@@ -338,6 +339,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
     s->last.residual = r.residual;
     s->last.rhs_norm = r.rhs_norm;
     s->last.box_cells = r.box_cells;
+    s->last.sweep_fused = std::max(s->pw.sk64.fused, s->pw.sk32.fused);
     s->last.ms = ms;
     s->last.sweep_ms = 0.0;
     s->last.sweeps = 0;
@@ -1214,7 +1216,8 @@ int xf_sim_apply_precond(xf_sim* s, const double* r, double* z) {
     if (check(s) || !r || !z) return set_err(-1, "null argument");
     const long N = s->g.N;
     HIPCHK(hipMemcpyAsync(s->pw.s, r, N * 8, hipMemcpyHostToDevice, s->st));
-    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, s->prm.sweep_kernel, s->pw, s->st);
+    const int sweep = s->prm.sweep_kernel == 0 ? 9 : s->prm.sweep_kernel;  // as in the solve
+    pcg_precond(s->g, s->code, s->pw.inv, s->pw.s, s->pw.q, s->pw.z, sweep, s->pw, s->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(z, s->pw.z, N * 8, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));

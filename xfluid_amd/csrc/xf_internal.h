@@ -49,6 +49,7 @@ struct SweepSkew {
     T *oR, *oQ, *oZ;
     uint32_t* oEdge;
     mutable uint32_t w8_launch;  // launches on oEdge (their tags)
+    mutable int fused;           // last application: 1 one fused forward + backward launch, 0 two launches
     // bounding box of the active rows (i0, i1, j0, j1, k0, k1; host copy), see pcg_active_box
     int box[6];
 };

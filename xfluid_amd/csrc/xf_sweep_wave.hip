@@ -882,7 +882,8 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     const int tiles = ntjb * ntkb;
     // t0 / t1 ride on the dispatch packets themselves, so they time the kernels and not
     // the marker packets (and their cache release) that hipEventRecord would add
-    if (w8_fused_fits<T>(tiles)) {
+    sk.fused = w8_fused_fits<T>(tiles) ? 1 : 0;
+    if (sk.fused) {
         const double* fp = fin ? fin->partial : nullptr;
         hipExtLaunchKernelGGL(k_sweep_w8_fb<T>, dim3(tiles), dim3(192), 0, st, t0, t1, 0, g, ntk, J0, K0, ntjb,
                               ntkb, s0, nS, f, b, S, tag1, tag2, flags, fp, fin ? fin->n : 0, fin ? fin->it : 0,
