@@ -6,11 +6,10 @@
   (90,720 particles) and 128^3 (3,175,200 particles).
 * The 64x32x32 dam break (dx 0.125, CFL 5, FLIP 0.95, FP64 PCG to 1e-6, blob
   radius 1.0 dx, iso 0.5): 30 x frameStep(1/30) (FluidSimulator.cpp:224-262)
-  on the GPU and the oracle side by side.  After 3 frames: positions within
-  1e-8 relative, material and particle count exact; after 30 frames (the
-  trajectories of a splashing liquid separate slowly): count exact, fluid cells
-  within 1 %, centre of mass within 1e-3; the surface cells, cube cases and the
-  mesh of the GPU's final density field equal the oracle's on that same field.
+  on the GPU and the oracle side by side: positions within 1e-10 relative
+  after every frame (observed: 1.6e-15 after frame 30), particle count and
+  final material exact; the surface cells, cube cases and the mesh of the
+  GPU's final density field equal the oracle's on that same field.
 * The 128^3 dam break (3,175,200 particles, dt 1/240): one substep stage by
   stage on identical inputs (material, forces, RHS, MIC(0) pivots, apply,
   extrapolation, advection bit-exact, one 8x8-tile preconditioner application
@@ -136,13 +135,9 @@ def test_c1_dam64x32x32_frames(xf, tmp_path):
         err = _rel(Pg, Po)
         if f in (0, 2, 9, 19, 29):
             say(f"[C1] frame {f + 1}: positions rel err {err:.3e}, PCG iterations {g.last_solve().iterations}")
-        if f == 2:
-            assert err <= 1e-8, f"positions after 3 frames: {err:.3e}"
-            _exact(g.get_field(FIELD["MATERIAL"]), o.get(FIELD["MATERIAL"]), "C1 material after 3 frames")
+        assert err <= 1e-10, f"positions after {f + 1} frames: {err:.3e}"
     assert g.num_particles() == n0 == len(Po)
-    mg, mo = g.get_field(FIELD["MATERIAL"]), o.get(FIELD["MATERIAL"])
-    assert abs(int((mg == 1).sum()) - int((mo == 1).sum())) <= 0.01 * (mo == 1).sum()
-    assert np.abs(Pg.mean(0) - Po.mean(0)).max() <= 1e-3 * np.abs(Po.mean(0)).max()
+    _exact(g.get_field(FIELD["MATERIAL"]), o.get(FIELD["MATERIAL"]), "C1 material after 30 frames")
     assert (tmp_path / "frame_29.ply").exists()
     # classification and mesh on the same (the GPU's) density field
     phi = g.get_field(FIELD["PHI"])
