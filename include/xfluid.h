@@ -197,6 +197,13 @@ int  xf_sim_set_particles(xf_sim* s, int64_t n, const double* px, const double* 
                           const double* vx, const double* vy, const double* vz);
 int  xf_sim_get_particles(xf_sim* s, double* px, double* py, double* pz,
                           double* vx, double* vy, double* vz);
+/* Device-memory variants (no host copy): positions and velocities as n x 3
+ * interleaved arrays in particle-id order (the order they were set in; the
+ * library keeps its own copies sorted by cell); fields as in xf_sim_set_field. */
+int  xf_sim_set_particles_device(xf_sim* s, int64_t n, const double* pos, const double* vel);
+int  xf_sim_get_particles_device(xf_sim* s, double* pos, double* vel);
+int  xf_sim_set_field_device(xf_sim* s, int which, const void* dev, size_t bytes);
+int  xf_sim_get_field_device(xf_sim* s, int which, void* dev, size_t bytes);
 size_t xf_sim_field_bytes(const xf_sim* s, int which);
 int  xf_sim_set_field(xf_sim* s, int which, const void* host, size_t bytes);
 int  xf_sim_get_field(xf_sim* s, int which, void* host, size_t bytes);
@@ -241,6 +248,12 @@ int  xf_sim_project_fields(xf_sim* s, double* u, double* v, double* w, int devic
  * place (A symmetric: the transposed solve is a PCG solve with A). */
 int  xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs);
 
+/* VJP of the velocity extrapolation (_expolateVelocity, FluidSimulator.cpp:
+ * 511-650) on the sim's material, in place on the three face arrays: layer L
+ * faces (the mean of their layer L-1 neighbours) hand g/count back to those
+ * neighbours, last layer first; faces of layers >= 1 then drop theirs. */
+int  xf_sim_extrapolate_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs);
+
 /* Particle transfers, particles as n x 3 arrays (x,y,z interleaved), on the
  * sim's current grid fields; grid gradients gu/gv/gw are ADDED to.
  * VJP of G2P sampling v_p = S(u; p_p) (_getVelocity, FluidSimulator.cpp:
@@ -248,9 +261,11 @@ int  xf_sim_project_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int d
  * velocities (the FLIP "old" field). */
 int  xf_sim_sample_adjoint(xf_sim* s, int64_t n, const double* pos, const double* gvel, double* gpos,
                            double* gu, double* gv, double* gw, int saved, int device_ptrs);
-/* VJP of the RK4/RK2 position update (FluidSimulator.cpp:896-907) without the
- * collision branch: gnew (gradient on the new positions) -> gpos (written),
- * gu/gv/gw (added). */
+/* VJP of the advection stage's position update: the RK4/RK2 target
+ * (FluidSimulator.cpp:896-907) and the collision branch on the sim's material
+ * (FluidSimulator.cpp:686-722: target kept, old position kept, or pulled back
+ * from the first solid voxel): gnew (gradient on the new positions) -> gpos
+ * (written), gu/gv/gw (added). */
 int  xf_sim_advect_adjoint(xf_sim* s, int64_t n, const double* pos, double dt, const double* gnew,
                            double* gpos, double* gu, double* gv, double* gw, int device_ptrs);
 /* VJP of P2G (FluidSimulator.cpp:909-1085) for the given particles: gu/gv/gw

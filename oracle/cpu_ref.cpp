@@ -800,6 +800,47 @@ struct Sim {
             cur.swap(nxt);
         }
     }
+    // VJP of extrapolate_axis for gradient gf (in place), fixed material.  Layer
+    // L = the faces first reached at step L (each the mean of its layer L-1
+    // neighbours).  Last layer first, a face of layer L-1 gathers g(t)/count(t)
+    // from its layer-L neighbours t (+i,-i,+j,-j,+k,-k); then the faces of
+    // layers >= 1 drop their gradient (their values were overwritten).
+    void extrapolate_adjoint_axis(int layers, int axis, double* gf) const {
+        const int di = axis == 0, dj = axis == 1, dk = axis == 2;
+        const int si = ni + di, sj = nj + dj, sk = nk + dk;
+        const long nf = (long)si * sj * sk;
+        auto F = [&](int i, int j, int k) { return i + (long)si * (j + (long)sj * k); };
+        auto in = [&](int i, int j, int k) { return i >= 0 && j >= 0 && k >= 0 && i < si && j < sj && k < sk; };
+        std::vector<int> mark(nf, -1);
+        for (int i = 0; i < ni; ++i) for (int j = 0; j < nj; ++j) for (int k = 0; k < nk; ++k)
+            if (mat[i + (long)ni * (j + (long)nj * k)] == FLUID) { mark[F(i, j, k)] = 0; mark[F(i + di, j + dj, k + dk)] = 0; }
+        const int ndi[6] = {1, -1, 0, 0, 0, 0}, ndj[6] = {0, 0, 1, -1, 0, 0}, ndk[6] = {0, 0, 0, 0, 1, -1};
+        auto count = [&](int i, int j, int k, int want) {
+            int c = 0;
+            for (int e = 0; e < 6; ++e)
+                if (in(i + ndi[e], j + ndj[e], k + ndk[e]) && mark[F(i + ndi[e], j + ndj[e], k + ndk[e])] == want) ++c;
+            return c;
+        };
+        for (int L = 1; L <= layers; ++L) {
+            std::vector<long> reached;
+            for (int k = 0; k < sk; ++k) for (int j = 0; j < sj; ++j) for (int i = 0; i < si; ++i)
+                if (mark[F(i, j, k)] == -1 && count(i, j, k, L - 1) > 0) reached.push_back(F(i, j, k));
+            for (long t : reached) mark[t] = L;
+        }
+        for (int L = layers; L >= 1; --L)
+            for (int k = 0; k < sk; ++k) for (int j = 0; j < sj; ++j) for (int i = 0; i < si; ++i) {
+                if (mark[F(i, j, k)] != L - 1) continue;
+                double acc = 0.0;
+                for (int e = 0; e < 6; ++e) {
+                    const int a = i + ndi[e], b = j + ndj[e], c = k + ndk[e];
+                    if (!in(a, b, c) || mark[F(a, b, c)] != L) continue;
+                    acc += gf[F(a, b, c)] / (double)count(a, b, c, L - 1);
+                }
+                gf[F(i, j, k)] += acc;
+            }
+        for (long t = 0; t < nf; ++t)
+            if (mark[t] >= 1) gf[t] = 0.0;
+    }
     void extrapolate() {
         int layers = (int)(prm.cfl + 5);
         extrapolate_axis(layers, 0); extrapolate_axis(layers, 1); extrapolate_axis(layers, 2);
@@ -909,8 +950,10 @@ struct Sim {
         return false;
     }
 
-    // collision.cpp:396-439, with the reference's repeated dir.x guard
-    static bool ray_box(const double* p0, const double* dir_in, const double* bmin, const double* bmax, double* hit) {
+    // collision.cpp:396-439, with the reference's repeated dir.x guard; also
+    // the axis of the face the ray enters through and whether it is the upper one
+    static bool ray_box(const double* p0, const double* dir_in, const double* bmin, const double* bmax, double* hit,
+                        int* axis = nullptr, bool* upper = nullptr) {
         double dir[3] = {dir_in[0], dir_in[1], dir_in[2]};
         double eps = 1e-10;
         if (std::fabs(dir[0]) < eps) dir[0] = dir[0] < 0 ? -eps : eps;
@@ -920,14 +963,59 @@ struct Sim {
         double t1 = (bmin[0] - p0[0]) * inv[0], t2 = (bmax[0] - p0[0]) * inv[0];
         double tmin = std::fmin(t1, t2), tmax = std::fmax(t1, t2);
         t1 = (bmin[1] - p0[1]) * inv[1]; t2 = (bmax[1] - p0[1]) * inv[1];
-        tmin = std::fmax(tmin, std::fmin(t1, t2)); tmax = std::fmin(tmax, std::fmax(t1, t2));
+        const double m1 = std::fmin(t1, t2);
+        tmin = std::fmax(tmin, m1); tmax = std::fmin(tmax, std::fmax(t1, t2));
         t1 = (bmin[2] - p0[2]) * inv[2]; t2 = (bmax[2] - p0[2]) * inv[2];
-        tmin = std::fmax(tmin, std::fmin(t1, t2)); tmax = std::fmin(tmax, std::fmax(t1, t2));
+        const double m2 = std::fmin(t1, t2);
+        tmin = std::fmax(tmin, m2); tmax = std::fmin(tmax, std::fmax(t1, t2));
         if (tmax > std::fmax(tmin, 0.0)) {
             for (int a = 0; a < 3; ++a) hit[a] = p0[a] + dir[a] * tmin;
+            if (axis) {
+                *axis = tmin == m2 ? 2 : tmin == m1 ? 1 : 0;
+                *upper = inv[*axis] < 0.0;
+            }
             return true;
         }
         return false;
+    }
+
+    // the end of a particle's move (FluidSimulator.cpp:686-722): 0 the
+    // integrated target, 1 the old position kept, 2 pulled back from the first
+    // solid voxel on the way; for 2, `c` holds the ray, the entry face's axis
+    // and its distance (what the adjoint differentiates)
+    struct Coll {
+        double rn[3], l, tmin, back;
+        int axis;
+    };
+    int resolve_move(const double* pos, const double* np_, double* out, Coll* c) const {
+        int gi = pos2idx(np_[0], dx), gj = pos2idx(np_[1], dx), gk = pos2idx(np_[2], dx);
+        if (mat_in(gi, gj, gk) && mat[gi + (long)ni * (gj + (long)nj * gk)] != SOLID) {
+            out[0] = np_[0]; out[1] = np_[1]; out[2] = np_[2];
+            return 0;
+        }
+        int vox[3];
+        if (!voxel_hit(pos, np_, vox)) return 1;
+        double dv[3] = {np_[0] - pos[0], np_[1] - pos[1], np_[2] - pos[2]};
+        double l = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        double rn[3] = {dv[0] / l, dv[1] / l, dv[2] / l};
+        double bmin[3] = {idx2pos(vox[0], dx), idx2pos(vox[1], dx), idx2pos(vox[2], dx)};
+        double bmax[3] = {bmin[0] + dx, bmin[1] + dx, bmin[2] + dx};
+        double cp[3];
+        int axis = 0;
+        bool upper = false;
+        if (!ray_box(pos, rn, bmin, bmax, cp, &axis, &upper)) return 1;
+        double back = (double)(float)(0.05 * dx);
+        double res[3] = {cp[0] - rn[0] * back, cp[1] - rn[1] * back, cp[2] - rn[2] * back};
+        int ri = pos2idx(res[0], dx), rj = pos2idx(res[1], dx), rk = pos2idx(res[2], dx);
+        // the reference indexes material without a range check; out of range is rejected here
+        if (!(mat_in(ri, rj, rk) && mat[ri + (long)ni * (rj + (long)nj * rk)] != SOLID)) return 1;
+        out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
+        for (int a = 0; a < 3; ++a) c->rn[a] = rn[a];
+        c->l = l;
+        c->axis = axis;
+        c->tmin = ((upper ? bmax[axis] : bmin[axis]) - pos[axis]) / rn[axis];
+        c->back = back;
+        return 2;
     }
 
     // FluidSimulator.cpp:652-723 (G2P blend, RK4 advection, collision)
@@ -996,30 +1084,11 @@ struct Sim {
             }
             vx[p] = vp[0]; vy[p] = vp[1]; vz[p] = vp[2];
 
-            double np_[3];
+            double np_[3], out[3];
             advect_point(pos, dt, np_);
-            int gi = pos2idx(np_[0], dx), gj = pos2idx(np_[1], dx), gk = pos2idx(np_[2], dx);
-            if (mat_in(gi, gj, gk) && mat[gi + (long)ni * (gj + (long)nj * gk)] != SOLID) {
-                px[p] = np_[0]; py[p] = np_[1]; pz[p] = np_[2];
-            } else {
-                int vox[3];
-                if (voxel_hit(pos, np_, vox)) {
-                    double dv[3] = {np_[0] - pos[0], np_[1] - pos[1], np_[2] - pos[2]};
-                    double l = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-                    double rn[3] = {dv[0] / l, dv[1] / l, dv[2] / l};
-                    double bmin[3] = {idx2pos(vox[0], dx), idx2pos(vox[1], dx), idx2pos(vox[2], dx)};
-                    double bmax[3] = {bmin[0] + dx, bmin[1] + dx, bmin[2] + dx};
-                    double cp[3];
-                    if (ray_box(pos, rn, bmin, bmax, cp)) {
-                        double back = (double)(float)(0.05 * dx);
-                        double res[3] = {cp[0] - rn[0] * back, cp[1] - rn[1] * back, cp[2] - rn[2] * back};
-                        int ri = pos2idx(res[0], dx), rj = pos2idx(res[1], dx), rk = pos2idx(res[2], dx);
-                        // the reference indexes material without a range check; out of range is rejected here
-                        if (mat_in(ri, rj, rk) && mat[ri + (long)ni * (rj + (long)nj * rk)] != SOLID) {
-                            px[p] = res[0]; py[p] = res[1]; pz[p] = res[2];
-                        }
-                    }
-                }
+            Coll c;
+            if (resolve_move(pos, np_, out, &c) != 1) {
+                px[p] = out[0]; py[p] = out[1]; pz[p] = out[2];
             }
         }
     }
@@ -1220,6 +1289,44 @@ struct Sim {
         }
     }
     // VJP of advect_point (RK4 / RK2, collision-free): gnew -> gp (overwritten), gF accumulated
+    // VJP of the whole position update p -> resolve_move(p, RK(p)) for gradient
+    // gout on the new position: the gradient reaching the integrated target
+    // (grk) and the part that goes to p directly (gextra).  Kept: all to p.
+    // Pulled back (X' = p + rn (t - back), t = (B - p_a) / rn_a, rn = d / |d|,
+    // d = target - p):  g_rn = (t - back) g - e_a (g.rn) t / rn_a,
+    // g_d = (I - rn rn^T) g_rn / |d|,  grk = g_d,  gextra = g - e_a (g.rn) / rn_a - g_d.
+    void move_adjoint(const double* p, const double* np_, const double* gout, double* grk, double* gextra) const {
+        double out[3];
+        Coll c;
+        const int kind = resolve_move(p, np_, out, &c);
+        for (int a = 0; a < 3; ++a) { grk[a] = kind == 0 ? gout[a] : 0.0; gextra[a] = kind == 1 ? gout[a] : 0.0; }
+        if (kind != 2) return;
+        const int ax = c.axis;
+        const double s = gout[0] * c.rn[0] + gout[1] * c.rn[1] + gout[2] * c.rn[2];
+        const double cc = c.tmin - c.back;
+        double grn[3];
+        for (int k = 0; k < 3; ++k) grn[k] = cc * gout[k] - (k == ax ? s * c.tmin / c.rn[ax] : 0.0);
+        const double rg = c.rn[0] * grn[0] + c.rn[1] * grn[1] + c.rn[2] * grn[2];
+        for (int k = 0; k < 3; ++k) {
+            grk[k] = (grn[k] - c.rn[k] * rg) / c.l;
+            gextra[k] = gout[k] - (k == ax ? s / c.rn[ax] : 0.0) - grk[k];
+        }
+    }
+    // the integrated target of advect_point followed by resolve_move
+    void move_point(const double* p, double dt, double* out) const {
+        double np_[3];
+        advect_point(p, dt, np_);
+        Coll c;
+        if (resolve_move(p, np_, out, &c) == 1) { out[0] = p[0]; out[1] = p[1]; out[2] = p[2]; }
+    }
+    // VJP of move_point: gnew on the new position -> gp (written), gF (added)
+    void move_adjoint_full(const double* p, double dt, const double* gnew, double* const gF[3], double* gp) const {
+        double np_[3], grk[3], gextra[3];
+        advect_point(p, dt, np_);
+        move_adjoint(p, np_, gnew, grk, gextra);
+        advect_adjoint(p, dt, grk, gF, gp);
+        for (int a = 0; a < 3; ++a) gp[a] += gextra[a];
+    }
     void advect_adjoint(const double* p, double dt, const double* gnew, double* const gF[3], double* gp) const {
         const double* F[3] = {u.data(), v.data(), w.data()};
         double k1[3], k2[3], k3[3], q2[3], q3[3], q4[3], g[3];
@@ -1482,11 +1589,26 @@ void xo_advect_points(void* h, int64_t n, const double* pos, double dt, double* 
     Sim* s = (Sim*)h;
     for (int64_t p = 0; p < n; ++p) s->advect_point(pos + 3 * p, dt, out + 3 * p);
 }
+// VJP of the position update with its collision branch (move_points): gnew on
+// the new positions -> gpos (written), gu/gv/gw (added)
 void xo_advect_adjoint(void* h, int64_t n, const double* pos, double dt, const double* gnew, double* gpos,
                        double* gu, double* gv, double* gw) {
     Sim* s = (Sim*)h;
     double* gF[3] = {gu, gv, gw};
-    for (int64_t p = 0; p < n; ++p) s->advect_adjoint(pos + 3 * p, dt, gnew + 3 * p, gF, gpos + 3 * p);
+    for (int64_t p = 0; p < n; ++p) s->move_adjoint_full(pos + 3 * p, dt, gnew + 3 * p, gF, gpos + 3 * p);
+}
+// VJP of the extrapolation stage on the handle's material (in place)
+void xo_extrapolate_adjoint(void* h, double* gu, double* gv, double* gw) {
+    Sim* s = (Sim*)h;
+    const int layers = (int)(s->prm.cfl + 5);
+    double* g[3] = {gu, gv, gw};
+    for (int a = 0; a < 3; ++a) s->extrapolate_adjoint_axis(layers, a, g[a]);
+}
+// the whole position update of the advection stage (RK4 / RK2 target, then the
+// collision branch) on the handle's fields and material
+void xo_move_points(void* h, int64_t n, const double* pos, double dt, double* out) {
+    Sim* s = (Sim*)h;
+    for (int64_t p = 0; p < n; ++p) s->move_point(pos + 3 * p, dt, out + 3 * p);
 }
 void xo_p2g_points(void* h, int64_t n, const double* pos, const double* vel, double* u, double* v, double* w) {
     Sim* s = (Sim*)h;

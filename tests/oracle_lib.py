@@ -95,6 +95,8 @@ def _load_oracle():
     lib.xo_sample_points.argtypes = [P, C.c_int64, _dp, _dp, C.c_int]
     lib.xo_sample_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
     lib.xo_advect_points.argtypes = [P, C.c_int64, _dp, C.c_double, _dp]
+    lib.xo_move_points.argtypes = [P, C.c_int64, _dp, C.c_double, _dp]
+    lib.xo_extrapolate_adjoint.argtypes = [P, _dp, _dp, _dp]
     lib.xo_advect_adjoint.argtypes = [P, C.c_int64, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]
     lib.xo_p2g_points.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp]
     lib.xo_p2g_adjoint.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp]

@@ -168,13 +168,69 @@ def test_advect_adjoint(integrator):
 
     def loss_pos(pos):
         out = np.empty((n, 3))
-        L.xo_advect_points(o.h, n, np.ascontiguousarray(pos), dt, out)
+        L.xo_move_points(o.h, n, np.ascontiguousarray(pos), dt, out)
         return float(np.sum(c * out))
 
     F = faces_of(o)
     gpos = np.zeros((n, 3))
     gF = [np.zeros_like(a) for a in F]
     L.xo_advect_adjoint(o.h, n, P, dt, c, gpos, *gF)
+    _fd_check(loss_pos, P, gpos, rng, 1e-8)
+
+    def loss_f(x):
+        fields = np.split(x, np.cumsum([a.size for a in F])[:-1])
+        for name, a, ref in zip(("U", "V", "W"), fields, F):
+            o.set(FIELD[name], a.reshape(ref.shape))
+        return loss_pos(P)
+
+    x0 = np.concatenate([a.ravel() for a in F])
+    g0 = np.concatenate([a.ravel() for a in gF])
+    _fd_check(loss_f, x0, g0, rng, 1e-7)
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_advect_adjoint_collisions(integrator):
+    """The collision branch of the move (FluidSimulator.cpp:686-722): fast
+    particles driven into the walls are pulled back from the first solid voxel
+    (X' = p + rn (t - back)) or keep their position; the VJP of the whole move
+    against central differences in positions and grid values."""
+    o, P, _ = particle_scene(integrator=integrator)
+    L = oracle()
+    n = len(P)
+    dt = 0.05
+    F = faces_of(o)
+    push = (-1.5, -2.0, 1.0)  # towards the -x, -y and +z walls
+    for name, a, extra in zip(("U", "V", "W"), F, push):
+        o.set(FIELD[name], 8.0 * a + extra)
+    F = faces_of(o)
+    target, moved = np.empty((n, 3)), np.empty((n, 3))
+    L.xo_advect_points(o.h, n, P, dt, target)
+    L.xo_move_points(o.h, n, P, dt, moved)
+    pulled = np.any(moved != target, 1) & np.any(moved != P, 1)
+    kept = np.all(moved == P, 1)
+    assert pulled.sum() > 50 and kept.sum() >= 0
+    rng = np.random.default_rng(24)
+    c = rng.standard_normal((n, 3))
+
+    def loss_pos(pos):
+        out = np.empty((n, 3))
+        L.xo_move_points(o.h, n, np.ascontiguousarray(pos), dt, out)
+        return float(np.sum(c * out))
+
+    gpos = np.zeros((n, 3))
+    gF = [np.zeros_like(a) for a in F]
+    L.xo_advect_adjoint(o.h, n, P, dt, c, gpos, *gF)
+    # the pulled-back particles' own gradients (a gather of the whole set would hide them)
+    cp = np.where(pulled[:, None], c, 0.0)
+    gp2 = np.zeros((n, 3))
+    L.xo_advect_adjoint(o.h, n, P, dt, cp, gp2, *[np.zeros_like(a) for a in F])
+
+    def loss_pulled(pos):
+        out = np.empty((n, 3))
+        L.xo_move_points(o.h, n, np.ascontiguousarray(pos), dt, out)
+        return float(np.sum(cp * out))
+
+    _fd_check(loss_pulled, P, gp2, rng, 1e-8)
     _fd_check(loss_pos, P, gpos, rng, 1e-8)
 
     def loss_f(x):
@@ -241,3 +297,48 @@ def test_density_adjoint():
     g12 = np.zeros((n, 3))
     L.xo_density_adjoint(o.h, n, P, np.ascontiguousarray(c + c2), g12)
     assert np.allclose(g12, gpos + g2, rtol=1e-12, atol=1e-12)
+
+
+def _extrapolated(o, fields):
+    for name, a in zip(("U", "V", "W"), fields):
+        o.set(FIELD[name], a)
+    o.stage(6)  # EXTRAPOLATE
+    return faces_of(o)
+
+
+@pytest.mark.parametrize("cfl", [1.0, 5.0])
+def test_extrapolation_adjoint_dot_identity(cfl):
+    """The extrapolation is linear for a fixed material: <E x, y> = <x, E^T y>."""
+    o, _, _ = particle_scene()
+    p = o.params
+    p.cfl = cfl
+    o.set_params(p)
+    o.stage(1)  # material from the particles
+    rng = np.random.default_rng(31)
+    F = faces_of(o)
+    for _ in range(3):
+        x = [rng.standard_normal(a.shape) for a in F]
+        y = [rng.standard_normal(a.shape) for a in F]
+        Ex = _extrapolated(o, [a.copy() for a in x])
+        ETy = [np.ascontiguousarray(a.copy()) for a in y]
+        oracle().xo_extrapolate_adjoint(o.h, *ETy)
+        lhs, rhs = dot(Ex, y), dot(x, ETy)
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0), (lhs, rhs)
+
+
+def test_extrapolation_adjoint_single_cell():
+    """One fluid cell: a layer-1 face's gradient goes to its layer-0 neighbours,
+    split by their count; faces beyond the layers keep theirs, faces of layers >= 1 lose theirs."""
+    o = oracle_dam_break(7, 7, 7, 0.1, (0.0, 0.0, 0.0), 0, small_params(cfl=1.0))  # 6 layers
+    mat = o.get(FIELD["MATERIAL"])
+    mat[3, 3, 3] = 1
+    o.set(FIELD["MATERIAL"], mat)
+    shapes = [a.shape for a in faces_of(o)]
+    gu = np.zeros(shapes[0])
+    gu[3, 3, 2] = 1.0   # layer 1: its only layer-0 neighbour is u[3, 3, 3] (the cell's -x face)
+    gu[3, 4, 3] = 2.0   # layer 1: neighbours u[3, 3, 3] (-j)
+    gv, gw = np.zeros(shapes[1]), np.zeros(shapes[2])
+    oracle().xo_extrapolate_adjoint(o.h, gu, gv, gw)
+    assert gu[3, 3, 2] == 0.0 and gu[3, 4, 3] == 0.0
+    assert gu[3, 3, 3] == 3.0 and gu[3, 3, 4] == 0.0
+    assert not gv.any() and not gw.any()

@@ -67,7 +67,7 @@ def test_projection_adjoint_matches_oracle(setup):
 def test_torch_autograd_and_checkpointing(setup):
     xf, s, mat = setup
     import torch
-    from xfluid_amd.autograd import checkpointed, project as tproject
+    from xfluid_amd.autograd import project as tproject, rollout
     say("[adjoint] torch autograd: finite differences + checkpointed rollout")
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(13)
@@ -84,7 +84,7 @@ def test_torch_autograd_and_checkpointing(setup):
         return tproject(s, u, v + gravity, w)
 
     def loss_fn(state, steps, every):
-        out = checkpointed(step, state, steps, every)
+        out = rollout(step, state, steps, every)
         return 0.5 * sum((m * o * o).sum() for m, o in zip(masks, out))
 
     leaves = [t.clone().requires_grad_(True) for t in u0]

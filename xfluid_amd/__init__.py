@@ -121,6 +121,11 @@ def lib():
     L.xf_sim_p2g_adjoint.argtypes = [V, C.c_int64, V, V, V, V, V, V, V, C.c_int]
     L.xf_sim_density_adjoint.argtypes = [V, C.c_int64, V, V, V, C.c_int]
     L.xf_sim_p2g_adjoint_noalloc.argtypes = [V, C.c_int64, V, V, V, V, V, V, V]
+    L.xf_sim_extrapolate_adjoint.argtypes = [V, V, V, V, C.c_int]
+    L.xf_sim_set_particles_device.argtypes = [V, C.c_int64, V, V]
+    L.xf_sim_get_particles_device.argtypes = [V, V, V]
+    L.xf_sim_set_field_device.argtypes = [V, C.c_int, V, C.c_size_t]
+    L.xf_sim_get_field_device.argtypes = [V, C.c_int, V, C.c_size_t]
     L.xf_sim_generate_mesh.argtypes = [V, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.xf_sim_get_mesh.argtypes = [V, V, V, V]
     L.xf_sim_write_mesh_ply.argtypes = [V, C.c_char_p]
@@ -309,6 +314,38 @@ class FluidSimulator:
     def project_fields(self, u, v, w):
         """u' = u - G A^-1 B u in place on the sim's material; returns the PCG status."""
         return self._faces_call(self.L.xf_sim_project_fields, u, v, w, "project_fields")
+
+    def extrapolate_adjoint(self, gu, gv, gw):
+        """Vector-Jacobian product of the velocity extrapolation on the sim's material, in place."""
+        return self._faces_call(self.L.xf_sim_extrapolate_adjoint, gu, gv, gw, "extrapolate_adjoint")
+
+    # -- device (torch) state transfer -------------------------------------------
+    @staticmethod
+    def _dev(t, what, dtype="torch.float64"):
+        if not getattr(t, "is_cuda", False) or not t.is_contiguous() or str(t.dtype) != dtype:
+            raise ValueError(f"{what}: expected a contiguous {dtype} device tensor")
+        return t.data_ptr()
+
+    def set_particles_device(self, pos, vel):
+        """Positions and velocities from n x 3 float64 device tensors (ids 0..n-1 in row order)."""
+        n = pos.shape[0]
+        _chk(self.L.xf_sim_set_particles_device(self.h, n, self._dev(pos, "pos"), self._dev(vel, "vel")),
+             "set_particles_device")
+
+    def get_particles_device(self, pos, vel):
+        """Positions and velocities into n x 3 float64 device tensors, in id order."""
+        _chk(self.L.xf_sim_get_particles_device(self.h, self._dev(pos, "pos"), self._dev(vel, "vel")),
+             "get_particles_device")
+
+    def set_field_device(self, which, t):
+        dt = "torch.uint8" if which == FIELD["MATERIAL"] else "torch.float64"
+        _chk(self.L.xf_sim_set_field_device(self.h, which, self._dev(t, "field", dt), t.numel() * t.element_size()),
+             "set_field_device")
+
+    def get_field_device(self, which, t):
+        dt = "torch.uint8" if which == FIELD["MATERIAL"] else "torch.float64"
+        _chk(self.L.xf_sim_get_field_device(self.h, which, self._dev(t, "field", dt), t.numel() * t.element_size()),
+             "get_field_device")
 
     def project_adjoint(self, gu, gv, gw):
         """Vector-Jacobian product of project_fields, in place; returns the PCG status."""

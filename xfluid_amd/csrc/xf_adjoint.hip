@@ -174,30 +174,65 @@ __global__ void k_sample_adj(Geo g, Faces F, FaceGrads G, long n, const double* 
     for (int a = 0; a < 3; ++a) gpos[3 * p + a] += gp[a];
 }
 
-// RK4 (FluidSimulator.cpp:896-907) / RK2 advection p' = p + dt * sum b_s k_s,
-// collision-free.  Reverse sweep over the stages; gpos is overwritten.
-__global__ void k_advect_adj(Geo g, Faces F, FaceGrads G, long n, const double* __restrict__ pos, double dt,
-                             int integrator, const double* __restrict__ gnew, double* __restrict__ gpos) {
+// The position update of the advection stage: the RK4 (FluidSimulator.cpp:
+// 896-907) / RK2 target np = p + dt * sum b_s k_s, then the collision branch
+// (resolve_move, FluidSimulator.cpp:686-722).  The VJP first splits the
+// gradient on the new position into the part that reaches the target (grk)
+// and the part that goes to p directly (gextra): target accepted: all to the
+// target; position kept: all to p; pulled back from the first solid voxel,
+// X' = p + rn (t - back) with t = (B - p_a) / rn_a, rn = d / |d|, d = np - p:
+//   g_rn = (t - back) g - e_a (g.rn) t / rn_a,  g_d = (I - rn rn^T) g_rn / |d|,
+//   grk = g_d,  gextra = g - e_a (g.rn) / rn_a - g_d.
+// Then a reverse sweep over the RK stages.  mat == null: no collision branch.
+// gpos is overwritten.
+__global__ void k_advect_adj(Geo g, Faces F, FaceGrads G, const uint8_t* __restrict__ mat, long n,
+                             const double* __restrict__ pos, double dt, int integrator,
+                             const double* __restrict__ gnew, double* __restrict__ gpos) {
     const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (p >= n) return;
     const double pp[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-    const double gn[3] = {gnew[3 * p], gnew[3 * p + 1], gnew[3 * p + 2]};
-    double gp[3] = {gn[0], gn[1], gn[2]};
-    double k1[3], k2[3], k3[3], q2[3], q3[3], q4[3];
+    const double go[3] = {gnew[3 * p], gnew[3 * p + 1], gnew[3 * p + 2]};
+    double k1[3], k2[3], k3[3], k4[3], q2[3], q3[3], q4[3], np_[3];
     sample(g, F, pp, k1);
+    for (int a = 0; a < 3; ++a) q2[a] = pp[a] + k1[a] * (0.5 * dt);
+    sample(g, F, q2, k2);
     if (integrator == XF_INTEGRATOR_RK2) {
-        for (int a = 0; a < 3; ++a) q2[a] = pp[a] + k1[a] * (0.5 * dt);
+        for (int a = 0; a < 3; ++a) np_[a] = pp[a] + k2[a] * dt;
+    } else {
+        for (int a = 0; a < 3; ++a) q3[a] = pp[a] + k2[a] * (0.5 * dt);
+        sample(g, F, q3, k3);
+        for (int a = 0; a < 3; ++a) q4[a] = pp[a] + k3[a] * dt;
+        sample(g, F, q4, k4);
+        for (int a = 0; a < 3; ++a) np_[a] = pp[a] + (k1[a] + k2[a] * 2.0 + k3[a] * 2.0 + k4[a]) * (dt / 6.0);
+    }
+    double gn[3] = {go[0], go[1], go[2]}, gextra[3] = {0.0, 0.0, 0.0};
+    if (mat) {
+        double out[3];
+        Collision c;
+        const int kind = resolve_move(g, mat, pp, np_, out, &c);
+        if (kind == MOVE_KEPT) {
+            for (int a = 0; a < 3; ++a) { gextra[a] = go[a]; gn[a] = 0.0; }
+        } else if (kind == MOVE_PULLED) {
+            const int ax = c.axis;
+            const double s = go[0] * c.rn[0] + go[1] * c.rn[1] + go[2] * c.rn[2];
+            const double cc = c.tmin - c.back;
+            double grn[3];
+            for (int k = 0; k < 3; ++k) grn[k] = cc * go[k] - (k == ax ? s * c.tmin / c.rn[ax] : 0.0);
+            const double rg = c.rn[0] * grn[0] + c.rn[1] * grn[1] + c.rn[2] * grn[2];
+            for (int k = 0; k < 3; ++k) {
+                gn[k] = (grn[k] - c.rn[k] * rg) / c.l;
+                gextra[k] = go[k] - (k == ax ? s / c.rn[ax] : 0.0) - gn[k];
+            }
+        }
+    }
+    double gp[3] = {gn[0], gn[1], gn[2]};
+    if (integrator == XF_INTEGRATOR_RK2) {
         const double gk2[3] = {gn[0] * dt, gn[1] * dt, gn[2] * dt};
         double gq2[3] = {0.0, 0.0, 0.0}, gk1[3];
         sample_adj(g, F, G, q2, gk2, gq2);
         for (int a = 0; a < 3; ++a) { gp[a] += gq2[a]; gk1[a] = gq2[a] * (0.5 * dt); }
         sample_adj(g, F, G, pp, gk1, gp);
     } else {
-        for (int a = 0; a < 3; ++a) q2[a] = pp[a] + k1[a] * (0.5 * dt);
-        sample(g, F, q2, k2);
-        for (int a = 0; a < 3; ++a) q3[a] = pp[a] + k2[a] * (0.5 * dt);
-        sample(g, F, q3, k3);
-        for (int a = 0; a < 3; ++a) q4[a] = pp[a] + k3[a] * dt;
         double gk1[3], gk2[3], gk3[3], gk4[3], gq[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < 3; ++a) {
             const double b = gn[a] * (dt / 6.0);
@@ -211,7 +246,7 @@ __global__ void k_advect_adj(Geo g, Faces F, FaceGrads G, long n, const double* 
         for (int a = 0; a < 3; ++a) { gp[a] += gq[a]; gk1[a] += gq[a] * (0.5 * dt); }
         sample_adj(g, F, G, pp, gk1, gp);
     }
-    for (int a = 0; a < 3; ++a) gpos[3 * p + a] = gp[a];
+    for (int a = 0; a < 3; ++a) gpos[3 * p + a] = gp[a] + gextra[a];
 }
 
 // P2G (FluidSimulator.cpp:909-1085): u_f = sum_p w_pf v_p / sum_p w_pf (0 when
@@ -302,11 +337,12 @@ void launch_sample_adjoint(const Geo& g, const double* const F[3], double* const
     k_sample_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, Faces{F[0], F[1], F[2]}, FaceGrads{G[0], G[1], G[2]}, n, pos,
                                                       gvel, gpos);
 }
-void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const G[3], long n, const double* pos,
-                           double dt, int integrator, const double* gnew, double* gpos, hipStream_t st) {
+void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const G[3], const uint8_t* mat, long n,
+                           const double* pos, double dt, int integrator, const double* gnew, double* gpos,
+                           hipStream_t st) {
     if (n <= 0) return;
-    k_advect_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, Faces{F[0], F[1], F[2]}, FaceGrads{G[0], G[1], G[2]}, n, pos,
-                                                      dt, integrator, gnew, gpos);
+    k_advect_adj<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, Faces{F[0], F[1], F[2]}, FaceGrads{G[0], G[1], G[2]}, mat, n,
+                                                      pos, dt, integrator, gnew, gpos);
 }
 // clears the weight and value sums of one face array (a kernel, so a caller
 // may record the launch into a graph)

@@ -893,6 +893,26 @@ int xf_sim_set_particles(xf_sim* s, int64_t n, const double* px, const double* p
     return upload_particles(s, n, px, py, pz, vx, vy, vz);
 }
 
+int xf_sim_set_particles_device(xf_sim* s, int64_t n, const double* pos, const double* vel) {
+    if (check(s)) return -1;
+    if (n < 0 || n >= (1LL << 31)) return set_err(-1, "particle count out of range");
+    if (n > 0 && (!pos || !vel)) return set_err(-1, "null particle buffer");
+    int rc = ensure_particles(s, n);
+    if (rc) return rc;
+    particles_from_n3(s->P, pos, vel, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->sorted = false;
+    return 0;
+}
+int xf_sim_get_particles_device(xf_sim* s, double* pos, double* vel) {
+    if (check(s)) return -1;
+    particles_to_n3(s->P, pos, vel, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
 // Particles come back in their original order (the device keeps them sorted by cell).
 int xf_sim_get_particles(xf_sim* s, double* px, double* py, double* pz, double* vx, double* vy, double* vz) {
     if (check(s)) return -1;
@@ -946,6 +966,29 @@ int xf_sim_set_field(xf_sim* s, int which, const void* host, size_t bytes) {
     HIPCHK(hipMemcpyAsync(d, host, b, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     if (which == XF_FIELD_MATERIAL) std::memcpy(s->h_mat.data(), host, b);
+    return 0;
+}
+int xf_sim_set_field_device(xf_sim* s, int which, const void* dev, size_t bytes) {
+    if (check(s) || !dev) return set_err(-1, "null argument");
+    size_t b = 0;
+    void* d = field_dev(s, which, &b);
+    if (!d) return set_err(-1, "unknown field %d", which);
+    if (b != bytes) return set_err(-1, "field %d holds %zu bytes, got %zu", which, b, bytes);
+    HIPCHK(hipMemcpyAsync(d, dev, b, hipMemcpyDeviceToDevice, s->st));
+    if (which == XF_FIELD_MATERIAL) {  // the host copy of the material follows
+        HIPCHK(hipMemcpyAsync(s->h_mat.data(), d, b, hipMemcpyDeviceToHost, s->st));
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_get_field_device(xf_sim* s, int which, void* dev, size_t bytes) {
+    if (check(s) || !dev) return set_err(-1, "null argument");
+    size_t b = 0;
+    void* d = field_dev(s, which, &b);
+    if (!d) return set_err(-1, "unknown field %d", which);
+    if (b != bytes) return set_err(-1, "field %d holds %zu bytes, got %zu", which, b, bytes);
+    HIPCHK(hipMemcpyAsync(dev, d, b, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
     return 0;
 }
 int xf_sim_get_field(xf_sim* s, int which, void* host, size_t bytes) {
@@ -1131,7 +1174,7 @@ int xf_sim_advect_adjoint(xf_sim* s, int64_t n, const double* pos, double dt, co
     STAGE_OR_FAIL(GW, gw, s->nw(), true, true);
     const double* F[3] = {s->u, s->v, s->w};
     double* G[3] = {GU.dev, GVV.dev, GW.dev};
-    launch_advect_adjoint(s->g, F, G, n, P.dev, dt, s->prm.integrator, GN.dev, GP.dev, s->st);
+    launch_advect_adjoint(s->g, F, G, s->mat, n, P.dev, dt, s->prm.integrator, GN.dev, GP.dev, s->st);
     HIPCHK(hipGetLastError());
     HIPCHK(GP.sync()); HIPCHK(GU.sync()); HIPCHK(GVV.sync()); HIPCHK(GW.sync());
     HIPCHK(hipStreamSynchronize(s->st));
@@ -1184,6 +1227,22 @@ int xf_sim_p2g_adjoint_noalloc(xf_sim* s, int64_t n, const double* pos, const do
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s->st);
     if (e != hipSuccess) return set_err(-2, "p2g adjoint (no allocation): %s", hipGetErrorString(e));
+    return 0;
+}
+
+int xf_sim_extrapolate_adjoint(xf_sim* s, double* gu, double* gv, double* gw, int device_ptrs) {
+    if (check(s) || !gu || !gv || !gw) return set_err(-1, "null argument");
+    const int layers = (int)(s->prm.cfl + 5);
+    if (layers > 120) return set_err(-1, "extrapolation layers %d exceed 120", layers);
+    hipError_t herr = hipSuccess;
+    STAGE_OR_FAIL(GU, gu, s->nu(), true, true);
+    STAGE_OR_FAIL(GVV, gv, s->nv(), true, true);
+    STAGE_OR_FAIL(GW, gw, s->nw(), true, true);
+    double* G[3] = {GU.dev, GVV.dev, GW.dev};
+    for (int a = 0; a < 3; ++a) launch_extrapolate_adjoint(s->g, s->mat, G[a], s->marker, a, layers, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(GU.sync()); HIPCHK(GVV.sync()); HIPCHK(GW.sync());
+    HIPCHK(hipStreamSynchronize(s->st));
     return 0;
 }
 

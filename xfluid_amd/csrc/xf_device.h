@@ -228,3 +228,110 @@ __device__ inline void bw_next(const BoxRange& b, unsigned& kq, unsigned& off) {
 }
 __device__ inline long bw_cell(const BoxRange& b, unsigned kq, unsigned off) { return b.base + (long)kq * b.nn + off; }
 
+// collision.cpp:293-385 (voxel walk to the first solid cell)
+__device__ inline bool voxel_hit(const Geo& g, const uint8_t* mat, const double* a, const double* b, int* vox) {
+    const double invdx = 1.0 / g.dx;
+    double p0[3] = {a[0] * invdx, a[1] * invdx, a[2] * invdx};
+    double p1[3] = {b[0] * invdx, b[1] * invdx, b[2] * invdx};
+    int g0[3], g1[3], sd[3], gg[3];
+    double gp[3], vv[3];
+    for (int t = 0; t < 3; ++t) {
+        g0[t] = (int)floor(p0[t]); g1[t] = (int)floor(p1[t]);
+        sd[t] = g1[t] > g0[t] ? 1 : g1[t] < g0[t] ? -1 : 0;
+        gg[t] = g0[t];
+        gp[t] = g0[t] + (g1[t] > g0[t] ? 1 : 0);
+        vv[t] = p1[t] == p0[t] ? 1 : p1[t] - p0[t];
+    }
+    double vxvy = vv[0] * vv[1], vxvz = vv[0] * vv[2], vyvz = vv[1] * vv[2];
+    double errx = (gp[0] - p0[0]) * vyvz, erry = (gp[1] - p0[1]) * vxvz, errz = (gp[2] - p0[2]) * vxvy;
+    double derrx = sd[0] * vyvz, derry = sd[1] * vxvz, derrz = sd[2] * vxvy;
+    // the walk ends after |dg| steps; bound it for safety
+    int budget = abs(g1[0] - g0[0]) + abs(g1[1] - g0[1]) + abs(g1[2] - g0[2]) + 1;
+    for (int step = 0; step <= budget; ++step) {
+        if (cin(g, gg[0], gg[1], gg[2]) && mat[cidx(g, gg[0], gg[1], gg[2])] == XF_SOLID) {
+            vox[0] = gg[0]; vox[1] = gg[1]; vox[2] = gg[2];
+            return true;
+        }
+        if (gg[0] == g1[0] && gg[1] == g1[1] && gg[2] == g1[2]) break;
+        double xr = fabs(errx), yr = fabs(erry), zr = fabs(errz);
+        if (sd[0] != 0 && (sd[1] == 0 || xr < yr) && (sd[2] == 0 || xr < zr)) { gg[0] += sd[0]; errx += derrx; }
+        else if (sd[1] != 0 && (sd[2] == 0 || yr < zr)) { gg[1] += sd[1]; erry += derry; }
+        else if (sd[2] != 0) { gg[2] += sd[2]; errz += derrz; }
+    }
+    return false;
+}
+
+// collision.cpp:396-439, including its repeated dir.x guard.  Also the axis
+// whose slab the ray enters last (the face it hits) and whether that face is
+// the box's upper bound.
+__device__ inline bool ray_box(const double* p0, const double* din, const double* bmin, const double* bmax,
+                               double* hit, int* axis = nullptr, bool* upper = nullptr) {
+    double dir[3] = {din[0], din[1], din[2]};
+    const double eps = 1e-10;
+    if (fabs(dir[0]) < eps) dir[0] = dir[0] < 0 ? -eps : eps;
+    if (fabs(dir[1]) < eps) dir[1] = dir[1] < 0 ? -eps : eps;
+    if (fabs(dir[0]) < eps) dir[2] = dir[2] < 0 ? -eps : eps;
+    double inv[3] = {1.0 / dir[0], 1.0 / dir[1], 1.0 / dir[2]};
+    double t1 = (bmin[0] - p0[0]) * inv[0], t2 = (bmax[0] - p0[0]) * inv[0];
+    double tmin = fmin(t1, t2), tmax = fmax(t1, t2);
+    const double m0 = tmin;
+    t1 = (bmin[1] - p0[1]) * inv[1]; t2 = (bmax[1] - p0[1]) * inv[1];
+    const double m1 = fmin(t1, t2);
+    tmin = fmax(tmin, m1); tmax = fmin(tmax, fmax(t1, t2));
+    t1 = (bmin[2] - p0[2]) * inv[2]; t2 = (bmax[2] - p0[2]) * inv[2];
+    const double m2 = fmin(t1, t2);
+    tmin = fmax(tmin, m2); tmax = fmin(tmax, fmax(t1, t2));
+    if (tmax > fmax(tmin, 0.0)) {
+        for (int a = 0; a < 3; ++a) hit[a] = p0[a] + dir[a] * tmin;
+        if (axis) {
+            const int ax = tmin == m2 ? 2 : tmin == m1 ? 1 : (tmin == m0 ? 0 : 0);
+            *axis = ax;
+            *upper = inv[ax] < 0.0;
+        }
+        return true;
+    }
+    return false;
+}
+
+// The end of a particle's move in _advectParticles (FluidSimulator.cpp:686-722):
+// the integrated target np_ when its cell is in range and not solid; else the
+// first solid voxel on the segment pos -> np_ (Collision::getLineSegmentVoxel-
+// Intersection), the ray's entry into that voxel's box (the reference passes
+// the voxel centre as the box's min corner, aabb.cpp:11-13), pulled back by
+// (float)(0.05 dx) along the ray, accepted if its cell is in range and not
+// solid; otherwise the particle keeps pos.  `c` records what the adjoint needs.
+enum { MOVE_TARGET = 0, MOVE_KEPT = 1, MOVE_PULLED = 2 };
+struct Collision {
+    double rn[3], l, tmin, back;
+    int axis;
+};
+__device__ inline int resolve_move(const Geo& g, const uint8_t* mat, const double* pos, const double* np_,
+                                   double* out, Collision* c) {
+    int gi = pos2idx(np_[0], g.dx), gj = pos2idx(np_[1], g.dx), gk = pos2idx(np_[2], g.dx);
+    if (cin(g, gi, gj, gk) && mat[cidx(g, gi, gj, gk)] != XF_SOLID) {
+        out[0] = np_[0]; out[1] = np_[1]; out[2] = np_[2];
+        return MOVE_TARGET;
+    }
+    int vox[3];
+    if (!voxel_hit(g, mat, pos, np_, vox)) return MOVE_KEPT;
+    double dv[3] = {np_[0] - pos[0], np_[1] - pos[1], np_[2] - pos[2]};
+    double l = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    double rn[3] = {dv[0] / l, dv[1] / l, dv[2] / l};
+    double bmin[3] = {idx2pos(vox[0], g.dx), idx2pos(vox[1], g.dx), idx2pos(vox[2], g.dx)};
+    double bmax[3] = {bmin[0] + g.dx, bmin[1] + g.dx, bmin[2] + g.dx};
+    double cp[3];
+    int axis = 0;
+    bool upper = false;
+    if (!ray_box(pos, rn, bmin, bmax, cp, &axis, &upper)) return MOVE_KEPT;
+    const double back = (double)(float)(0.05 * g.dx);
+    double res[3] = {cp[0] - rn[0] * back, cp[1] - rn[1] * back, cp[2] - rn[2] * back};
+    int ri = pos2idx(res[0], g.dx), rj = pos2idx(res[1], g.dx), rk = pos2idx(res[2], g.dx);
+    if (!(cin(g, ri, rj, rk) && mat[cidx(g, ri, rj, rk)] != XF_SOLID)) return MOVE_KEPT;
+    out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
+    for (int a = 0; a < 3; ++a) c->rn[a] = rn[a];
+    c->l = l;
+    c->axis = axis;
+    c->tmin = ((upper ? bmax[axis] : bmin[axis]) - pos[axis]) / rn[axis];
+    c->back = back;
+    return MOVE_PULLED;
+}

@@ -270,6 +270,64 @@ __global__ void k_extrap_layer(Geo g, double* f, int8_t* mark, int axis, int L) 
     }
 }
 
+// ---- adjoint of the extrapolation (fixed material) ----
+// Layer L holds the faces first reached at step L; each took the mean of its
+// neighbours in layer L-1.  The transpose runs the layers backwards: a face of
+// layer L-1 gathers g(t) / count(t) from its neighbours t in layer L (count(t)
+// = t's neighbours in layer L-1), in the order +i,-i,+j,-j,+k,-k; finally the
+// faces of layers >= 1 drop their gradient (their own values were overwritten).
+__global__ void k_extrap_mark(Geo g, int8_t* mark, int axis, int L) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)si * sj * sk || mark[t] != -1) return;
+    int i, j, k;
+    unflat3(t, si, sj, &i, &j, &k);
+    const long sij = (long)si * sj;
+    const int8_t want = (int8_t)(L - 1);
+    if ((i + 1 < si && mark[t + 1] == want) || (i - 1 >= 0 && mark[t - 1] == want) ||
+        (j + 1 < sj && mark[t + si] == want) || (j - 1 >= 0 && mark[t - si] == want) ||
+        (k + 1 < sk && mark[t + sij] == want) || (k - 1 >= 0 && mark[t - sij] == want))
+        mark[t] = (int8_t)L;
+}
+__device__ inline int extrap_count(const int8_t* mark, long t, int i, int j, int k, int si, int sj, int sk,
+                                   int8_t want) {
+    const long sij = (long)si * sj;
+    return (i + 1 < si && mark[t + 1] == want) + (i - 1 >= 0 && mark[t - 1] == want) +
+           (j + 1 < sj && mark[t + si] == want) + (j - 1 >= 0 && mark[t - si] == want) +
+           (k + 1 < sk && mark[t + sij] == want) + (k - 1 >= 0 && mark[t - sij] == want);
+}
+__global__ void k_extrap_layer_adj(Geo g, double* gf, const int8_t* __restrict__ mark, int axis, int L) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)si * sj * sk || mark[n] != (int8_t)(L - 1)) return;
+    int i, j, k;
+    unflat3(n, si, sj, &i, &j, &k);
+    const long sij = (long)si * sj;
+    const int8_t lay = (int8_t)L, prev = (int8_t)(L - 1);
+    const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+    double acc = 0.0;
+    for (int e = 0; e < 6; ++e) {
+        const int a = i + di[e], b = j + dj[e], c = k + dk[e];
+        if (a < 0 || b < 0 || c < 0 || a >= si || b >= sj || c >= sk) continue;
+        const long t = n + di[e] + (long)si * dj[e] + sij * dk[e];
+        if (mark[t] != lay) continue;
+        acc += gf[t] / (double)extrap_count(mark, t, a, b, c, si, sj, sk, prev);
+    }
+    gf[n] += acc;
+}
+__global__ void k_extrap_drop(long nf, double* gf, const int8_t* __restrict__ mark) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < nf && mark[t] >= 1) gf[t] = 0.0;
+}
+void launch_extrapolate_adjoint(const Geo& g, const uint8_t* mat, double* gf, int8_t* marker, int axis, int layers,
+                                hipStream_t st) {
+    const long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+    k_extrap_init<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, marker, axis);
+    for (int L = 1; L <= layers; ++L) k_extrap_mark<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, marker, axis, L);
+    for (int L = layers; L >= 1; --L) k_extrap_layer_adj<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, gf, marker, axis, L);
+    k_extrap_drop<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(nf, gf, marker);
+}
+
 void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
                         hipStream_t st) {
     long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));

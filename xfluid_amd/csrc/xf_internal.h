@@ -38,6 +38,9 @@ void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, do
                            int op = OP_VARIATIONAL);
 void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
                         hipStream_t st);
+// VJP of launch_extrapolate for gradient gf on one face array (in place)
+void launch_extrapolate_adjoint(const Geo& g, const uint8_t* mat, double* gf, int8_t* marker, int axis, int layers,
+                                hipStream_t st);
 
 // ---------------- pressure solve kernels (xf_pcg.hip) ----------------
 // Buffers of the pipelined 8x8-tile sweeps (xf_sweep_wave.hip), oct layout:
@@ -139,6 +142,10 @@ struct CellIndex {
     int scan_tmp_n;
 };
 void sort_particles(const Geo& g, Particles& P, CellIndex& ci, hipStream_t st);
+// n x 3 interleaved device arrays in id order <-> the SoA arrays (from_n3 sets
+// the ids to 0..n-1; pos or vel may be null in to_n3)
+void particles_to_n3(const Particles& P, double* pos, double* vel, hipStream_t st);
+void particles_from_n3(Particles& P, const double* pos, const double* vel, hipStream_t st);
 void launch_g2p_advect(const Geo& g, Particles& P, const uint8_t* mat, const double* u, const double* v,
                        const double* w, const double* us, const double* vs, const double* ws, double dt, double flip,
                        int integrator, int64_t limit_id, hipStream_t st);
@@ -187,8 +194,10 @@ void launch_rhs_adjoint(const Geo& g, const uint8_t* mat, const double* bbar, do
 // particle transfers (particles as n x 3 arrays); grid gradients accumulate
 void launch_sample_adjoint(const Geo& g, const double* const F[3], double* const G[3], long n, const double* pos,
                            const double* gvel, double* gpos, hipStream_t st);
-void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const G[3], long n, const double* pos,
-                           double dt, int integrator, const double* gnew, double* gpos, hipStream_t st);
+// the position update with its collision branch when mat is given
+void launch_advect_adjoint(const Geo& g, const double* const F[3], double* const G[3], const uint8_t* mat, long n,
+                           const double* pos, double dt, int integrator, const double* gnew, double* gpos,
+                           hipStream_t st);
 void launch_density_adjoint(const Geo& g, int d, double R, long n, const double* pos, const double* gphi,
                             double* gpos, hipStream_t st);
 void launch_p2g_adjoint(const Geo& g, int kernel, long n, const double* pos, const double* vel,

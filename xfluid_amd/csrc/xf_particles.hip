@@ -182,59 +182,6 @@ void sort_particles(const Geo& g, Particles& P, CellIndex& ci, hipStream_t st) {
     std::swap(P.pid, P.tpid);
 }
 
-// collision.cpp:293-385 (voxel walk to the first solid cell)
-__device__ bool voxel_hit(const Geo& g, const uint8_t* mat, const double* a, const double* b, int* vox) {
-    const double invdx = 1.0 / g.dx;
-    double p0[3] = {a[0] * invdx, a[1] * invdx, a[2] * invdx};
-    double p1[3] = {b[0] * invdx, b[1] * invdx, b[2] * invdx};
-    int g0[3], g1[3], sd[3], gg[3];
-    double gp[3], vv[3];
-    for (int t = 0; t < 3; ++t) {
-        g0[t] = (int)floor(p0[t]); g1[t] = (int)floor(p1[t]);
-        sd[t] = g1[t] > g0[t] ? 1 : g1[t] < g0[t] ? -1 : 0;
-        gg[t] = g0[t];
-        gp[t] = g0[t] + (g1[t] > g0[t] ? 1 : 0);
-        vv[t] = p1[t] == p0[t] ? 1 : p1[t] - p0[t];
-    }
-    double vxvy = vv[0] * vv[1], vxvz = vv[0] * vv[2], vyvz = vv[1] * vv[2];
-    double errx = (gp[0] - p0[0]) * vyvz, erry = (gp[1] - p0[1]) * vxvz, errz = (gp[2] - p0[2]) * vxvy;
-    double derrx = sd[0] * vyvz, derry = sd[1] * vxvz, derrz = sd[2] * vxvy;
-    // the walk ends after |dg| steps; bound it for safety
-    int budget = abs(g1[0] - g0[0]) + abs(g1[1] - g0[1]) + abs(g1[2] - g0[2]) + 1;
-    for (int step = 0; step <= budget; ++step) {
-        if (cin(g, gg[0], gg[1], gg[2]) && mat[cidx(g, gg[0], gg[1], gg[2])] == XF_SOLID) {
-            vox[0] = gg[0]; vox[1] = gg[1]; vox[2] = gg[2];
-            return true;
-        }
-        if (gg[0] == g1[0] && gg[1] == g1[1] && gg[2] == g1[2]) break;
-        double xr = fabs(errx), yr = fabs(erry), zr = fabs(errz);
-        if (sd[0] != 0 && (sd[1] == 0 || xr < yr) && (sd[2] == 0 || xr < zr)) { gg[0] += sd[0]; errx += derrx; }
-        else if (sd[1] != 0 && (sd[2] == 0 || yr < zr)) { gg[1] += sd[1]; erry += derry; }
-        else if (sd[2] != 0) { gg[2] += sd[2]; errz += derrz; }
-    }
-    return false;
-}
-
-// collision.cpp:396-439, including its repeated dir.x guard
-__device__ bool ray_box(const double* p0, const double* din, const double* bmin, const double* bmax, double* hit) {
-    double dir[3] = {din[0], din[1], din[2]};
-    const double eps = 1e-10;
-    if (fabs(dir[0]) < eps) dir[0] = dir[0] < 0 ? -eps : eps;
-    if (fabs(dir[1]) < eps) dir[1] = dir[1] < 0 ? -eps : eps;
-    if (fabs(dir[0]) < eps) dir[2] = dir[2] < 0 ? -eps : eps;
-    double inv[3] = {1.0 / dir[0], 1.0 / dir[1], 1.0 / dir[2]};
-    double t1 = (bmin[0] - p0[0]) * inv[0], t2 = (bmax[0] - p0[0]) * inv[0];
-    double tmin = fmin(t1, t2), tmax = fmax(t1, t2);
-    t1 = (bmin[1] - p0[1]) * inv[1]; t2 = (bmax[1] - p0[1]) * inv[1];
-    tmin = fmax(tmin, fmin(t1, t2)); tmax = fmin(tmax, fmax(t1, t2));
-    t1 = (bmin[2] - p0[2]) * inv[2]; t2 = (bmax[2] - p0[2]) * inv[2];
-    tmin = fmax(tmin, fmin(t1, t2)); tmax = fmin(tmax, fmax(t1, t2));
-    if (tmax > fmax(tmin, 0.0)) {
-        for (int a = 0; a < 3; ++a) hit[a] = p0[a] + dir[a] * tmin;
-        return true;
-    }
-    return false;
-}
 
 // _advectParticles (FluidSimulator.cpp:652-723): FLIP/PIC blend, RK4 (or RK2
 // midpoint) advection, collision resolution.  Particles whose original index
@@ -271,26 +218,38 @@ __global__ void k_g2p_advect(Geo g, long n, double* px, double* py, double* pz, 
         sample(g, Fn, q, k4);
         for (int a = 0; a < 3; ++a) np_[a] = pos[a] + (k1[a] + k2[a] * 2.0 + k3[a] * 2.0 + k4[a]) * (dt / 6.0);
     }
-    int gi = pos2idx(np_[0], g.dx), gj = pos2idx(np_[1], g.dx), gk = pos2idx(np_[2], g.dx);
-    if (cin(g, gi, gj, gk) && mat[cidx(g, gi, gj, gk)] != XF_SOLID) {
-        px[p] = np_[0]; py[p] = np_[1]; pz[p] = np_[2];
-        return;
+    double out[3];
+    Collision c;
+    if (resolve_move(g, mat, pos, np_, out, &c) != MOVE_KEPT) {
+        px[p] = out[0]; py[p] = out[1]; pz[p] = out[2];
     }
-    int vox[3];
-    if (!voxel_hit(g, mat, pos, np_, vox)) return;
-    double dv[3] = {np_[0] - pos[0], np_[1] - pos[1], np_[2] - pos[2]};
-    double l = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-    double rn[3] = {dv[0] / l, dv[1] / l, dv[2] / l};
-    double bmin[3] = {idx2pos(vox[0], g.dx), idx2pos(vox[1], g.dx), idx2pos(vox[2], g.dx)};
-    double bmax[3] = {bmin[0] + g.dx, bmin[1] + g.dx, bmin[2] + g.dx};
-    double cp[3];
-    if (!ray_box(pos, rn, bmin, bmax, cp)) return;
-    const double back = (double)(float)(0.05 * g.dx);
-    double res[3] = {cp[0] - rn[0] * back, cp[1] - rn[1] * back, cp[2] - rn[2] * back};
-    int ri = pos2idx(res[0], g.dx), rj = pos2idx(res[1], g.dx), rk = pos2idx(res[2], g.dx);
-    if (cin(g, ri, rj, rk) && mat[cidx(g, ri, rj, rk)] != XF_SOLID) {
-        px[p] = res[0]; py[p] = res[1]; pz[p] = res[2];
-    }
+}
+
+// Particles as n x 3 interleaved arrays in id order (the order they were set
+// in): gather from / scatter to the sorted SoA arrays by their ids.
+__global__ void k_to_n3(long n, const double* __restrict__ px, const double* __restrict__ py,
+                        const double* __restrict__ pz, const double* __restrict__ vx, const double* __restrict__ vy,
+                        const double* __restrict__ vz, const int32_t* __restrict__ pid, double* pos, double* vel) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long q = 3 * (long)pid[p];
+    if (pos) { pos[q] = px[p]; pos[q + 1] = py[p]; pos[q + 2] = pz[p]; }
+    if (vel) { vel[q] = vx[p]; vel[q + 1] = vy[p]; vel[q + 2] = vz[p]; }
+}
+__global__ void k_from_n3(long n, const double* __restrict__ pos, const double* __restrict__ vel, double* px,
+                          double* py, double* pz, double* vx, double* vy, double* vz, int32_t* pid) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    px[p] = pos[3 * p]; py[p] = pos[3 * p + 1]; pz[p] = pos[3 * p + 2];
+    vx[p] = vel[3 * p]; vy[p] = vel[3 * p + 1]; vz[p] = vel[3 * p + 2];
+    pid[p] = (int32_t)p;
+}
+void particles_to_n3(const Particles& P, double* pos, double* vel, hipStream_t st) {
+    if (P.n > 0) k_to_n3<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, P.px, P.py, P.pz, P.vx, P.vy, P.vz, P.pid, pos, vel);
+}
+void particles_from_n3(Particles& P, const double* pos, const double* vel, hipStream_t st) {
+    if (P.n > 0)
+        k_from_n3<<<xf_blocks(P.n), XF_BLOCK, 0, st>>>(P.n, pos, vel, P.px, P.py, P.pz, P.vx, P.vy, P.vz, P.pid);
 }
 
 void launch_g2p_advect(const Geo& g, Particles& P, const uint8_t* mat, const double* u, const double* v,
