@@ -259,6 +259,223 @@ def run_source_demo(xf, frames, cpu_frames):
                       "ms_last_frame": round(per_frame[-1] * 1e3, 2), "cpu_baseline": cpu}), flush=True)
 
 
+def _face_boxes(ni, nj, nk, dx, lo, hi, torch):
+    """0/1 masks of the MAC faces whose positions lie in [lo, hi]^3 (device tensors)."""
+    out = []
+    for comp, shape in enumerate([(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]):
+        k, j, i = np.meshgrid(*(np.arange(s_) for s_ in shape), indexing="ij")
+        off = [0.5, 0.5, 0.5]
+        off[comp] = 0.0
+        x, y, z = (i + off[0]) * dx, (j + off[1]) * dx, (k + off[2]) * dx
+        m = (x >= lo) & (x <= hi) & (y >= lo) & (y <= hi) & (z >= lo) & (z <= hi)
+        out.append(torch.tensor(m.astype(np.float64), device="cuda"))
+    return out
+
+
+def run_diff_projection(xf, n, steps, every, dirs, eps):
+    """The differentiable-projection workload: an n^3 unit cube with 1-cell walls,
+    liquid below y = 0.7 (8 particles per cell, one per sub-cell, seed 3), initial
+    face velocity = curl of a seeded Perlin vector potential (unit RMS) + N(0, 0.1),
+    rounded to float32; gravity -9.81; `steps` substeps of dt = 1/240 with FLIP 0
+    (the particles carry the grid velocity: advect -> P2G -> project), FP64 PCG to
+    1e-12; loss 1/2 |u|^2 over the faces in [0.4, 0.6]^3 after the last substep.
+    The gradient with respect to the initial face velocity comes from the adjoint
+    kernels with a checkpoint every `every` substeps; it is checked by central
+    differences along `dirs` seeded directions (step eps).  A direction whose
+    +eps or -eps run changes some substep's material (a cell gaining its first or
+    losing its last particle: a discontinuity of the loss) is counted as an
+    outlier, not compared."""
+    import torch
+    import xfluid_amd.autograd as ag
+    from xfluid_amd.scenes import curl_noise_faces
+    dx, dt = 1.0 / n, 1.0 / 240
+    p = xf.default_params(flip_ratio=0.0, pcg_tol=1e-12, pcg_max_iter=4000, tail_skip=0, density_every_substep=0)
+    s = xf.FluidSimulator(n, n, n, p)
+    s.setCellWidth(dx)
+    s.setSolidBoundary()
+    s.addCubicFluid((0.0, 0.0, 0.0), (1.0, 0.7, 1.0))
+    s.initParticles(xf.SEED_SUBCELL, 3)
+    s.addConstantForce(0.0, -9.81, 0.0)
+    npart = s.num_particles()
+    X0 = torch.empty((npart, 3), dtype=torch.float64, device="cuda")
+    V0 = torch.empty_like(X0)
+    s.get_particles_device(X0, V0)
+    U0 = [torch.tensor(a, device="cuda") for a in curl_noise_faces(n, n, n, dx, seed=3)]
+    boxes = _face_boxes(n, n, n, dx, 0.4, 0.6, torch)
+    log(f"[diff-projection] {n}^3, {npart} particles, {steps} substeps, checkpoint every {every}")
+
+    def loss_of(U, grad=False, mats=None):
+        leaves = [t.clone().requires_grad_(grad) for t in U]
+        step = lambda X, V, u, v, w: ag.substep(s, dt, X, V, u, v, w, material_log=mats)  # noqa: E731
+        _, _, u, v, w = ag.rollout(step, (X0, V0, *leaves), steps, every)
+        loss = 0.5 * sum((b * t * t).sum() for b, t in zip(boxes, (u, v, w)))
+        if grad:
+            loss.backward()
+            return loss.item(), [t.grad for t in leaves]
+        return loss.item()
+
+    loss_of(U0)  # warm-up (first-use allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        mats0 = []
+        L0 = loss_of(U0, mats=mats0)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, grad = loss_of(U0, grad=True)
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    gnorm = float(torch.sqrt(sum((g * g).sum() for g in grad)))
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    errs, outliers = [], []
+    for q in range(dirs):
+        d = [torch.randn(t.shape, generator=gen, dtype=torch.float64, device="cuda") for t in U0]
+        mp, mm = [], []
+        with torch.no_grad():
+            lp = loss_of([a + eps * b for a, b in zip(U0, d)], mats=mp)
+            lm = loss_of([a - eps * b for a, b in zip(U0, d)], mats=mm)
+        flips = sum(int((a != b).sum()) for a, b in zip(mp + mm, mats0 + mats0))
+        fd = (lp - lm) / (2 * eps)
+        an = float(sum((g * b).sum() for g, b in zip(grad, d)))
+        rel = abs(fd - an) / abs(an)
+        if flips:
+            outliers.append({"direction": q, "material_changes": flips, "rel_err": rel})
+        else:
+            errs.append(rel)
+        if q % 8 == 7:
+            log(f"[diff-projection] {q + 1} directions: worst {max(errs) if errs else 0:.2e}, "
+                f"{len(outliers)} outliers")
+    errs_sorted = sorted(errs)
+    print(json.dumps({
+        "metric": f"differentiable projection {n}^3: adjoint gradient vs central differences (eps {eps:g}, "
+                  f"{dirs} directions)",
+        "value": max(errs) if errs else None, "unit": "max relative error (directions without material change)",
+        "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "dtype": "f64",
+        "config": {"workload": f"{n}^3 unit cube, walls, liquid y < 0.7, curl-noise initial velocity (seed 3), "
+                               f"gravity, FLIP 0, dt 1/240, FP64 PCG 1e-12",
+                   "grid": [n, n, n], "particles": npart, "substeps": steps, "checkpoint_every": every},
+        "loss": L0, "grad_norm": gnorm,
+        "median_rel_err": errs_sorted[len(errs_sorted) // 2] if errs else None,
+        "directions_compared": len(errs), "outliers": outliers,
+        "forward_s": round(t_fwd, 3), "forward_backward_s": round(t_fb, 3),
+        "substeps_per_s_forward": round(steps / t_fwd, 2),
+        "substeps_per_s_forward_backward": round(steps / t_fb, 2)}), flush=True)
+    s.close()
+
+
+def _trilinear_upsample(coarse, pts, torch):
+    """Trilinear interpolation of coarse (C, Z, Y, X) cell-centre values spread over
+    the domain box at the points pts = (x, y, z) normalised to [0, 1]; values past
+    the outer cell centres are clamped to them."""
+    C, Z, Y, X = coarse.shape
+    gx = (pts[0] * X - 0.5).clamp(0, X - 1)
+    gy = (pts[1] * Y - 0.5).clamp(0, Y - 1)
+    gz = (pts[2] * Z - 0.5).clamp(0, Z - 1)
+    x0, y0, z0 = gx.floor().long().clamp(max=X - 2), gy.floor().long().clamp(max=Y - 2), gz.floor().long().clamp(max=Z - 2)
+    fx, fy, fz = gx - x0, gy - y0, gz - z0
+    out = 0.0
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx_ in (0, 1):
+                wgt = (fx if dx_ else 1 - fx) * (fy if dy else 1 - fy) * (fz if dz else 1 - fz)
+                out = out + wgt * coarse[:, z0 + dz, y0 + dy, x0 + dx_]
+    return out
+
+
+def run_force_control(xf, iters, steps, every, lr):
+    """Gradient-based body-force control: a 128 x 64 x 64 tank (dx 1/64, walls),
+    fluid y < 0.3 with 8 particles per cell (seed 4); a learnable float32 force
+    on a 16 x 8 x 8 coarse grid (3 components) is trilinearly upsampled to the
+    MAC faces and added each substep (on the faces next to fluid, beside gravity
+    -9.81); `steps` substeps of dt 1/240, FLIP 0.95, mixed-precision PCG.  Loss:
+    squared L2 distance between the final particle-blob density field and a
+    target = the initial fluid's mean interior density below a Gaussian mound
+    y_t(x) = 0.3 + 0.5 exp(-(x - 0.75)^2 / (2 * 0.15^2)), zero above.  Backward
+    through P2G, advection, G2P, extrapolation, projection and the density field,
+    checkpointed every `every` substeps; `iters` Adam steps."""
+    import torch
+    import xfluid_amd.autograd as ag
+    ni, nj, nk = 128, 64, 64
+    dx, dt = 1.0 / 64, 1.0 / 240
+    p = xf.default_params(flip_ratio=0.95, pcg_tol=1e-6, pcg_max_iter=400, precision=xf.PREC_MIXED, tail_skip=0,
+                          density_every_substep=0, mixed_inner_tol=1e-4)
+    s = xf.FluidSimulator(ni, nj, nk, p)
+    s.setCellWidth(dx)
+    s.setSolidBoundary()
+    s.addCubicFluid((0.0, 0.0, 0.0), (ni * dx, 0.3, nk * dx))
+    s.initParticles(xf.SEED_SUBCELL, 4)
+    s.addConstantForce(0.0, -9.81, 0.0)
+    npart = s.num_particles()
+    X0 = torch.empty((npart, 3), dtype=torch.float64, device="cuda")
+    V0 = torch.empty_like(X0)
+    s.get_particles_device(X0, V0)
+    U0 = [torch.zeros(sh, dtype=torch.float64, device="cuda") for sh in ((nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni))]
+    Lx, Ly, Lz = ni * dx, nj * dx, nk * dx
+    face_pts = []
+    for comp, shape in enumerate([(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]):
+        k, j, i = torch.meshgrid(*(torch.arange(s_, device="cuda", dtype=torch.float64) for s_ in shape), indexing="ij")
+        off = [0.5, 0.5, 0.5]
+        off[comp] = 0.0
+        face_pts.append(((i + off[0]) * dx / Lx, (j + off[1]) * dx / Ly, (k + off[2]) * dx / Lz))
+    coarse = torch.zeros((3, 8, 8, 16), dtype=torch.float32, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([coarse], lr=lr)
+    # target density
+    with torch.no_grad():
+        phi0 = ag.density(s, X0)
+        d = p.sub_division
+        kk, jj, ii = torch.meshgrid(*(torch.arange(m, device="cuda", dtype=torch.float64) for m in phi0.shape),
+                                    indexing="ij")
+        sub = dx / d
+        xv, yv = (ii + 0.5) * sub, (jj + 0.5) * sub
+        inner = (yv < 0.2) & (yv > 0.05) & (xv > 0.1) & (xv < Lx - 0.1)
+        level = float(phi0[inner].mean())
+        yt = 0.3 + 0.5 * torch.exp(-((xv - 0.75) ** 2) / (2 * 0.15 ** 2))
+        target = torch.where(yv < yt, level, 0.0)
+    free0, total = torch.cuda.mem_get_info()
+    peak = [total - free0]
+
+    def probe():
+        free, _ = torch.cuda.mem_get_info()
+        peak[0] = max(peak[0], total - free)
+
+    def step(X, V, u, v, w):
+        out = ag.substep(s, dt, X, V, u, v, w, force)
+        probe()
+        return out
+
+    log(f"[force-control] {ni}x{nj}x{nk}, {npart} particles, {steps} substeps, {iters} Adam steps")
+    losses, times = [], []
+    for it in range(iters):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        force = tuple(_trilinear_upsample(coarse.double(), pts, torch)[comp] for comp, pts in enumerate(face_pts))
+        X, V, u, v, w = ag.rollout(step, (X0, V0, *U0), steps, every)
+        phi = ag.density(s, X)
+        loss = ((phi - target) ** 2).sum()
+        loss.backward()
+        probe()
+        opt.step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        losses.append(loss.item())
+        if it % 10 == 0 or it == iters - 1:
+            log(f"[force-control] step {it}: loss {losses[-1]:.6e}, {times[-1]:.2f} s")
+    t_iter = sorted(times[1:])[len(times[1:]) // 2] if len(times) > 1 else times[0]
+    print(json.dumps({
+        "metric": "body-force control 128x64x64: forward+backward substeps/s",
+        "value": round(steps / t_iter, 3), "unit": "substeps/s (forward + backward, checkpointed)",
+        "higher_is_better": True, "n_gpus": 1, "data": "synthetic",
+        "dtype": "f64 (PCG bulk f32 + f64 dots/refinement; force parameters f32)",
+        "config": {"workload": "128x64x64 tank, dx 1/64, fluid y < 0.3 (seed 4), 16x8x8 learnable force, "
+                               "gravity, FLIP 0.95, dt 1/240, mixed PCG 1e-6",
+                   "particles": npart, "substeps": steps, "checkpoint_every": every, "adam_steps": iters, "lr": lr},
+        "loss_first": losses[0], "loss_last": losses[-1], "loss_min": min(losses),
+        "median_iteration_s": round(t_iter, 3), "peak_hbm_bytes": int(peak[0]),
+        "peak_hbm_GiB": round(peak[0] / 2 ** 30, 3)}), flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,6 +492,11 @@ def main():
                     help="only run the N^3 sphere-scene projection benchmark (N = 256 for the full size) and exit")
     ap.add_argument("--source-demo", type=int, default=0, metavar="FRAMES",
                     help="only run the reference's fluid-source demo for FRAMES frames and exit")
+    ap.add_argument("--diff-projection", type=int, default=0, metavar="N",
+                    help="only run the differentiable-projection workload on an N^3 grid (N = 96 for the full size)")
+    ap.add_argument("--directions", type=int, default=64, help="finite-difference directions (--diff-projection)")
+    ap.add_argument("--force-control", type=int, default=0, metavar="ITERS",
+                    help="only run the body-force control workload for ITERS Adam steps (50 for the full run)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -291,6 +513,12 @@ def main():
         raise SystemExit("bench.py: no HIP device")
     if args.projection:
         run_projection(xf, args.projection, 2)
+        return
+    if args.diff_projection:
+        run_diff_projection(xf, args.diff_projection, 16, 4, args.directions, 1e-4)
+        return
+    if args.force_control:
+        run_force_control(xf, args.force_control, 40, 4, 2.0)
         return
     if args.source_demo:
         run_source_demo(xf, args.source_demo, 0 if args.no_cpu else 2)

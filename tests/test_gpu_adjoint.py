@@ -122,6 +122,7 @@ def test_particle_adjoints_match_oracle(integrator, kernel):
     p = xf.default_params(integrator=integrator, p2g_kernel=kernel)
     s = xf.FluidSimulator(o.ni, o.nj, o.nk, p)
     s.setCellWidth(o.dx)
+    s.set_field(FIELD["MATERIAL"], o.get(FIELD["MATERIAL"]))  # (the advection's collision branch reads it)
     F = faces_of(o)
     for name, a in zip(("U", "V", "W"), F):
         s.set_field(FIELD[name], a)

@@ -268,9 +268,9 @@ def test_differentiable_projection_workload_small(env):
         boxes.append(((x >= 0.4) & (x <= 0.6) & (y >= 0.4) & (y <= 0.6) & (z >= 0.4) & (z <= 0.6)).astype(np.float64))
     Bd = [T(b) for b in boxes]
 
-    def loss_of(U, grad=False):
+    def loss_of(U, grad=False, log=None):
         leaves = [T(a).requires_grad_(grad) for a in U]
-        step = lambda X, V, u, v, w: ag.substep(g, dt, X, V, u, v, w)  # noqa: E731
+        step = lambda X, V, u, v, w: ag.substep(g, dt, X, V, u, v, w, material_log=log)  # noqa: E731
         _, _, u, v, w = ag.rollout(step, (X0, V0, *leaves), steps, every)
         loss = 0.5 * sum((b * t * t).sum() for b, t in zip(Bd, (u, v, w)))
         if grad:
@@ -279,19 +279,35 @@ def test_differentiable_projection_workload_small(env):
         return loss.item()
 
     L0, grad = loss_of(U0h, True)
+    with torch.no_grad():
+        mats0 = []
+        loss_of(U0h, log=mats0)
     (_, _, Uo), (_, _, gU, _) = diff_chain.rollout_grad(
         o, dt, X0h, V0h, U0h, steps, lambda X, V, U: (np.zeros_like(X), np.zeros_like(V), [b * u for b, u in zip(boxes, U)]))
     err_cpu = _rel(grad, gU)
     say(f"[diff] loss {L0:.6e}; gradient vs FP64 CPU adjoint {err_cpu:.2e}")
     assert err_cpu <= 1e-8
+    # central differences along seeded directions; a direction whose +eps or
+    # -eps run changes the material of some substep (a cell gains its first or
+    # loses its last particle) crosses a discontinuity the gradient does not
+    # see, and is reported as an outlier rather than compared
     rng = np.random.default_rng(3)
     eps = 1e-4
-    worst = 0.0
+    errs, outliers = [], 0
     for _ in range(8):
         d = [rng.standard_normal(a.shape) for a in U0h]
+        mp, mm = [], []
         with torch.no_grad():
-            fd = (loss_of([a + eps * b for a, b in zip(U0h, d)]) - loss_of([a - eps * b for a, b in zip(U0h, d)])) / (2 * eps)
+            lp = loss_of([a + eps * b for a, b in zip(U0h, d)], log=mp)
+            lm = loss_of([a - eps * b for a, b in zip(U0h, d)], log=mm)
+        flips = sum(int((a != b).sum()) for a, b in zip(mp + mm, mats0 + mats0))
+        fd = (lp - lm) / (2 * eps)
         an = sum(float(np.sum(a * b)) for a, b in zip(grad, d))
-        worst = max(worst, abs(fd - an) / abs(an))
-    say(f"[diff] central differences (eps 1e-4, 8 directions): worst relative error {worst:.2e}")
-    assert worst <= 1e-4
+        if flips:
+            outliers += 1
+            say(f"[diff]   direction with {flips} material changes: fd {fd:.6e}, adjoint {an:.6e} (outlier)")
+            continue
+        errs.append(abs(fd - an) / abs(an))
+    say(f"[diff] central differences (eps 1e-4): {len(errs)} directions, worst relative error {max(errs):.2e}, "
+        f"{outliers} outliers")
+    assert len(errs) >= 6 and max(errs) <= 1e-4

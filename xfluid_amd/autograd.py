@@ -118,14 +118,16 @@ def _forward_to_extrapolation(sim, dt, X, V, U, force):
 
 class _Substep(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, sim, dt, X, V, u, v, w, fu, fv, fw):
+    def forward(ctx, sim, dt, log, X, V, u, v, w, fu, fv, fw):
         p = sim.params
         if p.tail_skip:
             raise ValueError("substep: set tail_skip = 0 (every particle moves)")
         force = None if fu is None else (fu, fv, fw)
         X, V = X.detach().contiguous(), V.detach().contiguous()
         U = [t.detach().contiguous() for t in (u, v, w)]
-        _forward_to_extrapolation(sim, dt, X, V, U, force)
+        mat, _ = _forward_to_extrapolation(sim, dt, X, V, U, force)
+        if log is not None:
+            log.append(mat)
         sim.stage(STAGE["ADVECT"], dt)
         sim.stage(STAGE["P2G"])
         Xn, Vn = torch.empty_like(X), torch.empty_like(V)
@@ -184,15 +186,17 @@ class _Substep(torch.autograd.Function):
         if force is not None:
             gF = [dt * m * g for m, g in zip(masks, gU3)]
         gUin = [a + b for a, b in zip(gU3, gUs)]
-        return (None, None, gX, gV, gUin[0], gUin[1], gUin[2], gF[0], gF[1], gF[2])
+        return (None, None, None, gX, gV, gUin[0], gUin[1], gUin[2], gF[0], gF[1], gF[2])
 
 
-def substep(sim, dt, X, V, u, v, w, force=None):
+def substep(sim, dt, X, V, u, v, w, force=None, material_log=None):
     """One differentiable substep; returns (X', V', u', v', w').  `force` is an
     optional triple of per-face accelerations (added times dt on the faces next
-    to fluid, after the sim's constant forces)."""
+    to fluid, after the sim's constant forces).  `material_log`, a list, gets
+    the substep's material (uint8 device tensor) appended: the piecewise-
+    constant part of the step that the gradient holds fixed."""
     f = (None, None, None) if force is None else force
-    return _Substep.apply(sim, dt, X, V, u, v, w, *f)
+    return _Substep.apply(sim, dt, material_log, X, V, u, v, w, *f)
 
 
 class _Density(torch.autograd.Function):
