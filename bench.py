@@ -327,24 +327,32 @@ def run_diff_projection(xf, n, steps, every, dirs, eps):
     t_fb = time.perf_counter() - t0
     gnorm = float(torch.sqrt(sum((g * g).sum() for g in grad)))
     gen = torch.Generator(device="cuda").manual_seed(3)
-    errs, outliers = [], []
-    for q in range(dirs):
-        d = [torch.randn(t.shape, generator=gen, dtype=torch.float64, device="cuda") for t in U0]
-        mp, mm = [], []
-        with torch.no_grad():
-            lp = loss_of([a + eps * b for a, b in zip(U0, d)], mats=mp)
-            lm = loss_of([a - eps * b for a, b in zip(U0, d)], mats=mm)
-        flips = sum(int((a != b).sum()) for a, b in zip(mp + mm, mats0 + mats0))
-        fd = (lp - lm) / (2 * eps)
-        an = float(sum((g * b).sum() for g, b in zip(grad, d)))
-        rel = abs(fd - an) / abs(an)
-        if flips:
-            outliers.append({"direction": q, "material_changes": flips, "rel_err": rel})
-        else:
-            errs.append(rel)
-        if q % 8 == 7:
-            log(f"[diff-projection] {q + 1} directions: worst {max(errs) if errs else 0:.2e}, "
-                f"{len(outliers)} outliers")
+
+    def fd_pass(eps_, ndirs):
+        """(relative errors, errors / |grad|) of the directions without material
+        change, and the outliers"""
+        rel_, nrm_, outl = [], [], []
+        for q in range(ndirs):
+            d = [torch.randn(t.shape, generator=gen, dtype=torch.float64, device="cuda") for t in U0]
+            mp, mm = [], []
+            with torch.no_grad():
+                lp = loss_of([a + eps_ * b for a, b in zip(U0, d)], mats=mp)
+                lm = loss_of([a - eps_ * b for a, b in zip(U0, d)], mats=mm)
+            flips = sum(int((a != b).sum()) for a, b in zip(mp + mm, mats0 + mats0))
+            fd = (lp - lm) / (2 * eps_)
+            an = float(sum((g * b).sum() for g, b in zip(grad, d)))
+            if flips:
+                outl.append({"direction": q, "material_changes": flips, "rel_err": abs(fd - an) / abs(an)})
+            else:
+                rel_.append(abs(fd - an) / abs(an))
+                nrm_.append(abs(fd - an) / gnorm)
+            if q % 8 == 7:
+                log(f"[diff-projection] eps {eps_:g}: {q + 1} directions, worst {max(rel_) if rel_ else 0:.2e}, "
+                    f"{len(outl)} outliers")
+        return rel_, nrm_, outl
+
+    errs, norm_errs, outliers = fd_pass(eps, dirs)
+    small = fd_pass(1e-6, max(dirs // 4, 1))
     errs_sorted = sorted(errs)
     print(json.dumps({
         "metric": f"differentiable projection {n}^3: adjoint gradient vs central differences (eps {eps:g}, "
@@ -356,7 +364,11 @@ def run_diff_projection(xf, n, steps, every, dirs, eps):
                    "grid": [n, n, n], "particles": npart, "substeps": steps, "checkpoint_every": every},
         "loss": L0, "grad_norm": gnorm,
         "median_rel_err": errs_sorted[len(errs_sorted) // 2] if errs else None,
+        "max_err_over_grad_norm": max(norm_errs) if norm_errs else None,
         "directions_compared": len(errs), "outliers": outliers,
+        "eps_1e-6": {"directions_compared": len(small[0]), "max_rel_err": max(small[0]) if small[0] else None,
+                     "median_rel_err": sorted(small[0])[len(small[0]) // 2] if small[0] else None,
+                     "outliers": len(small[2])},
         "forward_s": round(t_fwd, 3), "forward_backward_s": round(t_fb, 3),
         "substeps_per_s_forward": round(steps / t_fwd, 2),
         "substeps_per_s_forward_backward": round(steps / t_fb, 2)}), flush=True)
