@@ -268,9 +268,11 @@ def test_differentiable_projection_workload_small(env):
         boxes.append(((x >= 0.4) & (x <= 0.6) & (y >= 0.4) & (y <= 0.6) & (z >= 0.4) & (z <= 0.6)).astype(np.float64))
     Bd = [T(b) for b in boxes]
 
-    def loss_of(U, grad=False, log=None):
+    def loss_of(U, grad=False, log=None, frozen=None):
         leaves = [T(a).requires_grad_(grad) for a in U]
-        step = lambda X, V, u, v, w: ag.substep(g, dt, X, V, u, v, w, material_log=log)  # noqa: E731
+        mats = iter(frozen) if frozen is not None else None
+        step = lambda X, V, u, v, w: ag.substep(g, dt, X, V, u, v, w, material_log=log,  # noqa: E731
+                                                material=next(mats) if mats is not None else None)
         _, _, u, v, w = ag.rollout(step, (X0, V0, *leaves), steps, every)
         loss = 0.5 * sum((b * t * t).sum() for b, t in zip(Bd, (u, v, w)))
         if grad:
@@ -287,37 +289,42 @@ def test_differentiable_projection_workload_small(env):
     err_cpu = _rel(grad, gU)
     say(f"[diff] loss {L0:.6e}; gradient vs FP64 CPU adjoint {err_cpu:.2e}")
     assert err_cpu <= 1e-8
-    # central differences along seeded directions at eps = 1e-4 and 1e-6; a
-    # direction whose +eps or -eps run changes the material of some substep (a
-    # cell gains its first or loses its last particle) crosses a discontinuity
-    # the gradient does not see, and is reported as an outlier rather than
-    # compared.  The rest differ by the kinks of trilinear sampling (a particle
-    # crossing a cell face between the two runs), which shrink with eps.
+    # central differences along seeded directions.  At eps = 1e-4 with each
+    # substep's material held at the unperturbed run's (the branch of the step
+    # the gradient differentiates); and at eps = 1e-4 and 1e-6 with the
+    # material the particles make, where a direction whose +eps or -eps run
+    # changes some substep's material (a cell gains its first or loses its last
+    # particle: a jump of the loss) is an outlier, not compared.  The rest differ
+    # by the kinks of trilinear sampling (a particle crossing a cell face between
+    # the two runs), which shrink with eps.
     rng = np.random.default_rng(3)
     gnorm = float(np.sqrt(sum(np.sum(a * a) for a in grad)))
-    res = {1e-4: [], 1e-6: []}
+    res = {"frozen 1e-4": [], "1e-4": [], "1e-6": []}
     outliers = 0
     for _ in range(8):
         d = [rng.standard_normal(a.shape) for a in U0h]
         an = sum(float(np.sum(a * b)) for a, b in zip(grad, d))
         row, flips = {}, 0
-        for eps in res:
+        for key, eps in (("frozen 1e-4", 1e-4), ("1e-4", 1e-4), ("1e-6", 1e-6)):
             mp, mm = [], []
+            frozen = mats0 if key.startswith("frozen") else None
             with torch.no_grad():
-                lp = loss_of([a + eps * b for a, b in zip(U0h, d)], log=mp)
-                lm = loss_of([a - eps * b for a, b in zip(U0h, d)], log=mm)
-            flips += sum(int((a != b).sum()) for a, b in zip(mp + mm, mats0 + mats0))
-            row[eps] = (lp - lm) / (2 * eps)
+                lp = loss_of([a + eps * b for a, b in zip(U0h, d)], log=mp, frozen=frozen)
+                lm = loss_of([a - eps * b for a, b in zip(U0h, d)], log=mm, frozen=frozen)
+            if frozen is None:
+                flips += sum(int((a != b).sum()) for a, b in zip(mp + mm, mats0 + mats0))
+            row[key] = (lp - lm) / (2 * eps)
+        res["frozen 1e-4"].append((abs(row["frozen 1e-4"] - an) / abs(an), abs(row["frozen 1e-4"] - an) / gnorm))
         if flips:
             outliers += 1
-            say(f"[diff]   direction with {flips} material changes: fd {row[1e-4]:.6e}, adjoint {an:.6e} (outlier)")
+            say(f"[diff]   direction with {flips} material changes: fd {row['1e-4']:.6e}, adjoint {an:.6e} (outlier)")
             continue
-        for eps, fd in row.items():
-            res[eps].append((abs(fd - an) / abs(an), abs(fd - an) / gnorm))
-    for eps, r in res.items():
-        say(f"[diff] central differences eps {eps:g}: {len(r)} directions, worst relative error "
+        for key in ("1e-4", "1e-6"):
+            res[key].append((abs(row[key] - an) / abs(an), abs(row[key] - an) / gnorm))
+    for key, r in res.items():
+        say(f"[diff] central differences {key}: {len(r)} directions, worst relative error "
             f"{max(x[0] for x in r):.2e}, worst error / |grad| {max(x[1] for x in r):.2e}")
     say(f"[diff] {outliers} outlier directions")
-    assert len(res[1e-4]) >= 5
-    assert max(x[0] for x in res[1e-6]) <= 1e-4
-    assert max(x[1] for x in res[1e-4]) <= 1e-4 and max(x[0] for x in res[1e-4]) <= 1e-3
+    assert len(res["1e-6"]) >= 5
+    assert max(x[0] for x in res["1e-6"]) <= 1e-4
+    assert max(x[0] for x in res["frozen 1e-4"]) <= 1e-3

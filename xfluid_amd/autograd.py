@@ -14,8 +14,9 @@ parameters, and its fields are overwritten by every call.
   the particles, save, forces (the sim's constant forces, plus `force`, an
   optional per-face acceleration on the faces next to fluid), projection,
   extrapolation, G2P FLIP/PIC blend, RK4/RK2 advection with collisions, P2G.
-  The forward runs the library's own stages; the backward recomputes them from
-  the saved inputs and runs the adjoint kernels in reverse (P2G, advection with
+  The forward runs the library's own stages and keeps the substep's material,
+  extrapolated field and moved particles; the backward puts that state back
+  into the sim and runs the adjoint kernels in reverse (P2G, advection with
   its collision branch, G2P sampling of the new and the saved field,
   extrapolation, projection).  Material changes and the P2G normalisation's
   cut-off are piecewise constant and not differentiated.
@@ -87,18 +88,23 @@ def _faces_like(sim, device):
     return [torch.zeros(s, dtype=_F64, device=device) for s in ((nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni))]
 
 
-def _forward_to_extrapolation(sim, dt, X, V, U, force):
-    """Runs material .. extrapolation on the sim from the given state; returns
-    the material and the face masks (masks only when a force is given)."""
+def _forward_to_extrapolation(sim, dt, X, V, U, force, material=None):
+    """Runs material .. extrapolation on the sim from the given state (the
+    material from the particles, or `material` when given); returns the
+    material and the face masks (masks only when a force is given)."""
     _sync()
     sim.set_particles_device(X, V)
     for name, t in zip(("U", "V", "W"), U):
         sim.set_field_device(FIELD[name], t)
-    sim.stage(STAGE["MATERIAL"])
+    if material is None:
+        sim.stage(STAGE["MATERIAL"])
+        mat = torch.empty((sim.nk, sim.nj, sim.ni), dtype=torch.uint8, device=X.device)
+        sim.get_field_device(FIELD["MATERIAL"], mat)
+    else:
+        mat = material.contiguous()
+        sim.set_field_device(FIELD["MATERIAL"], mat)
     sim.stage(STAGE["SAVE"])
     sim.stage(STAGE["FORCES"], dt)
-    mat = torch.empty((sim.nk, sim.nj, sim.ni), dtype=torch.uint8, device=X.device)
-    sim.get_field_device(FIELD["MATERIAL"], mat)
     masks = None
     if force is not None:
         masks = face_masks(mat)
@@ -118,16 +124,19 @@ def _forward_to_extrapolation(sim, dt, X, V, U, force):
 
 class _Substep(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, sim, dt, log, X, V, u, v, w, fu, fv, fw):
+    def forward(ctx, sim, dt, log, material, X, V, u, v, w, fu, fv, fw):
         p = sim.params
         if p.tail_skip:
             raise ValueError("substep: set tail_skip = 0 (every particle moves)")
         force = None if fu is None else (fu, fv, fw)
         X, V = X.detach().contiguous(), V.detach().contiguous()
         U = [t.detach().contiguous() for t in (u, v, w)]
-        mat, _ = _forward_to_extrapolation(sim, dt, X, V, U, force)
+        mat, masks = _forward_to_extrapolation(sim, dt, X, V, U, force, material)
         if log is not None:
             log.append(mat)
+        U3 = _faces_like(sim, X.device)  # the extrapolated field the particles move in
+        for name, t in zip(("U", "V", "W"), U3):
+            sim.get_field_device(FIELD[name], t)
         sim.stage(STAGE["ADVECT"], dt)
         sim.stage(STAGE["P2G"])
         Xn, Vn = torch.empty_like(X), torch.empty_like(V)
@@ -136,31 +145,36 @@ class _Substep(torch.autograd.Function):
         for name, t in zip(("U", "V", "W"), Un):
             sim.get_field_device(FIELD[name], t)
         ctx.sim, ctx.dt, ctx.has_force = sim, dt, force is not None
-        ctx.save_for_backward(X, V, *U, *(force if force is not None else ()))
+        # what the backward needs besides the inputs: the material, the
+        # extrapolated field and the moved particles (P2G does not move them)
+        ctx.save_for_backward(X, V, *U, mat, *U3, Xn, Vn, *(masks if masks is not None else ()))
         return Xn, Vn, Un[0], Un[1], Un[2]
 
     @staticmethod
     def backward(ctx, gXn, gVn, gun, gvn, gwn):
         sim, dt = ctx.sim, ctx.dt
         saved = ctx.saved_tensors
-        X, V, U = saved[0], saved[1], list(saved[2:5])
-        force = tuple(saved[5:8]) if ctx.has_force else None
+        X, V, U, mat = saved[0], saved[1], list(saved[2:5]), saved[5]
+        U3, Xa, Va = list(saved[6:9]), saved[9], saved[10]
+        masks = list(saved[11:14]) if ctx.has_force else None
         dev = X.device
         z = lambda t, like: torch.zeros_like(like) if t is None else t.detach().contiguous()  # noqa: E731
         gXn, gVn = z(gXn, X), z(gVn, V)
         gU = [z(g, t) for g, t in zip((gun, gvn, gwn), U)]
-        # recompute the forward up to the new particles (the sim then holds the
-        # extrapolated field, the saved field and the material of this substep)
-        _mat, masks = _forward_to_extrapolation(sim, dt, X, V, U, force)
-        sim.stage(STAGE["ADVECT"], dt)
-        Xa, Va = torch.empty_like(X), torch.empty_like(V)
-        sim.get_particles_device(Xa, Va)
+        # the sim's state of this substep: material, extrapolated field, saved field
+        _sync()
+        sim.set_field_device(FIELD["MATERIAL"], mat)
+        for name, t in zip(("U", "V", "W"), U3):
+            sim.set_field_device(FIELD[name], t)
+        for name, t in zip(("USAVED", "VSAVED", "WSAVED"), U):
+            sim.set_field_device(FIELD[name], t)
         # P2G: U' = P2G(X', V')
         gpos = gXn.clone()
         gvel = torch.zeros_like(V)
         ni, nj, nk = sim.ni, sim.nj, sim.nk
         faces = max((ni + 1) * nj * nk, ni * (nj + 1) * nk, ni * nj * (nk + 1))
         p2g_adj = sim.p2g_adjoint_noalloc if faces <= NOALLOC_MAX_FACES else sim.p2g_adjoint
+        _sync()
         p2g_adj(Xa, Va, *gU, gvel, gpos)
         gV_new = gVn + gvel
         # advection: X' = move(X; U3)
@@ -179,24 +193,26 @@ class _Substep(torch.autograd.Function):
             g_old = (-r * gV_new).contiguous()
             _sync()
             sim.sample_adjoint(X, g_old, gX, *gUs, saved=True)
-        # extrapolation and projection of this substep's material (still in the sim)
+        # extrapolation and projection of this substep's material (set above)
         sim.extrapolate_adjoint(*gU3)
         _check(sim.project_adjoint(*gU3), "substep adjoint")
         gF = [None, None, None]
-        if force is not None:
+        if masks is not None:
             gF = [dt * m * g for m, g in zip(masks, gU3)]
         gUin = [a + b for a, b in zip(gU3, gUs)]
-        return (None, None, None, gX, gV, gUin[0], gUin[1], gUin[2], gF[0], gF[1], gF[2])
+        return (None, None, None, None, gX, gV, gUin[0], gUin[1], gUin[2], gF[0], gF[1], gF[2])
 
 
-def substep(sim, dt, X, V, u, v, w, force=None, material_log=None):
+def substep(sim, dt, X, V, u, v, w, force=None, material_log=None, material=None):
     """One differentiable substep; returns (X', V', u', v', w').  `force` is an
     optional triple of per-face accelerations (added times dt on the faces next
     to fluid, after the sim's constant forces).  `material_log`, a list, gets
     the substep's material (uint8 device tensor) appended: the piecewise-
-    constant part of the step that the gradient holds fixed."""
+    constant part of the step that the gradient holds fixed.  `material`, when
+    given, is used as this substep's material instead of the one the particles
+    make (the branch of the step the gradient differentiates)."""
     f = (None, None, None) if force is None else force
-    return _Substep.apply(sim, dt, material_log, X, V, u, v, w, *f)
+    return _Substep.apply(sim, dt, material_log, material, X, V, u, v, w, *f)
 
 
 class _Density(torch.autograd.Function):
