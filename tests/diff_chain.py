@@ -76,18 +76,23 @@ def substep_vjp(o, dt, X, V, U, force, gXn, gVn, gUn):
     return gX, gV_new * r, [a + b for a, b in zip(gU3, gUs)], gF
 
 
-def rollout_grad(o, dt, X, V, U, steps, loss_grad, force=None):
+def rollout_grad(o, dt, X, V, U, steps, loss_grad, force=None, progress=None):
     """Runs `steps` substeps from (X, V, U), then the VJP chain backwards with
     the gradient loss_grad(Xn, Vn, Un) -> (gX, gV, gU) of the loss on the final
-    state.  Returns the final state and (gX0, gV0, gU0, gF)."""
+    state.  Returns the final state and (gX0, gV0, gU0, gF).  progress(msg), if
+    given, is called after every substep of either pass."""
     states = [(X, V, U)]
-    for _ in range(steps):
+    for t in range(steps):
         states.append(substep(o, dt, *states[-1], force))
+        if progress:
+            progress(f"forward substep {t + 1}/{steps}")
     g = loss_grad(*states[-1])
     gF = None if force is None else [np.zeros_like(a) for a in force]
     for t in range(steps - 1, -1, -1):
         gX, gV, gU, gf = substep_vjp(o, dt, *states[t], force, *g)
         g = (gX, gV, gU)
+        if progress:
+            progress(f"backward substep {t + 1}")
         if gF is not None:
             gF = [a + b for a, b in zip(gF, gf)]
     return states[-1], (g[0], g[1], g[2], gF)

@@ -51,7 +51,8 @@ def test_full_size_gradient_matches_cpu_adjoint():
     grad = [t.grad.cpu().numpy() for t in leaves]
     t0 = time.time()
     (_, _, Uo), (_, _, gU, _) = diff_chain.rollout_grad(
-        o, dt, X0h, V0h, U0h, steps, lambda X, V, U: (np.zeros_like(X), np.zeros_like(V), [b * q for b, q in zip(boxes, U)]))
+        o, dt, X0h, V0h, U0h, steps, lambda X, V, U: (np.zeros_like(X), np.zeros_like(V), [b * q for b, q in zip(boxes, U)]),
+        progress=lambda m: say(f"[full] CPU adjoint: {m} ({time.time() - t0:.0f} s)"))
     Lo = 0.5 * sum(float(np.sum(b * q * q)) for b, q in zip(boxes, Uo))
     num = max(float(np.abs(a - b).max()) for a, b in zip(grad, gU))
     den = max(float(np.abs(b).max()) for b in gU)
