@@ -315,73 +315,19 @@ void launch_source(const Geo& g, Particles& P, uint8_t* mat, const int32_t* cell
 }
 
 // ---------------------------------------------------------------------------
-// P2G, _updateVelocityByParticles (FluidSimulator.cpp:909-1085), as a gather:
-// face f = sum_p W(|q_p - c_f|^2) v_p / sum_p W, q_p = p + dx/2 e_axis,
-// c_f = idx2Pos(f) -- the same per-pair arithmetic as the reference; only the
-// summation order differs (cells in k,j,i order, particles by index).
-// Faces with sum W <= 1e-7 get 0.  Candidate cells span the kernel support.
-__global__ void k_p2g(Geo g, int axis, int kern, const double* __restrict__ px, const double* __restrict__ py,
-                      const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
-                      double* __restrict__ out) {
-    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
-    long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    long nf = (long)si * sj * sk;
-    if (t >= nf) return;
-    int fi = (int)(t % si), fj = (int)((t / si) % sj), fk = (int)(t / ((long)si * sj));
-    const double dx = g.dx;
-    const bool tri = kern == 1;
-    const double r = tri ? dx : 1.5 * dx;
-    const Wyvill W = make_wyvill(r);
-    // support in cells: Wyvill r=1.5dx -> [f-2, f+1] along the axis, [f-1, f+1] across;
-    // trilinear r=dx -> [f-1, f] along, [f-1, f+1] across
-    int lo[3], hi[3];
-    int fc[3] = {fi, fj, fk};
-    for (int a = 0; a < 3; ++a) {
-        if (a == axis) { lo[a] = fc[a] - (tri ? 1 : 2); hi[a] = fc[a] + (tri ? 0 : 1); }
-        else { lo[a] = fc[a] - 1; hi[a] = fc[a] + 1; }
-    }
-    if (lo[0] < 0) lo[0] = 0;
-    if (lo[1] < 0) lo[1] = 0;
-    if (lo[2] < 0) lo[2] = 0;
-    if (hi[0] > g.ni - 1) hi[0] = g.ni - 1;
-    if (hi[1] > g.nj - 1) hi[1] = g.nj - 1;
-    if (hi[2] > g.nk - 1) hi[2] = g.nk - 1;
-    const double cx = idx2pos(fi, dx), cy = idx2pos(fj, dx), cz = idx2pos(fk, dx);
-    const double sx = axis == 0 ? 0.5 * dx : 0.0, sy = axis == 1 ? 0.5 * dx : 0.0, sz = axis == 2 ? 0.5 * dx : 0.0;
-    double wsum = 0.0, vsum = 0.0;
-    for (int k = lo[2]; k <= hi[2]; ++k)
-        for (int j = lo[1]; j <= hi[1]; ++j) {
-            long row = (long)g.ni * j + g.nn * k;
-            int a = start[row + lo[0]], b = start[row + hi[0] + 1];
-            for (int p = a; p < b; ++p) {
-                double ex = (px[p] + sx) - cx, ey = (py[p] + sy) - cy, ez = (pz[p] + sz) - cz;
-                double cw;
-                if (tri) {
-                    double ax = fabs(ex) / dx, ay = fabs(ey) / dx, az = fabs(ez) / dx;
-                    cw = (ax < 1.0 && ay < 1.0 && az < 1.0) ? (1.0 - ax) * (1.0 - ay) * (1.0 - az) : 0.0;
-                } else {
-                    double s = ex * ex + ey * ey + ez * ez;
-                    cw = wyvill(W, s);
-                }
-                if (cw != 0.0) {
-                    wsum += cw;
-                    vsum += vc[p] * cw;
-                }
-            }
-        }
-    out[t] = wsum > 0.0000001 ? vsum / wsum : 0.0;
-}
-
-// The same gather over bricks of 8x8x4 faces.  A workgroup first reads the
-// cell starts of all its candidate slabs (one round of loads), then walks the
-// k-slabs that hold particles: it copies the slab's particles (one contiguous
-// run per cell row of the sorted arrays) into LDS with several loads in flight
-// per thread, then every face adds the slab's rows inside its support, reading
-// particles from LDS.  Each face still visits cells in k, j, i order and
-// particles by index, with the per-pair arithmetic of k_p2g, so the result is
-// bit-identical to it; each particle is fetched once per brick instead of once
-// per face, and bricks in air cost one round of loads.  A slab with more
-// particles than fit is read from global memory in the same order.
+// P2G, _updateVelocityByParticles (FluidSimulator.cpp:909-1085), as a gather
+// over bricks of 8x8x4 faces: face f = sum_p W(|q_p - c_f|^2) v_p / sum_p W,
+// q_p = p + dx/2 e_axis, c_f = idx2Pos(f); faces with sum W <= 1e-7 get 0.  A
+// workgroup first reads the cell starts of all its candidate slabs (one round
+// of loads), then walks the k-slabs that hold particles: it copies the slab's
+// particles (one contiguous run per cell row of the sorted arrays) into LDS
+// with several loads in flight per thread, then every face adds the slab's
+// rows inside its support, reading particles from LDS, cells in k, j, i order
+// and particles by index.  Each particle is fetched once per brick instead of
+// once per face, and bricks in air cost one round of loads.  A slab with more
+// particles than fit is read from global memory in the same order.  Only the
+// summation order differs from the reference's, so the result equals it to
+// rounding (the weights may use fused multiply-adds).
 #ifndef P2G_BJ
 #define P2G_BJ 8
 #endif
@@ -397,6 +343,7 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     Geo g, int axis, int kern, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
     const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
     double* __restrict__ out) {
+#pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
     typedef double d2 __attribute__((ext_vector_type(2)));
     __shared__ d2 lp[2 * P2G_CAP];  // slab particle q: {x, y} at 2q, {z, v} at 2q + 1
     __shared__ int raw[P2G_BK + 3][P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slabs' cells, per row
@@ -533,6 +480,7 @@ template <int D>
 __global__ void k_density(Geo g, const double* __restrict__ px, const double* __restrict__ py,
                           const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi,
                           double R) {
+#pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
     const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
     long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n1) return;
