@@ -174,6 +174,17 @@ __global__ void k_fwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     long c = cidx(g, i, j, k);
     uint16_t cc = code[c];
     T v = r[c];
+    if constexpr (sizeof(T) == 4) {
+        // FP32: the 8x8-tile sweep's order (-i, -j, -k entries, fused multiply-adds)
+        int cl = code_dir(cc, D_XM);
+        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - 1]), q[c - 1], v);
+        cl = code_dir(cc, D_YM);
+        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - g.ni]), q[c - g.ni], v);
+        cl = code_dir(cc, D_ZM);
+        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - g.nn]), q[c - g.nn], v);
+        q[c] = v * inv[c];
+        return;
+    }
     int cl = code_dir(cc, D_ZM);
     if (present(cl)) v = v - ((T)code_coef(cl) * inv[c - g.nn]) * q[c - g.nn];
     cl = code_dir(cc, D_YM);
@@ -197,6 +208,17 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     uint16_t cc = code[c];
     T ic = inv[c];
     T v = q[c];
+    if constexpr (sizeof(T) == 4) {
+        // FP32: fused multiply-adds, as the 8x8-tile sweep
+        int cl = code_dir(cc, D_XP);
+        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + 1], v);
+        cl = code_dir(cc, D_YP);
+        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + g.ni], v);
+        cl = code_dir(cc, D_ZP);
+        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + g.nn], v);
+        z[c] = v * ic;
+        return;
+    }
     int cl = code_dir(cc, D_XP);
     if (present(cl)) v = v - ((T)code_coef(cl) * ic) * z[c + 1];
     cl = code_dir(cc, D_YP);

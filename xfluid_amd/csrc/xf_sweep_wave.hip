@@ -50,6 +50,21 @@ __device__ inline void wv_tile(int b, int ntj, int ntk, bool fwd, int* J, int* K
     if (!fwd) { *J = ntj - 1 - *J; *K = ntk - 1 - *K; }
 }
 
+// Tile of workgroup b in the fused kernels, whose tiles are all resident at
+// once, so any one-to-one map is valid.  Workgroups are dispatched to the 8
+// XCDs in turn (b % 8); each XCD gets a compact block of tiles (bands of 4
+// k-rows walked j by j, cut into 8 consecutive runs), so most tile-to-tile
+// hand-offs stay within an XCD, whose device-coherent round trips are shorter.
+__device__ inline void w8_tile_xcd(int b, int ntj, int ntk, int* J, int* K) {
+    const int tiles = ntj * ntk, q = tiles >> 3, r = tiles & 7, x = b & 7, m = b >> 3;
+    const int p = x < r ? x * (q + 1) + m : r * (q + 1) + (x - r) * q + m;
+    constexpr int H = 4;
+    const int band = p / (H * ntj), k0 = band * H, hb = ntk - k0 < H ? ntk - k0 : H;
+    const int w = p - band * H * ntj;
+    *J = w / hb;
+    *K = k0 + w % hb;
+}
+
 template <int N>
 __device__ inline void vm_wait() {
     static_assert(N >= 0 && N < 64, "vmcnt field");
@@ -306,9 +321,14 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
 // tile's last 7 steps wait for those, for cells outside the box whose entries
 // are absent, so every step checks its edge words the same way.
 //
-// The ring holds R steps, the records of LEAD steps in flight (32 and 24 or
-// 20 for the sweeps: some 4 us of lead at 0.16 us a step; 16 and 12 for the
-// pivots, whose records are twice as long).  The compute wave runs in blocks
+// The ring holds R steps, the records of LEAD steps in flight (32 and 24 for
+// the FP32 sweeps, 32 and 16 for FP64, 16 and 12 for the pivots, whose records
+// are twice as long).  Record loads take some 4-8 us under the sweep's load, so
+// with 32 slots the record wave, which may run at most R - 1 steps ahead of the
+// compute wave, holds an FP32 tile back to about 0.2 us a step.  The deep ring
+// (DEEP: 64 slots, 31 steps in flight, the most one wave's 6-bit vmcnt allows
+// at two loads a step) takes 99 KB of LDS, so one workgroup per CU; the host
+// uses it when every tile fits.  The compute wave runs in blocks
 // of W8_B steps unrolled so that every ring slot sits at a fixed LDS offset
 // from the block's base.  With one wave per SIMD a step costs about
 // its instruction count times the issue interval, so the step is kept short:
@@ -325,24 +345,25 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
 // cM the sum of the neighbour's coefficients that the fill-in drops, and
 // there is no input vector.
 // LDS ring of the 8x8-tile pipeline: depth in steps and bytes per step
-template <class T, bool MIC>
+template <class T, bool MIC, bool DEEP = false>
 struct W8Ring {
+    static_assert(!DEEP || (sizeof(T) == 4 && !MIC), "deep ring: FP32 sweeps");
     static constexpr int NS = sizeof(T) / 4;
-    static constexpr int R = MIC ? 16 : 32;
+    static constexpr int R = MIC ? 16 : DEEP ? 64 : 32;
     static constexpr int SLOT = 64 * (MIC ? 8 : 4) * (int)sizeof(T) + (MIC ? 0 : 64 * (int)sizeof(T)) + 2 * 16 * 8 * NS + 16;
 };
 
 // One sweep of tile (J, K) by the workgroup's three waves (ring_ and ctr_ are
 // the workgroup's LDS).
-template <class T, bool FWD, bool MIC>
+template <class T, bool FWD, bool MIC, bool DEEP = false>
 __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, int J, int K, int J0, int K0, int ntjb,
                                                                int ntkb, int s0, int nS, W8Planes<T> P,
                                                                uint32_t tagbase, int* err, double tau, double sigma,
                                                                unsigned char* ring_, int* ctr_) {
     static_assert(!MIC || (FWD && sizeof(T) == 8), "pivots: forward, FP64");
     constexpr int NS = sizeof(T) / 4;
-    constexpr int R = MIC ? 16 : 32;                      // ring depth in steps
-    constexpr int LEAD = MIC ? 12 : NS == 1 ? 24 : 16;    // steps of record loads in flight
+    constexpr int R = W8Ring<T, MIC, DEEP>::R;                      // ring depth in steps
+    constexpr int LEAD = MIC ? 12 : DEEP ? 31 : NS == 1 ? 24 : 16;  // steps of record loads in flight
     static_assert((R & (R - 1)) == 0 && R % W8_B == 0 && LEAD + 4 <= R, "ring depth");
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
@@ -358,7 +379,7 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
     constexpr bool NATIN = FWD && !MIC;
     constexpr int PERD = RECB / 1024 + (INB ? (NATIN ? NS : 1) : 0);  // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
-    static_assert(SLOT == W8Ring<T, MIC>::SLOT && R == W8Ring<T, MIC>::R, "ring layout");
+    static_assert(SLOT == W8Ring<T, MIC, DEEP>::SLOT, "ring layout");
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int SL = w8_slots(g), ER = w8_erows(g);
     const long myTile = (long)J * ntk + K;
@@ -616,7 +637,17 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
             d -= L * L;
             d -= (T)tau * L * miss;
             q = diag == 0 ? (T)0 : (d < (T)sigma * diag) ? 1 / sqrt(diag) : 1 / sqrt(d);
+        } else if constexpr (NS == 1) {
+            // FP32: fused multiply-adds, the own previous value first and the permuted
+            // k-neighbour, the last to arrive, last (k_fwd_plane / k_bwd_plane<float>
+            // use the same order, so the fallback gives the same bits)
+            const T Li = FWD ? cur.L2 : cur.L0, Lk = FWD ? cur.L0 : cur.L2;
+            v = wv_sel<T>(w8_present(Li), __builtin_fmaf(-Li, qp, v), v);
+            v = wv_sel<T>(w8_present(cur.L1), __builtin_fmaf(-cur.L1, qj, v), v);
+            v = wv_sel<T>(w8_present(Lk), __builtin_fmaf(-Lk, qk, v), v);
+            q = v * cur.ic;
         } else {
+            // FP64: the reference's order
             if (FWD) {
                 // column order c-nn, c-ni, c-1
                 w = v - cur.L0 * qk; v = wv_sel<T>(w8_present(cur.L0), w, v);
@@ -710,12 +741,12 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
 // begins the backward one, with no launch in between).  Every tile waits on
 // tiles that start later, so all tiles must be resident at once: the host
 // uses this kernel only when they fit (w8_fused_fits).
-template <class T>
+template <class T, bool DEEP>
 __global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0,
                                                      int nS, W8Planes<T> Pf, W8Planes<T> Pb, const PcgScalars* S,
                                                      uint32_t tag1, uint32_t tag2, int* err, const double* fin_partial,
                                                      int fin_n, int fin_it, PcgScalars* Sw, PcgScalars* mirror) {
-    __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, false>::R * W8Ring<T, false>::SLOT];
+    __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, false, DEEP>::R * W8Ring<T, false, DEEP>::SLOT];
     __shared__ int ctr_[2];
     if (fin_partial) {
         // the iteration's residual test (k_finish FIN_RES): every workgroup reduces the
@@ -754,12 +785,14 @@ __global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int
         return;
     }
     int J, K;
-    wv_tile(blockIdx.x, ntjb, ntkb, true, &J, &K);
-    w8_sweep<T, true, false>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_, ctr_);
+    w8_tile_xcd(blockIdx.x, ntjb, ntkb, &J, &K);
+    w8_sweep<T, true, false, DEEP>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_,
+                                   ctr_);
     // this tile's forward output is its backward input: stores complete, LDS free
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    w8_sweep<T, false, false>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pb, tag2, err, 0.0, 0.0, ring_, ctr_);
+    w8_sweep<T, false, false, DEEP>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pb, tag2, err, 0.0, 0.0, ring_,
+                                    ctr_);
 }
 
 // MIC(0) records (k_sweep_w8<double, true, true>): {cL0, cL1, cL2, diag, cM0, cM1,
@@ -847,15 +880,15 @@ void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const Sw
     k_w8_records<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, inv, sk.oRec);
 }
 
-// whether all `tiles` workgroups of k_sweep_w8_fb<T> can be resident at once
-template <class T>
+// whether all `tiles` workgroups of k_sweep_w8_fb<T, DEEP> can be resident at once
+template <class T, bool DEEP>
 static bool w8_fused_fits(int tiles) {
     static int cap = -1;
     if (cap < 0) {
         int dev = 0, cus = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep_w8_fb<T>, 192, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep_w8_fb<T, DEEP>, 192, 0) != hipSuccess)
             cus = per = 0;
         cap = cus * per;
     }
@@ -882,12 +915,15 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     const int tiles = ntjb * ntkb;
     // t0 / t1 ride on the dispatch packets themselves, so they time the kernels and not
     // the marker packets (and their cache release) that hipEventRecord would add
-    sk.fused = w8_fused_fits<T>(tiles) ? 1 : 0;
+    // (FP32: the deep ring when every tile gets a CU of its own)
+    const bool deep = sizeof(T) == 4 && w8_fused_fits<T, sizeof(T) == 4>(tiles);
+    sk.fused = deep || w8_fused_fits<T, false>(tiles) ? 1 : 0;
     if (sk.fused) {
         const double* fp = fin ? fin->partial : nullptr;
-        hipExtLaunchKernelGGL(k_sweep_w8_fb<T>, dim3(tiles), dim3(192), 0, st, t0, t1, 0, g, ntk, J0, K0, ntjb,
-                              ntkb, s0, nS, f, b, S, tag1, tag2, flags, fp, fin ? fin->n : 0, fin ? fin->it : 0,
-                              fin ? fin->S : (PcgScalars*)nullptr, fin ? fin->mirror : (PcgScalars*)nullptr);
+        hipExtLaunchKernelGGL(deep ? k_sweep_w8_fb<T, sizeof(T) == 4> : k_sweep_w8_fb<T, false>, dim3(tiles),
+                              dim3(192), 0, st, t0, t1, 0, g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, b, S, tag1, tag2,
+                              flags, fp, fin ? fin->n : 0, fin ? fin->it : 0, fin ? fin->S : (PcgScalars*)nullptr,
+                              fin ? fin->mirror : (PcgScalars*)nullptr);
         if (fin) fin->ran = true;
     } else {
         if (fin) {
