@@ -456,7 +456,7 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
               dalloc(&s->pw.inv, N) == hipSuccess && dalloc(&s->pw.x32, N) == hipSuccess &&
               dalloc(&s->pw.r32, N) == hipSuccess && dalloc(&s->pw.s32, N) == hipSuccess &&
               dalloc(&s->pw.z32, N) == hipSuccess && dalloc(&s->pw.q32, N) == hipSuccess &&
-              dalloc(&s->pw.inv32, N) == hipSuccess && dalloc(&s->pw.partial, 2 * RED_N) == hipSuccess &&
+              dalloc(&s->pw.inv32, N) == hipSuccess &&
               dalloc(&s->pw.S, 2) == hipSuccess && dalloc(&s->red, RED_N) == hipSuccess &&
               dalloc(&s->surf_cnt, N + 1) == hipSuccess && dalloc(&s->surf_start, N + 2) == hipSuccess;
     if (!ok) return fail(set_err(-2, "device allocation failed"));
@@ -469,10 +469,9 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
         SweepSkew<double>& a = s->pw.sk64;
         SweepSkew<float>& b = s->pw.sk32;
         const long W8 = sweep_w8_size(s->g), E8 = sweep_w8_edge_words(s->g);
-        if (dalloc(&a.oRec, 8 * W8) != hipSuccess || dalloc(&a.oR, W8) != hipSuccess || dalloc(&a.oQ, W8) != hipSuccess ||
-            dalloc(&a.oZ, W8) != hipSuccess || dalloc(&a.oEdge, E8) != hipSuccess || dalloc(&b.oRec, 8 * W8) != hipSuccess ||
-            dalloc(&b.oR, W8) != hipSuccess || dalloc(&b.oQ, W8) != hipSuccess || dalloc(&b.oZ, W8) != hipSuccess ||
-            dalloc(&b.oEdge, E8) != hipSuccess)
+        if (dalloc(&a.oRec, 8 * W8) != hipSuccess || dalloc(&a.oQ, 2 * W8) != hipSuccess ||
+            dalloc(&a.oEdge, E8) != hipSuccess || dalloc(&b.oRec, 8 * W8) != hipSuccess ||
+            dalloc(&b.oQ, 2 * W8) != hipSuccess || dalloc(&b.oEdge, E8) != hipSuccess)
             return fail(set_err(-2, "alloc sweep buffers"));
         // tags are never zero: zeroed edge words read as not yet written
         hipMemsetAsync(a.oEdge, 0, E8 * sizeof(uint32_t), s->st);
@@ -481,7 +480,13 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     hipMemsetAsync(s->pw.sweep_flags, 0, s->pw.nflags * sizeof(int), s->st);
     s->ci.scan_tmp_n = (int)scan_tmp_size(N + 2);
     if (dalloc(&s->ci.scan_tmp, s->ci.scan_tmp_n) != hipSuccess) return fail(set_err(-2, "alloc scan"));
-    s->pw.npartial = RED_N;
+    {
+        // partial halves: the reduction blocks, or the tiles of a whole-grid box
+        const int all[6] = {0, s->g.ni - 1, 0, s->g.nj - 1, 0, s->g.nk - 1};
+        const int tiles = w8_box_tiles(s->g, all);
+        s->pw.npartial = tiles > RED_N ? tiles : RED_N;
+        if (dalloc(&s->pw.partial, 2 * (long)s->pw.npartial) != hipSuccess) return fail(set_err(-2, "alloc partials"));
+    }
     if (hipHostMalloc((void**)&s->pw.S_host, 3 * sizeof(PcgScalars)) != hipSuccess ||
         hipEventCreateWithFlags(&s->pw.chunk_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s->pw.chunk_ev[1], hipEventDisableTiming) != hipSuccess ||
@@ -522,8 +527,8 @@ void xf_sim_destroy(xf_sim* s) {
     dfree(s->mat); dfree(s->code); dfree(s->marker); dfree(s->pw.S); dfree(s->pw.sweep_flags);
     dfree(s->pw.box_dev);
     if (s->pw.box_host) { hipHostFree(s->pw.box_host); s->pw.box_host = nullptr; }
-    dfree(s->pw.sk64.oRec); dfree(s->pw.sk64.oR); dfree(s->pw.sk64.oQ); dfree(s->pw.sk64.oZ); dfree(s->pw.sk64.oEdge);
-    dfree(s->pw.sk32.oRec); dfree(s->pw.sk32.oR); dfree(s->pw.sk32.oQ); dfree(s->pw.sk32.oZ); dfree(s->pw.sk32.oEdge);
+    dfree(s->pw.sk64.oRec); dfree(s->pw.sk64.oQ); dfree(s->pw.sk64.oEdge);
+    dfree(s->pw.sk32.oRec); dfree(s->pw.sk32.oQ); dfree(s->pw.sk32.oEdge);
     mesh_free(s->mesh);
     for (auto& src : s->sources) dfree(src.cells);
     dfree(s->ga_u); dfree(s->ga_v); dfree(s->ga_w); dfree(s->ga_p);

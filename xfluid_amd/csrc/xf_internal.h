@@ -44,12 +44,12 @@ void launch_extrapolate_adjoint(const Geo& g, const uint8_t* mat, double* gf, in
 
 // ---------------- pressure solve kernels (xf_pcg.hip) ----------------
 // Buffers of the pipelined 8x8-tile sweeps (xf_sweep_wave.hip), oct layout:
-// forward and backward records, input / forward / backward vectors, tagged edge
-// words.
+// forward and backward records, the forward output ((q, r) pairs: 2 values per
+// element), tagged edge words.
 template <class T>
 struct SweepSkew {
     T* oRec;
-    T *oR, *oQ, *oZ;
+    T* oQ;
     uint32_t* oEdge;
     mutable uint32_t w8_launch;  // launches on oEdge (their tags)
     mutable int fused;           // last application: 1 one fused forward + backward launch, 0 two launches
@@ -76,8 +76,8 @@ struct PcgWork {
     // FP32 copies for the mixed-precision inner solve
     float *x32, *r32, *s32, *z32, *q32, *inv32;
     float* s2_32;
-    double* partial;     // per-block partial sums / maxima
-    int npartial;
+    double* partial;     // per-block partial sums / maxima: two halves of npartial
+    int npartial;        // (at least the reduction blocks and the tiles of any box)
     PcgScalars* S;       // device scalars
     PcgScalars* S_host;  // pinned mirror: [0] inner solve, [1] outer, [2] inner (odd chunks)
     hipEvent_t chunk_ev[2];  // ends of the PCG's iteration chunks in flight
@@ -117,13 +117,19 @@ long sweep_w8_size(const Geo& g);
 long sweep_w8_edge_words(const Geo& g);
 template <class T>
 void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st);
+// z = M^-1 r over sk.box, z written at the box's tiles' cells (natural layout;
+// other cells untouched).  dot_partial: also the tiles' partials of dot(z, r),
+// w8_box_tiles of them, in w8_dot_tiles' order.
 template <class T>
-// dot_partial: also the block partials of dot(z, r) over dot_blocks blocks of
-// XF_BLOCK threads (k_dot's cells and order)
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st,
-                      double* dot_partial = nullptr, int dot_blocks = 0, const BoxRange* dot_range = nullptr,
-                      FinRes* fin = nullptr);
+                      double* dot_partial = nullptr, FinRes* fin = nullptr);
+// tiles of the 8x8-tile schedule over a box (i0, i1, j0, j1, k0, k1)
+int w8_box_tiles(const Geo& g, const int* box);
+// the tiles' partials of dot(z, r) over a box, as the pipelined backward sweep forms them
+template <class T>
+void w8_dot_tiles(const Geo& g, const int* box, const T* z, const T* r, double* partial, const PcgScalars* S,
+                  hipStream_t st);
 
 // ---------------- particle kernels (xf_particles.hip) ----------------
 struct Particles {

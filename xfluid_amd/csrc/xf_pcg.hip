@@ -228,13 +228,13 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     z[c] = v * ic;
 }
 
-// dot_partial: when given, the 8x8-tile schedule also writes the block
-// partials of dot(z, r) (returns true then; the caller runs k_dot otherwise)
+// dot_partial: when given, the 8x8-tile schedule also writes the tiles'
+// partials of dot(z, r) (returns true then; the caller runs w8_dot_tiles
+// otherwise, which gives the same partials)
 template <class T>
 static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
                       int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
-                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr,
-                      const BoxRange* dot_range = nullptr, FinRes* fin = nullptr) {
+                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr, FinRes* fin = nullptr) {
     // a pending residual test runs first: inside the 8x8-tile schedule's fused
     // kernel, else as k_finish here
     if (fin && !(sweep_kernel == 9 && sk)) {
@@ -248,9 +248,9 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
             a = tm->ev[tm->used];
             b = tm->ev[tm->used + 1];
         }
-        if (sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st, dot_partial, RED_BLOCKS, dot_range, fin)) {
+        if (sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st, dot_partial, fin)) {
             if (a) tm->used += 2;
-            return dot_partial && dot_range;
+            return dot_partial != nullptr;
         }
     }
     if (fin && !fin->ran) {
@@ -280,6 +280,8 @@ void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const do
         w.sk64.box[1] = g.ni - 1; w.sk64.box[3] = g.nj - 1; w.sk64.box[5] = g.nk - 1;
     }
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, inv, w.sk64, st);
+    // (the pipelined sweeps write z at the box's tiles only)
+    k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g.N, z, 0.0);
     precond_t<double>(g, code, inv, r, q, z, nullptr, sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, nullptr, st);
 }
 
@@ -601,21 +603,30 @@ void launch_finish_res(const FinRes& f, hipStream_t st) {
 // x must be zeroed by the caller; rhs is consumed as the initial residual r.
 template <class T>
 static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* s2, T* z, T* q,
-                     double* partial, PcgScalars* S, PcgScalars* S_host, hipEvent_t* chunk_ev, double tol_factor,
+                     double* partial, int half, PcgScalars* S, PcgScalars* S_host, hipEvent_t* chunk_ev,
+                     double tol_factor,
                      int max_iter, int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
                      SweepTimer* tm, hipStream_t st) {
     const long N = g.N;
     const int tm_start = tm ? tm->used : 0;
     if (tm) tm->k = 0;
-    double* partial2 = partial + RED_BLOCKS;
+    double* partial2 = partial + half;
     // the iteration kernels walk only the rows of the active rows' bounding box
-    // (every vector is zero elsewhere: b has no rows there and x starts at zero)
+    // (every vector is zero elsewhere: b has no rows there and x starts at zero;
+    // z is written at the box's tiles only, and never read outside the box)
     const BoxRange bxr = sk ? box_range(g.ni, g.nj, g.nk, sk->box, RED_BLOCKS * XF_BLOCK) : full_range(g);
+    // dot(z, r) comes as the partials of the box's 8x8 tiles (ndot of them) from the
+    // pipelined backward sweep or w8_dot_tiles, whichever schedule ran: the same bits
+    const int ndot = sk ? w8_box_tiles(g, sk->box) : RED_BLOCKS;
+    auto dot_zr = [&]() {
+        if (sk) w8_dot_tiles<T>(g, sk->box, z, r, partial, S, st);
+        else k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
+    };
+    k_fill<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, (T)0);
     k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
-    if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &bxr))
-        k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
-    k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, RED_BLOCKS, S, tol_factor, 0);
+    if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial)) dot_zr();
+    k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, ndot, S, tol_factor, 0);
     k_copy<T, T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, s, z, S);
     const int CH = 4;
     int it = 0;
@@ -635,8 +646,8 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             if (it == 0) {
                 k_spmv_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, sc, q, partial, S, bxr);
             } else {
-                k_spmv_upd<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, z, sc, sn, q, partial, RED_BLOCKS,
-                                                               partial2, S, bxr);
+                k_spmv_upd<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, z, sc, sn, q, partial, ndot, partial2, S,
+                                                               bxr);
                 T* t = sc;
                 sc = sn;
                 sn = t;
@@ -644,8 +655,8 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
                                                             it == 0 ? partial2 : partial, S);
             FinRes fin{it == 0 ? partial2 : partial, RED_BLOCKS, it, S, it == end - 1 ? slot : nullptr, false};
-            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &bxr, &fin))
-                k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
+            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &fin))
+                dot_zr();
         }
         hipEventRecord(chunk_ev[c & 1], st);
     };
@@ -697,7 +708,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
-        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.S, w.S_host, w.chunk_ev, tol_factor,
+        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.npartial, w.S, w.S_host, w.chunk_ev, tol_factor,
                          max_iter,
                          sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, &w.timer, st);
         res.status = w.S_host->status;
@@ -734,7 +745,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
-        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.S, w.S_host,
+        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.npartial, w.S, w.S_host,
                         w.chunk_ev, itol,
                         max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
         total += w.S_host->iters;

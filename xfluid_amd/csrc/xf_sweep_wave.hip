@@ -65,6 +65,15 @@ __device__ inline void w8_tile_xcd(int b, int ntj, int ntk, int* J, int* K) {
     *K = k0 + w % hb;
 }
 
+// Sum over the 64 lanes of a wave (every lane gets it), in a fixed order: the
+// backward sweep's dot(z, r) partials and k_dot_tiles share it, so both give the
+// same bits.
+__device__ inline double w8_wave_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v = v + __shfl_xor(v, o);
+    return v;
+}
+
 template <int N>
 __device__ inline void vm_wait() {
     static_assert(N >= 0 && N < 64, "vmcnt field");
@@ -218,35 +227,27 @@ __global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restr
     dst[c] = src[(t * w8_slots(g) + i + jl + kl) * 64 + jl + 8 * kl];
 }
 
-// z from the oct layout as k_from_oct, and the block partials of dot(z, r)
-// over the same grid-stride cells in the same order as k_dot (pcg_solver.h:
-// 273, rho = dot(z, r)), so the partials are bit-identical to k_dot's
-// over the rows of the box (BoxRange bx); z elsewhere is left as it is (the
-// PCG kernels never read it: they walk the same rows)
+// dot(z, r) partials of the tiles of the box in the backward sweep's order
+// (per lane over the slots from last to first, then w8_wave_sum), from z and r
+// in the natural layout: the per-hyperplane sweeps give the same partials as
+// the pipelined ones.  One 64-lane block per tile.
 template <class T>
-__global__ void k_from_oct_dot(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ z, const T* __restrict__ r,
-                               double* __restrict__ partial, const PcgScalars* S, Box8 b, BoxRange bx) {
-    __shared__ double sh[16];
+__global__ __launch_bounds__(64) void k_dot_tiles(Geo g, int J0, int K0, int ntkb, int s0, int nS,
+                                                  const T* __restrict__ z, const T* __restrict__ r,
+                                                  double* __restrict__ partial, const PcgScalars* S) {
     if (S && S->done) return;
+    const int tb = blockIdx.x, J = J0 + tb / ntkb, K = K0 + tb % ntkb;
+    const int jl = threadIdx.x & 7, kl = threadIdx.x >> 3, j = 8 * J + jl, k = 8 * K + kl;
     double acc = 0.0;
-    const unsigned stride = gridDim.x * blockDim.x;
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
-    bw_init(bx, t, kq, off);
-    for (; t < bx.n; t += stride) {
-        const long c = bw_cell(bx, kq, off);
-        const int k = b.k0 + (int)kq, j = b.j0 + (int)(off / (unsigned)g.ni), i = (int)(off % (unsigned)g.ni);
-        bw_next(bx, kq, off);
-        T v = (T)0;
-        if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
-            const long t = (long)(j >> 3) * ntk + (k >> 3);
-            const int jl = j & 7, kl = k & 7;
-            v = src[(t * w8_slots(g) + i + jl + kl) * 64 + jl + 8 * kl];
+    if (j < g.nj && k < g.nk) {
+        const long base = (long)g.ni * j + (long)g.nn * k;
+        for (int t = 0; t < nS; ++t) {
+            const int i = s0 + nS - 1 - t - jl - kl;
+            if ((unsigned)i < (unsigned)g.ni) acc = acc + (double)z[base + i] * (double)r[base + i];
         }
-        z[c] = v;
-        acc += (double)v * (double)r[c];
     }
-    acc = block_reduce<double, false>(acc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+    acc = w8_wave_sum(acc);
+    if (threadIdx.x == 0) partial[tb] = acc;
 }
 
 // records: rec[4e..4e+3] forward, rec[4n + 4e..] backward
@@ -283,11 +284,14 @@ __global__ void k_w8_records(Geo g, int ntk, long n, const uint16_t* __restrict_
 
 template <class T>
 struct W8Planes {
-    const T* in;     // input, oct layout
-    T* out;          // output, oct layout
+    const T* in;     // input: natural layout (forward sweep), (q, r) pairs in the oct layout (backward)
+    T* out;          // forward output: (q, r) pairs in the oct layout (the pivots: q alone)
     const T* rec;    // the sweep's records
     uint32_t* edge;  // tagged edge words
+    T* nat;          // backward output z, natural layout
+    double* dot;     // backward: per-tile partials of dot(z, r), or null
 };
+
 
 // all-ones where the record's coefficient is present (sign bit set)
 __device__ inline uint32_t w8_present(float L) { return (uint32_t)(__float_as_int(L) >> 31); }
@@ -350,7 +354,7 @@ struct W8Ring {
     static_assert(!DEEP || (sizeof(T) == 4 && !MIC), "deep ring: FP32 sweeps");
     static constexpr int NS = sizeof(T) / 4;
     static constexpr int R = MIC ? 16 : DEEP ? 64 : 32;
-    static constexpr int SLOT = 64 * (MIC ? 8 : 4) * (int)sizeof(T) + (MIC ? 0 : 64 * (int)sizeof(T)) + 2 * 16 * 8 * NS + 16;
+    static constexpr int SLOT = 64 * (MIC ? 8 : 4) * (int)sizeof(T) + (MIC ? 0 : 128 * (int)sizeof(T)) + 2 * 16 * 8 * NS + 16;
 };
 
 // One sweep of tile (J, K) by the workgroup's three waves (ring_ and ctr_ are
@@ -368,15 +372,17 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
     constexpr int RECB = 64 * (MIC ? 8 : 4) * sizeof(T);  // record bytes per step (1, 2 or 4 KB)
-    constexpr int INB = MIC ? 0 : 64 * sizeof(T);         // input bytes per step
+    constexpr int INB = MIC ? 0 : 128 * sizeof(T);        // input bytes per step (room for the backward pairs)
     constexpr int EOFF = RECB + INB;             // upstream edge rows (j, then k) within a slot
     constexpr int DOFF = EOFF + 2 * ROWB;        // the slot's dummy word
     constexpr int EB = 2 * ROWB + 16;            // edge bytes per slot
     constexpr int SLOT = EOFF + EB;
     // the forward sweep gathers its input from the natural layout (one 4-byte
-    // load per lane and word: no separate layout pass); the backward sweep
-    // reads the forward output in the oct layout (16-byte loads)
+    // load per lane and word: no separate layout pass) and stores (q, r) pairs in
+    // the oct layout; the backward sweep reads those pairs (16-byte loads), so it
+    // has r for dot(z, r) at hand, and stores z in the natural layout
     constexpr bool NATIN = FWD && !MIC;
+    constexpr bool PAIR = FWD && !MIC;
     constexpr int PERD = RECB / 1024 + (INB ? (NATIN ? NS : 1) : 0);  // record + input loads per step
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
     static_assert(SLOT == W8Ring<T, MIC, DEEP>::SLOT, "ring layout");
@@ -435,7 +441,8 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
                     __builtin_amdgcn_global_load_lds(a, dst + RECB, 4, 0, 0);
                     if (NS == 2) __builtin_amdgcn_global_load_lds(a + 4, dst + RECB + 256, 4, 0, 0);
                 } else if (INB && lane < INB / 16) {
-                    __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + e0) + lane * 16, dst + RECB, 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const unsigned char*)(P.in + 2 * e0) + lane * 16, dst + RECB, 16, 0,
+                                                     0);
                 }
             }
         }
@@ -514,8 +521,18 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
     // (buffer stores beyond num_records are dropped by the hardware)
     const int estore = oJ || oK ? eout : 0x7ffff000;
     // this tile's output plane and edge rows (offsets in bytes from the tile's start)
-    const __amdgpu_buffer_rsrc_t orc =
-        __builtin_amdgcn_make_buffer_rsrc(P.out + myTile * SL * 64, (short)0, SL * 64 * (int)sizeof(T), 0x00020000);
+    constexpr int OPC = PAIR ? 2 : 1;  // values per cell in the output plane
+    const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
+        FWD ? P.out + myTile * SL * 64 * OPC : nullptr, (short)0, FWD ? SL * 64 * OPC * (int)sizeof(T) : 0, 0x00020000);
+    // backward: z in the natural layout, from the tile's first cell; a lane off the
+    // grid (or a cell with i outside [0, ni)) stores out of the buffer's range
+    const long norg = (long)g.ni * 8 * J + (long)g.nn * 8 * K;
+    const bool njk = 8 * J + jl < g.nj && 8 * K + kl < g.nk;
+    const int nrel = g.ni * jl + (int)g.nn * kl;
+    const long nrem = FWD || !P.nat ? 0 : (g.N - norg) * (long)sizeof(T);
+    const __amdgpu_buffer_rsrc_t zrc = __builtin_amdgcn_make_buffer_rsrc(
+        FWD || !P.nat ? nullptr : P.nat + norg, (short)0, (int)(nrem < 0x7fffff00L ? nrem : 0x7fffff00L), 0x00020000);
+    double dacc = 0.0;  // backward: this lane's sum of z * r
     const __amdgpu_buffer_rsrc_t erc = __builtin_amdgcn_make_buffer_rsrc(
         (unsigned char*)P.edge + myTile * ER * ROWB, (short)0, ER * ROWB, 0x00020000);
     typedef T T2v __attribute__((ext_vector_type(2)));
@@ -523,7 +540,7 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
     typedef __attribute__((address_space(3))) const uint64_t ldsW;
     typedef __attribute__((address_space(3))) const wv_u32x4 ldsW4;
     typedef __attribute__((address_space(3))) const T ldsT;
-    struct Rec { T L0, L1, L2, ic, rin, M0, M1, M2; };
+    struct Rec { T L0, L1, L2, ic, rin, M0, M1, M2, rd; };
     int seen = 0;
     // wait until the record wave has published `upto` steps
     auto need_rec = [&](int upto) __attribute__((always_inline)) {
@@ -556,7 +573,12 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
         } else if constexpr (NS == 1) {
             const T4v r4 = *(__attribute__((address_space(3))) const T4v*)(sl + lane * 16);
             R.L0 = r4.x; R.L1 = r4.y; R.L2 = r4.z; R.ic = r4.w;
-            R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
+            if constexpr (NATIN) {
+                R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
+            } else {
+                const T2v pr = *(__attribute__((address_space(3))) const T2v*)(sl + RECB + lane * 2 * sizeof(T));
+                R.rin = pr.x; R.rd = pr.y;
+            }
         } else {
             const T2v a = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32);
             const T2v b = *(__attribute__((address_space(3))) const T2v*)(sl + lane * 32 + 16);
@@ -566,7 +588,8 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
                 const uint32_t lo = *(ldsU*)(sl + RECB + lane * 4), hi = *(ldsU*)(sl + RECB + 256 + lane * 4);
                 R.rin = (T)__longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
             } else {
-                R.rin = *(ldsT*)(sl + RECB + lane * sizeof(T));
+                const T2v pr = *(__attribute__((address_space(3))) const T2v*)(sl + RECB + lane * 2 * sizeof(T));
+                R.rin = pr.x; R.rd = pr.y;
             }
         }
     };
@@ -661,13 +684,30 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
             }
             q = v * cur.ic;
         }
-        constexpr int SB = 64 * (int)sizeof(T);
-        const int so = (FWD ? s0 + sb : s0 + nS - 1 - sb - ub) * SB, oi = (FWD ? u : ub - u) * SB;
         const uint64_t qb = NS == 1 ? (uint64_t)__float_as_uint((float)q) : (uint64_t)__double_as_longlong((double)q);
-        if constexpr (NS == 1) {
-            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)qb, orc, lane * 4 + oi, so, 0);
+        if constexpr (FWD) {
+            // (q, r) pairs (the pivots: q), step by step in the tile's plane
+            constexpr int SB = 64 * OPC * (int)sizeof(T);
+            const int so = (s0 + sb) * SB, oi = u * SB;
+            if constexpr (!PAIR) {
+                __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, (uint32_t)(qb >> 32)}, orc, lane * 8 + oi,
+                                                      so, 0);
+            } else if constexpr (NS == 1) {
+                __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, __float_as_uint((float)cur.rin)}, orc,
+                                                      lane * 8 + oi, so, 0);
+            } else {
+                const uint64_t rb = (uint64_t)__double_as_longlong((double)cur.rin);
+                __builtin_amdgcn_raw_buffer_store_b128(
+                    wv_u32x4{(uint32_t)qb, (uint32_t)(qb >> 32), (uint32_t)rb, (uint32_t)(rb >> 32)}, orc, lane * 16 + oi,
+                    so, 0);
+            }
         } else {
-            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, (uint32_t)(qb >> 32)}, orc, lane * 8 + oi, so, 0);
+            // z at cell i of this lane's column; dot(z, r) in the step order
+            const int i = s0 + nS - 1 - s - jl - kl;
+            const int zo = njk && (unsigned)i < (unsigned)g.ni ? (nrel + i) * (int)sizeof(T) : 0x7fffff80;
+            if constexpr (NS == 1) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)qb, zrc, zo, 0, 0);
+            else __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, (uint32_t)(qb >> 32)}, zrc, zo, 0, 0);
+            dacc = dacc + (double)q * (double)cur.rd;
         }
         // edge words: every lane stores, those off the edge out of the buffer's range
         const uint32_t t = tagbase + (uint32_t)s;
@@ -710,6 +750,11 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
         step(s, s & (R - 1), ra, rb, s, 0, 0);
         if (s + 1 < nS) step(s + 1, (s + 1) & (R - 1), rb, ra, s + 1, 0, 0);
         publish(s + 2 < nS ? s + 2 : nS);
+    }
+    if constexpr (!FWD) {
+        // the tile's partial of dot(z, r) (k_dot_tiles sums in the same order)
+        dacc = w8_wave_sum(dacc);
+        if (P.dot && lane == 0) P.dot[(J - J0) * ntkb + (K - K0)] = dacc;
     }
     // edge rows of the steps after the last: zeros, for the downstream tiles' last 7 steps
     if (oJ || oK) {
@@ -864,7 +909,7 @@ bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double
     k_mic_records<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, sk.oRec);
     const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
     const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
-    const W8Planes<double> P{nullptr, sk.oQ, sk.oRec, sk.oEdge};
+    const W8Planes<double> P{nullptr, sk.oQ, sk.oRec, sk.oEdge, nullptr, nullptr};
     k_sweep_w8<double, true, true><<<(J1 - J0 + 1) * (K1 - K0 + 1), 192, 0, st>>>(
         g, ntk, J0, K0, J1 - J0 + 1, K1 - K0 + 1, s0, s1 - s0 + 1, P, nullptr, tag, flags, tau, sigma);
     const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
@@ -895,10 +940,36 @@ static bool w8_fused_fits(int tiles) {
     return tiles <= cap;
 }
 
+// the box's tiles and slots: tiles [J0, J0 + ntjb) x [K0, K0 + ntkb), slots [s0, s0 + nS)
+struct W8Box { int J0, K0, ntjb, ntkb, s0, nS; };
+static W8Box w8_box(const Geo& g, const int* box) {
+    W8Box w;
+    w.J0 = box[2] / 8;
+    w.K0 = box[4] / 8;
+    w.ntjb = box[3] / 8 - w.J0 + 1;
+    w.ntkb = box[5] / 8 - w.K0 + 1;
+    w.s0 = box[0];
+    const int s1 = box[1] + W8_PAD < w8_slots(g) ? box[1] + W8_PAD : w8_slots(g) - 1;
+    w.nS = s1 - w.s0 + 1;
+    return w;
+}
+
+int w8_box_tiles(const Geo& g, const int* box) {
+    const W8Box w = w8_box(g, box);
+    return w.ntjb * w.ntkb;
+}
+
+template <class T>
+void w8_dot_tiles(const Geo& g, const int* box, const T* z, const T* r, double* partial, const PcgScalars* S,
+                  hipStream_t st) {
+    const W8Box w = w8_box(g, box);
+    k_dot_tiles<T><<<w.ntjb * w.ntkb, 64, 0, st>>>(g, w.J0, w.K0, w.ntkb, w.s0, w.nS, z, r, partial, S);
+}
+
 template <class T>
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
-                      int dot_blocks, const BoxRange* dot_range, FinRes* fin) {
+                      FinRes* fin) {
     int ntj, ntk;
     w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
@@ -907,11 +978,12 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
     if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, sk, &tag1, st) ||
         !w8_next_tag(g, sk, &tag2, st))
         return false;
-    const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
-    const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
-    const W8Planes<T> f{r, sk.oQ, sk.oRec, sk.oEdge};  // (natural-layout input)
-    const W8Planes<T> b{sk.oQ, sk.oZ, sk.oRec + 4 * n, sk.oEdge};
-    const int ntjb = J1 - J0 + 1, ntkb = K1 - K0 + 1, nS = s1 - s0 + 1;
+    const W8Box w = w8_box(g, sk.box);
+    const int J0 = w.J0, K0 = w.K0, ntjb = w.ntjb, ntkb = w.ntkb, s0 = w.s0, nS = w.nS;
+    // forward: r (natural) -> (q, r) pairs; backward: pairs -> z (natural) and the
+    // tiles' dot(z, r) partials
+    const W8Planes<T> f{r, sk.oQ, sk.oRec, sk.oEdge, nullptr, nullptr};
+    const W8Planes<T> b{sk.oQ, nullptr, sk.oRec + 4 * n, sk.oEdge, z, dot_partial};
     const int tiles = ntjb * ntkb;
     // t0 / t1 ride on the dispatch packets themselves, so they time the kernels and not
     // the marker packets (and their cache release) that hipEventRecord would add
@@ -935,19 +1007,18 @@ bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, co
         hipExtLaunchKernelGGL(k_sweep_w8<T, false>, dim3(tiles), dim3(192), 0, st, (hipEvent_t) nullptr, t1, 0, g,
                               ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags, 0.0, 0.0);
     }
-    const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
-    if (dot_partial && dot_range)
-        k_from_oct_dot<T><<<dot_blocks, XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, r, dot_partial, S, bx, *dot_range);
-    else
-        k_from_oct<T><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oZ, z, S, bx);
     return true;
 }
 
 template void sweep_prepare_w8<double>(const Geo&, const uint16_t*, const double*, const SweepSkew<double>&, hipStream_t);
 template void sweep_prepare_w8<float>(const Geo&, const uint16_t*, const float*, const SweepSkew<float>&, hipStream_t);
 template bool sweep_precond_w8<double>(const Geo&, const SweepSkew<double>&, const double*, double*, const PcgScalars*,
-                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int,
-                                       const BoxRange*, FinRes*);
+                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*,
+                                       FinRes*);
 template bool sweep_precond_w8<float>(const Geo&, const SweepSkew<float>&, const float*, float*, const PcgScalars*,
-                                      int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*, int,
-                                      const BoxRange*, FinRes*);
+                                      int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*,
+                                      FinRes*);
+template void w8_dot_tiles<double>(const Geo&, const int*, const double*, const double*, double*, const PcgScalars*,
+                                   hipStream_t);
+template void w8_dot_tiles<float>(const Geo&, const int*, const float*, const float*, double*, const PcgScalars*,
+                                  hipStream_t);
