@@ -483,7 +483,8 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     {
         // partial halves: the reduction blocks, or the tiles of a whole-grid box
         const int all[6] = {0, s->g.ni - 1, 0, s->g.nj - 1, 0, s->g.nk - 1};
-        const int tiles = w8_box_tiles(s->g, all);
+        const int t0 = w8_box_tiles(s->g, 0, all), t2 = w8_box_tiles(s->g, 2, all);
+        const int tiles = t0 > t2 ? t0 : t2;
         s->pw.npartial = tiles > RED_N ? tiles : RED_N;
         if (dalloc(&s->pw.partial, 2 * (long)s->pw.npartial) != hipSuccess) return fail(set_err(-2, "alloc partials"));
     }

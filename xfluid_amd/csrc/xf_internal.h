@@ -53,6 +53,7 @@ struct SweepSkew {
     uint32_t* oEdge;
     mutable uint32_t w8_launch;  // launches on oEdge (their tags)
     mutable int fused;           // last application: 1 one fused forward + backward launch, 0 two launches
+    int ax;                      // axis of the tiles' lines: 0 along i, 2 along k (w8_choose_axis)
     // bounding box of the active rows (i0, i1, j0, j1, k0, k1; host copy), see pcg_active_box
     int box[6];
 };
@@ -124,11 +125,13 @@ template <class T>
 bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
                       unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st,
                       double* dot_partial = nullptr, FinRes* fin = nullptr);
-// tiles of the 8x8-tile schedule over a box (i0, i1, j0, j1, k0, k1)
-int w8_box_tiles(const Geo& g, const int* box);
+// tiles of the 8x8-tile schedule with lines along axis ax over a box (i0, i1, j0, j1, k0, k1)
+int w8_box_tiles(const Geo& g, int ax, const int* box);
+// the axis (0: i, 2: k) whose lines give the shortest pipeline over a box
+int w8_choose_axis(const Geo& g, const int* box);
 // the tiles' partials of dot(z, r) over a box, as the pipelined backward sweep forms them
 template <class T>
-void w8_dot_tiles(const Geo& g, const int* box, const T* z, const T* r, double* partial, const PcgScalars* S,
+void w8_dot_tiles(const Geo& g, int ax, const int* box, const T* z, const T* r, double* partial, const PcgScalars* S,
                   hipStream_t st);
 
 // ---------------- particle kernels (xf_particles.hip) ----------------

@@ -165,7 +165,7 @@ void pcg_mic0(const Geo& g, const uint16_t* code, double* inv, double tau, doubl
 // with L_{c,k} = A_{c,k} * inv_k.
 template <class T>
 __global__ void k_fwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__ code, const T* __restrict__ inv,
-                            const T* __restrict__ r, T* __restrict__ q, const PcgScalars* S) {
+                            const T* __restrict__ r, T* __restrict__ q, const PcgScalars* S, int ax) {
     if (S && S->done) return;
     long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     int j = (int)(tid % g.nj), k = kmin + (int)(tid / g.nj);
@@ -175,13 +175,16 @@ __global__ void k_fwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     uint16_t cc = code[c];
     T v = r[c];
     if constexpr (sizeof(T) == 4) {
-        // FP32: the 8x8-tile sweep's order (-i, -j, -k entries, fused multiply-adds)
-        int cl = code_dir(cc, D_XM);
-        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - 1]), q[c - 1], v);
-        cl = code_dir(cc, D_YM);
-        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - g.ni]), q[c - g.ni], v);
-        cl = code_dir(cc, D_ZM);
-        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - g.nn]), q[c - g.nn], v);
+        // FP32: the 8x8-tile sweep's order with lines along axis ax (the line's
+        // previous cell, then the lanes' A and B neighbours: -i, -j, -k for ax 0,
+        // -k, -i, -j for ax 2), fused multiply-adds
+        const int d[3] = {ax == 2 ? D_ZM : D_XM, ax == 2 ? D_XM : D_YM, ax == 2 ? D_YM : D_ZM};
+        const long o[3] = {ax == 2 ? g.nn : 1, ax == 2 ? 1 : (long)g.ni, ax == 2 ? (long)g.ni : g.nn};
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const int cl = code_dir(cc, d[e]);
+            if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * inv[c - o[e]]), q[c - o[e]], v);
+        }
         q[c] = v * inv[c];
         return;
     }
@@ -198,7 +201,7 @@ __global__ void k_fwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
 //   z_c = (((q_c - L_{c+1,c} z_{c+1}) - L_{c+ni,c} z_{c+ni}) - L_{c+nn,c} z_{c+nn}) * inv_c
 template <class T>
 __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__ code, const T* __restrict__ inv,
-                            const T* __restrict__ q, T* __restrict__ z, const PcgScalars* S) {
+                            const T* __restrict__ q, T* __restrict__ z, const PcgScalars* S, int ax) {
     if (S && S->done) return;
     long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     int j = (int)(tid % g.nj), k = kmin + (int)(tid / g.nj);
@@ -209,13 +212,15 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
     T ic = inv[c];
     T v = q[c];
     if constexpr (sizeof(T) == 4) {
-        // FP32: fused multiply-adds, as the 8x8-tile sweep
-        int cl = code_dir(cc, D_XP);
-        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + 1], v);
-        cl = code_dir(cc, D_YP);
-        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + g.ni], v);
-        cl = code_dir(cc, D_ZP);
-        if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + g.nn], v);
+        // FP32: fused multiply-adds in the 8x8-tile sweep's order (+i, +j, +k for
+        // lines along i, +k, +i, +j along k)
+        const int d[3] = {ax == 2 ? D_ZP : D_XP, ax == 2 ? D_XP : D_YP, ax == 2 ? D_YP : D_ZP};
+        const long o[3] = {ax == 2 ? g.nn : 1, ax == 2 ? 1 : (long)g.ni, ax == 2 ? (long)g.ni : g.nn};
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const int cl = code_dir(cc, d[e]);
+            if (present(cl)) v = __builtin_fmaf(-((T)code_coef(cl) * ic), z[c + o[e]], v);
+        }
         z[c] = v * ic;
         return;
     }
@@ -262,13 +267,13 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
         int kmin;
         long nt;
         plane_range(g, t, &kmin, &nt);
-        k_fwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, r, q, S);
+        k_fwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, r, q, S, sk ? sk->ax : 0);
     }
     for (int t = planes - 1; t >= 0; --t) {
         int kmin;
         long nt;
         plane_range(g, t, &kmin, &nt);
-        k_bwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, q, z, S);
+        k_bwd_plane<T><<<xf_blocks(nt), XF_BLOCK, 0, st>>>(g, t, kmin, code, inv, q, z, S, sk ? sk->ax : 0);
     }
     return false;
 }
@@ -279,6 +284,7 @@ void pcg_precond(const Geo& g, const uint16_t* code, const double* inv, const do
         w.sk64.box[0] = w.sk64.box[2] = w.sk64.box[4] = 0;
         w.sk64.box[1] = g.ni - 1; w.sk64.box[3] = g.nj - 1; w.sk64.box[5] = g.nk - 1;
     }
+    w.sk64.ax = w8_choose_axis(g, w.sk64.box);
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, inv, w.sk64, st);
     // (the pipelined sweeps write z at the box's tiles only)
     k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g.N, z, 0.0);
@@ -617,9 +623,9 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
     const BoxRange bxr = sk ? box_range(g.ni, g.nj, g.nk, sk->box, RED_BLOCKS * XF_BLOCK) : full_range(g);
     // dot(z, r) comes as the partials of the box's 8x8 tiles (ndot of them) from the
     // pipelined backward sweep or w8_dot_tiles, whichever schedule ran: the same bits
-    const int ndot = sk ? w8_box_tiles(g, sk->box) : RED_BLOCKS;
+    const int ndot = sk ? w8_box_tiles(g, sk->ax, sk->box) : RED_BLOCKS;
     auto dot_zr = [&]() {
-        if (sk) w8_dot_tiles<T>(g, sk->box, z, r, partial, S, st);
+        if (sk) w8_dot_tiles<T>(g, sk->ax, sk->box, z, r, partial, S, st);
         else k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
     };
     k_fill<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, (T)0);
@@ -701,6 +707,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         box[1] = g.ni - 1; box[3] = g.nj - 1; box[5] = g.nk - 1;
     }
     for (int d = 0; d < 6; ++d) w.sk64.box[d] = w.sk32.box[d] = box[d];
+    w.sk64.ax = w.sk32.ax = w8_choose_axis(g, box);
     res.box_cells = (long)(box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (box[5] - box[4] + 1);
     if (!(sweep_kernel == 9 && mic0_w8(g, code, w.inv, tau, sigma, w.sk64, w.sweep_flags, &w.epoch, st)))
         pcg_mic0(g, code, w.inv, tau, sigma, box, st);

@@ -170,61 +170,80 @@ typedef uint32_t wv_u32x2 __attribute__((ext_vector_type(2)));
 #define W8_B 16  // steps per unrolled block of the compute wave
 
 
-static inline void w8_tiles(const Geo& g, int* ntj, int* ntk) {
-    *ntj = (g.nj + 7) / 8;
-    *ntk = (g.nk + 7) / 8;
+// Geometry of the schedule on one of two axes.  The lines (a cell per lane and
+// step) run along axis L; the lanes cover axis A (lane & 7: the -1 neighbour is
+// one lane down) and axis B (lane >> 3: eight lanes down).  ax 0: L = i, A = j,
+// B = k; ax 2: L = k, A = i, B = j.  Tile (J, K) covers A in [8J, 8J + 8) and B in
+// [8K, 8K + 8); its index is J * ntB + K.  Strides are the natural layout's.
+struct W8Geo {
+    int ax, nL, nA, nB, ntA, ntB, SL;
+    long sL, sA, sB, N;
+};
+__host__ __device__ inline W8Geo w8_geo(const Geo& g, int ax) {
+    W8Geo G;
+    G.ax = ax;
+    if (ax == 2) {
+        G.nL = g.nk; G.nA = g.ni; G.nB = g.nj;
+        G.sL = g.nn; G.sA = 1; G.sB = g.ni;
+    } else {
+        G.nL = g.ni; G.nA = g.nj; G.nB = g.nk;
+        G.sL = 1; G.sA = g.ni; G.sB = g.nn;
+    }
+    G.ntA = (G.nA + 7) / 8;
+    G.ntB = (G.nB + 7) / 8;
+    G.SL = G.nL + W8_PAD;
+    G.N = g.N;
+    return G;
 }
-__host__ __device__ inline int w8_slots(const Geo& g) { return g.ni + W8_PAD; }
 // edge rows per tile: one per step and 7 written after the last step (+1 spare)
-__host__ __device__ inline int w8_erows(const Geo& g) { return w8_slots(g) + 8; }
+__host__ __device__ inline int w8_erows(const W8Geo& G) { return G.SL + 8; }
+// buffer sizes: the larger of the two axes
 long sweep_w8_size(const Geo& g) {
-    int ntj, ntk;
-    w8_tiles(g, &ntj, &ntk);
-    return (long)ntj * ntk * w8_slots(g) * 64;
+    long n = 0;
+    for (int ax : {0, 2}) {
+        const W8Geo G = w8_geo(g, ax);
+        const long m = (long)G.ntA * G.ntB * G.SL * 64;
+        n = m > n ? m : n;
+    }
+    return n;
 }
 long sweep_w8_edge_words(const Geo& g) {
-    int ntj, ntk;
-    w8_tiles(g, &ntj, &ntk);
-    return (long)ntj * ntk * w8_erows(g) * 16 * 4;  // u32 units, sized for FP64 words
+    long n = 0;
+    for (int ax : {0, 2}) {
+        const W8Geo G = w8_geo(g, ax);
+        const long m = (long)G.ntA * G.ntB * w8_erows(G) * 16 * 4;  // u32 units, sized for FP64 words
+        n = m > n ? m : n;
+    }
+    return n;
 }
 
-__device__ inline void w8_cell(const Geo& g, int ntk, long e, int* i, int* j, int* k) {
-    const int SL = w8_slots(g);
+// natural-layout cell (i, j, k) of element e; false if it lies off the grid
+__device__ inline bool w8_cell(const W8Geo& G, long e, int* i, int* j, int* k) {
     const int lane = (int)(e & 63);
     const long row = e >> 6;
-    const long t = row / SL;
-    const int slot = (int)(row - t * SL);
+    const long t = row / G.SL;
+    const int slot = (int)(row - t * G.SL);
     const int jl = lane & 7, kl = lane >> 3;
-    *j = (int)(t / ntk) * 8 + jl;
-    *k = (int)(t % ntk) * 8 + kl;
-    *i = slot - jl - kl;
-}
-
-template <class T>
-__global__ void k_to_oct(Geo g, int ntk, long n, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S) {
-    if (S && S->done) return;
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    int i, j, k;
-    w8_cell(g, ntk, e, &i, &j, &k);
-    T v = (T)0;
-    if (j < g.nj && k < g.nk && i >= 0 && i < g.ni) v = src[cidx(g, i, j, k)];
-    dst[e] = v;
+    const int a = (int)(t / G.ntB) * 8 + jl, b = (int)(t % G.ntB) * 8 + kl, l = slot - jl - kl;
+    if (G.ax == 2) { *i = a; *j = b; *k = l; }
+    else { *i = l; *j = a; *k = b; }
+    return a < G.nA && b < G.nB && l >= 0 && l < G.nL;
 }
 
 struct Box8 { int i0, i1, j0, j1, k0, k1; };
 // cells outside the active box are zero (the sweep did not visit their slots)
 template <class T>
-__global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S,
+__global__ void k_from_oct(Geo g, W8Geo G, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S,
                            Box8 b) {
     if (S && S->done) return;
     const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / g.nn);
     if (i < b.i0 || i > b.i1 || j < b.j0 || j > b.j1 || k < b.k0 || k > b.k1) { dst[c] = (T)0; return; }
-    const long t = (long)(j >> 3) * ntk + (k >> 3);
-    const int jl = j & 7, kl = k & 7;
-    dst[c] = src[(t * w8_slots(g) + i + jl + kl) * 64 + jl + 8 * kl];
+    const int l = G.ax == 2 ? k : i, a = G.ax == 2 ? i : j, bb = G.ax == 2 ? j : k;
+    const long t = (long)(a >> 3) * G.ntB + (bb >> 3);
+    const int jl = a & 7, kl = bb & 7;
+    dst[c] = src[(t * G.SL + l + jl + kl) * 64 + jl + 8 * kl];
 }
 
 // dot(z, r) partials of the tiles of the box in the backward sweep's order
@@ -232,18 +251,21 @@ __global__ void k_from_oct(Geo g, int ntk, const T* __restrict__ src, T* __restr
 // in the natural layout: the per-hyperplane sweeps give the same partials as
 // the pipelined ones.  One 64-lane block per tile.
 template <class T>
-__global__ __launch_bounds__(64) void k_dot_tiles(Geo g, int J0, int K0, int ntkb, int s0, int nS,
+__global__ __launch_bounds__(64) void k_dot_tiles(W8Geo G, int J0, int K0, int ntkb, int s0, int nS,
                                                   const T* __restrict__ z, const T* __restrict__ r,
                                                   double* __restrict__ partial, const PcgScalars* S) {
     if (S && S->done) return;
     const int tb = blockIdx.x, J = J0 + tb / ntkb, K = K0 + tb % ntkb;
-    const int jl = threadIdx.x & 7, kl = threadIdx.x >> 3, j = 8 * J + jl, k = 8 * K + kl;
+    const int jl = threadIdx.x & 7, kl = threadIdx.x >> 3, a = 8 * J + jl, b = 8 * K + kl;
     double acc = 0.0;
-    if (j < g.nj && k < g.nk) {
-        const long base = (long)g.ni * j + (long)g.nn * k;
+    if (a < G.nA && b < G.nB) {
+        const long base = G.sA * a + G.sB * b;
         for (int t = 0; t < nS; ++t) {
-            const int i = s0 + nS - 1 - t - jl - kl;
-            if ((unsigned)i < (unsigned)g.ni) acc = acc + (double)z[base + i] * (double)r[base + i];
+            const int l = s0 + nS - 1 - t - jl - kl;
+            if ((unsigned)l < (unsigned)G.nL) {
+                const long c = base + G.sL * l;
+                acc = acc + (double)z[c] * (double)r[c];
+            }
         }
     }
     acc = w8_wave_sum(acc);
@@ -252,14 +274,13 @@ __global__ __launch_bounds__(64) void k_dot_tiles(Geo g, int J0, int K0, int ntk
 
 // records: rec[4e..4e+3] forward, rec[4n + 4e..] backward
 template <class T>
-__global__ void k_w8_records(Geo g, int ntk, long n, const uint16_t* __restrict__ code, const T* __restrict__ inv,
+__global__ void k_w8_records(Geo g, W8Geo G, long n, const uint16_t* __restrict__ code, const T* __restrict__ inv,
                              T* __restrict__ rec) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
     int i, j, k;
-    w8_cell(g, ntk, e, &i, &j, &k);
     T lf[3] = {0, 0, 0}, lb[3] = {0, 0, 0}, ic = 0;
-    if (j < g.nj && k < g.nk && i >= 0 && i < g.ni) {
+    if (w8_cell(G, e, &i, &j, &k)) {
         const long c = cidx(g, i, j, k);
         const uint16_t cd = code[c];
         ic = inv[c];
@@ -359,8 +380,8 @@ struct W8Ring {
 
 // One sweep of tile (J, K) by the workgroup's three waves (ring_ and ctr_ are
 // the workgroup's LDS).
-template <class T, bool FWD, bool MIC, bool DEEP = false>
-__device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, int J, int K, int J0, int K0, int ntjb,
+template <class T, bool FWD, bool MIC, bool DEEP, int AX>
+__device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, int J, int K, int J0, int K0, int ntjb,
                                                                int ntkb, int s0, int nS, W8Planes<T> P,
                                                                uint32_t tagbase, int* err, double tau, double sigma,
                                                                unsigned char* ring_, int* ctr_) {
@@ -387,12 +408,12 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
     constexpr int CJ = FWD ? 7 : 0;              // jl of the edge column (and of the row's corner)
     static_assert(SLOT == W8Ring<T, MIC, DEEP>::SLOT, "ring layout");
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int SL = w8_slots(g), ER = w8_erows(g);
-    const long myTile = (long)J * ntk + K;
+    const int SL = G.SL, ER = w8_erows(G);
+    const long myTile = (long)J * G.ntB + K;
     const int uJ = FWD ? J - 1 : J + 1, uK = FWD ? K - 1 : K + 1;
     const bool hasJ = uJ >= J0 && uJ < J0 + ntjb, hasK = uK >= K0 && uK < K0 + ntkb;
     auto slot_of = [s0, nS](int t) __attribute__((always_inline)) { return FWD ? s0 + t : s0 + nS - 1 - t; };
-    const long tJ = (long)uJ * ntk + K, tK = (long)J * ntk + uK;
+    const long tJ = (long)uJ * G.ntB + K, tK = (long)J * G.ntB + uK;
     lds_u8* ring = (lds_u8*)ring_;
     lds_int* ctr = (lds_int*)ctr_;
     const uint32_t ctrA = (uint32_t)(uintptr_t)ctr, ringA = (uint32_t)(uintptr_t)ring;
@@ -411,8 +432,11 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
         // count it published, with R - LEAD >= 4 slots of slack.
         int cons_seen = 0;
         const int rjl = lane & 7, rkl = lane >> 3;
-        const bool rjk = 8 * J + rjl < g.nj && 8 * K + rkl < g.nk;
-        const long rbase = rjk ? (long)g.ni * (8 * J + rjl) + (long)g.nn * (8 * K + rkl) : 0;
+        const bool rjk = 8 * J + rjl < G.nA && 8 * K + rkl < G.nB;
+        // cell of slot t: rrow + sL * t (the line's cell l = t - rjl - rkl); the
+        // step's part sL * t is wave-uniform, so no per-lane multiply
+        const T* rrow = P.in + (rjk ? G.sA * (8 * J + rjl) + G.sB * (8 * K + rkl) - G.sL * (rjl + rkl) : 0);
+        const int rl0 = rjl + rkl;
         for (int x = -LEAD; x < nS; ++x) {
             if (x >= 0) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -435,9 +459,10 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
                 if constexpr (NATIN) {
                     // lane's cell at this slot, or cell 0 for a lane outside the grid
                     // (its records are zero, so its input only has to be finite)
-                    const int i = slot_of(s) - rjl - rkl;
-                    const long c = rjk && (unsigned)i < (unsigned)g.ni ? rbase + i : 0;
-                    const unsigned char* a = (const unsigned char*)(P.in + c);
+                    const int t = slot_of(s);
+                    const bool ok = rjk & ((unsigned)(t - rl0) < (unsigned)G.nL);
+                    const long so = G.sL * t;
+                    const unsigned char* a = (const unsigned char*)(ok ? rrow + so : P.in);
                     __builtin_amdgcn_global_load_lds(a, dst + RECB, 4, 0, 0);
                     if (NS == 2) __builtin_amdgcn_global_load_lds(a + 4, dst + RECB + 256, 4, 0, 0);
                 } else if (INB && lane < INB / 16) {
@@ -526,10 +551,13 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
         FWD ? P.out + myTile * SL * 64 * OPC : nullptr, (short)0, FWD ? SL * 64 * OPC * (int)sizeof(T) : 0, 0x00020000);
     // backward: z in the natural layout, from the tile's first cell; a lane off the
     // grid (or a cell with i outside [0, ni)) stores out of the buffer's range
-    const long norg = (long)g.ni * 8 * J + (long)g.nn * 8 * K;
-    const bool njk = 8 * J + jl < g.nj && 8 * K + kl < g.nk;
-    const int nrel = g.ni * jl + (int)g.nn * kl;
-    const long nrem = FWD || !P.nat ? 0 : (g.N - norg) * (long)sizeof(T);
+    const long norg = G.sA * 8 * J + G.sB * 8 * K;
+    const bool njk = 8 * J + jl < G.nA && 8 * K + kl < G.nB;
+    // byte offset of step s's cell: zb - s * sLb (a wave-uniform product), its line
+    // cell l = ls - s
+    const int sLb = (int)G.sL * (int)sizeof(T), ls = s0 + nS - 1 - jl - kl;
+    const int zb = (int)(G.sA * jl + G.sB * kl) * (int)sizeof(T) + ls * sLb;
+    const long nrem = FWD || !P.nat ? 0 : (G.N - norg) * (long)sizeof(T);
     const __amdgpu_buffer_rsrc_t zrc = __builtin_amdgcn_make_buffer_rsrc(
         FWD || !P.nat ? nullptr : P.nat + norg, (short)0, (int)(nrem < 0x7fffff00L ? nrem : 0x7fffff00L), 0x00020000);
     double dacc = 0.0;  // backward: this lane's sum of z * r
@@ -646,42 +674,45 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
         qj = eJ ? ew_val(wj) : qj;
         qk = eK ? ew_val(wk) : qk;
         T v = cur.rin, w, q;
+        // the neighbour values in the records' direction order (forward -k, -j, -i,
+        // backward +i, +j, +k): qp is the line's previous cell, qj the A neighbour,
+        // qk the B neighbour
+        const T n0 = FWD ? (AX == 2 ? qp : qk) : (AX == 2 ? qj : qp);
+        const T n1 = FWD ? (AX == 2 ? qk : qj) : (AX == 2 ? qk : qj);
+        const T n2 = FWD ? (AX == 2 ? qj : qp) : (AX == 2 ? qp : qk);
         if constexpr (MIC) {
             // d = diag - sum over the -k, -j, -i entries of L^2 + tau L miss
             const T diag = cur.ic;
             T d = diag;
-            T L = cur.L0 * qk, miss = cur.M0 * qk;
+            T L = cur.L0 * n0, miss = cur.M0 * n0;
             d -= L * L;
             d -= (T)tau * L * miss;
-            L = cur.L1 * qj; miss = cur.M1 * qj;
+            L = cur.L1 * n1; miss = cur.M1 * n1;
             d -= L * L;
             d -= (T)tau * L * miss;
-            L = cur.L2 * qp; miss = cur.M2 * qp;
+            L = cur.L2 * n2; miss = cur.M2 * n2;
             d -= L * L;
             d -= (T)tau * L * miss;
             q = diag == 0 ? (T)0 : (d < (T)sigma * diag) ? 1 / sqrt(diag) : 1 / sqrt(d);
         } else if constexpr (NS == 1) {
-            // FP32: fused multiply-adds, the own previous value first and the permuted
-            // k-neighbour, the last to arrive, last (k_fwd_plane / k_bwd_plane<float>
-            // use the same order, so the fallback gives the same bits)
-            const T Li = FWD ? cur.L2 : cur.L0, Lk = FWD ? cur.L0 : cur.L2;
-            v = wv_sel<T>(w8_present(Li), __builtin_fmaf(-Li, qp, v), v);
-            v = wv_sel<T>(w8_present(cur.L1), __builtin_fmaf(-cur.L1, qj, v), v);
-            v = wv_sel<T>(w8_present(Lk), __builtin_fmaf(-Lk, qk, v), v);
+            // FP32: fused multiply-adds, the line's previous cell first and the B
+            // neighbour (permuted across rows, the last to arrive) last;
+            // k_fwd_plane / k_bwd_plane<float> use the same order, so the fallback
+            // gives the same bits.  (Record index of the own, A and B entries.)
+            constexpr int IO = FWD ? (AX == 2 ? 0 : 2) : (AX == 2 ? 2 : 0);
+            constexpr int IA = FWD ? (AX == 2 ? 2 : 1) : (AX == 2 ? 0 : 1);
+            constexpr int IB = 3 - IO - IA;
+            const T Ls[3] = {cur.L0, cur.L1, cur.L2};
+            v = wv_sel<T>(w8_present(Ls[IO]), __builtin_fmaf(-Ls[IO], qp, v), v);
+            v = wv_sel<T>(w8_present(Ls[IA]), __builtin_fmaf(-Ls[IA], qj, v), v);
+            v = wv_sel<T>(w8_present(Ls[IB]), __builtin_fmaf(-Ls[IB], qk, v), v);
             q = v * cur.ic;
         } else {
-            // FP64: the reference's order
-            if (FWD) {
-                // column order c-nn, c-ni, c-1
-                w = v - cur.L0 * qk; v = wv_sel<T>(w8_present(cur.L0), w, v);
-                w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
-                w = v - cur.L2 * qp; v = wv_sel<T>(w8_present(cur.L2), w, v);
-            } else {
-                // row order c+1, c+ni, c+nn
-                w = v - cur.L0 * qp; v = wv_sel<T>(w8_present(cur.L0), w, v);
-                w = v - cur.L1 * qj; v = wv_sel<T>(w8_present(cur.L1), w, v);
-                w = v - cur.L2 * qk; v = wv_sel<T>(w8_present(cur.L2), w, v);
-            }
+            // FP64: the reference's order (forward c-nn, c-ni, c-1; backward c+1,
+            // c+ni, c+nn)
+            w = v - cur.L0 * n0; v = wv_sel<T>(w8_present(cur.L0), w, v);
+            w = v - cur.L1 * n1; v = wv_sel<T>(w8_present(cur.L1), w, v);
+            w = v - cur.L2 * n2; v = wv_sel<T>(w8_present(cur.L2), w, v);
             q = v * cur.ic;
         }
         const uint64_t qb = NS == 1 ? (uint64_t)__float_as_uint((float)q) : (uint64_t)__double_as_longlong((double)q);
@@ -702,9 +733,9 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
                     so, 0);
             }
         } else {
-            // z at cell i of this lane's column; dot(z, r) in the step order
-            const int i = s0 + nS - 1 - s - jl - kl;
-            const int zo = njk && (unsigned)i < (unsigned)g.ni ? (nrel + i) * (int)sizeof(T) : 0x7fffff80;
+            // z at cell l of this lane's line; dot(z, r) in the step order
+            const bool ok = njk & ((unsigned)(ls - s) < (unsigned)G.nL);
+            const int zo = ok ? zb - s * sLb : 0x7fffff80;
             if constexpr (NS == 1) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)qb, zrc, zo, 0, 0);
             else __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, (uint32_t)(qb >> 32)}, zrc, zo, 0, 0);
             dacc = dacc + (double)q * (double)cur.rd;
@@ -769,16 +800,17 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(Geo g, int ntk, i
     }
 }
 
-template <class T, bool FWD, bool MIC = false>
-__global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0, int nS,
+template <class T, bool FWD, bool MIC, int AX>
+__global__ __launch_bounds__(192) void k_sweep_w8(W8Geo G, int J0, int K0, int ntjb, int ntkb, int s0, int nS,
                                                   W8Planes<T> P, const PcgScalars* S, uint32_t tagbase, int* err,
-                                                  double tau = 0.0, double sigma = 0.0) {
+                                                  double tau, double sigma) {
     __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, MIC>::R * W8Ring<T, MIC>::SLOT];
     __shared__ int ctr_[2];  // [0] steps whose records are in, [1] steps consumed
     if (S && S->done) return;
     int J, K;
     wv_tile(blockIdx.x, ntjb, ntkb, FWD, &J, &K);
-    w8_sweep<T, FWD, MIC>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, P, tagbase, err, tau, sigma, ring_, ctr_);
+    w8_sweep<T, FWD, MIC, false, AX>(G, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, P, tagbase, err, tau, sigma, ring_,
+                                     ctr_);
 }
 
 // Forward and backward sweep in one launch: each workgroup sweeps its tile
@@ -786,8 +818,8 @@ __global__ __launch_bounds__(192) void k_sweep_w8(Geo g, int ntk, int J0, int K0
 // begins the backward one, with no launch in between).  Every tile waits on
 // tiles that start later, so all tiles must be resident at once: the host
 // uses this kernel only when they fit (w8_fused_fits).
-template <class T, bool DEEP>
-__global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int K0, int ntjb, int ntkb, int s0,
+template <class T, bool DEEP, int AX>
+__global__ __launch_bounds__(192) void k_sweep_w8_fb(W8Geo G, int J0, int K0, int ntjb, int ntkb, int s0,
                                                      int nS, W8Planes<T> Pf, W8Planes<T> Pb, const PcgScalars* S,
                                                      uint32_t tag1, uint32_t tag2, int* err, const double* fin_partial,
                                                      int fin_n, int fin_it, PcgScalars* Sw, PcgScalars* mirror) {
@@ -831,12 +863,12 @@ __global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int
     }
     int J, K;
     w8_tile_xcd(blockIdx.x, ntjb, ntkb, &J, &K);
-    w8_sweep<T, true, false, DEEP>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_,
+    w8_sweep<T, true, false, DEEP, AX>(G, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_,
                                    ctr_);
     // this tile's forward output is its backward input: stores complete, LDS free
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    w8_sweep<T, false, false, DEEP>(g, ntk, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pb, tag2, err, 0.0, 0.0, ring_,
+    w8_sweep<T, false, false, DEEP, AX>(G, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pb, tag2, err, 0.0, 0.0, ring_,
                                     ctr_);
 }
 
@@ -844,13 +876,12 @@ __global__ __launch_bounds__(192) void k_sweep_w8_fb(Geo g, int ntk, int J0, int
 // cM2, 0} per element, directions -k, -j, -i; cM sums the neighbour's
 // coefficients in the two directions k_mic_plane drops (exact halves, so
 // cM * inv rounds like the sum of the two products).
-__global__ void k_mic_records(Geo g, int ntk, long n, const uint16_t* __restrict__ code, double* __restrict__ rec) {
+__global__ void k_mic_records(Geo g, W8Geo G, long n, const uint16_t* __restrict__ code, double* __restrict__ rec) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
     int i, j, k;
-    w8_cell(g, ntk, e, &i, &j, &k);
     double cL[3] = {0, 0, 0}, cM[3] = {0, 0, 0}, diag = 0;
-    if (j < g.nj && k < g.nk && i >= 0 && i < g.ni) {
+    if (w8_cell(G, e, &i, &j, &k)) {
         const long c = cidx(g, i, j, k);
         const uint16_t cc = code[c];
         diag = code_diag(cc);
@@ -887,8 +918,8 @@ __global__ void k_w8_clear(long n, uint32_t* __restrict__ p) {
 // the buffer is cleared, so no word left from 32768 launches earlier can carry
 // a current tag.
 template <class T>
-static bool w8_next_tag(const Geo& g, const SweepSkew<T>& sk, uint32_t* tag, hipStream_t st) {
-    if (w8_erows(g) >= 0xffff) return false;
+static bool w8_next_tag(const Geo& g, const W8Geo& G, const SweepSkew<T>& sk, uint32_t* tag, hipStream_t st) {
+    if (w8_erows(G) >= 0xffff) return false;
     const uint32_t n = sk.w8_launch++ & 0x7fffu;
     if (n == 0 && sk.w8_launch > 1) {
         const long words = sweep_w8_edge_words(g);
@@ -898,34 +929,64 @@ static bool w8_next_tag(const Geo& g, const SweepSkew<T>& sk, uint32_t* tag, hip
     return true;
 }
 
+// the box's tiles and slots on axis G.ax: tiles [J0, J0 + ntjb) x [K0, K0 + ntkb),
+// slots [s0, s0 + nS); box = (i0, i1, j0, j1, k0, k1)
+struct W8Box { int J0, K0, ntjb, ntkb, s0, nS; };
+static W8Box w8_box(const W8Geo& G, const int* box) {
+    const int* L = G.ax == 2 ? box + 4 : box;      // line axis range
+    const int* A = G.ax == 2 ? box : box + 2;      // A range
+    const int* B = G.ax == 2 ? box + 2 : box + 4;  // B range
+    W8Box w;
+    w.J0 = A[0] / 8;
+    w.K0 = B[0] / 8;
+    w.ntjb = A[1] / 8 - w.J0 + 1;
+    w.ntkb = B[1] / 8 - w.K0 + 1;
+    w.s0 = L[0];
+    const int s1 = L[1] + W8_PAD < G.SL ? L[1] + W8_PAD : G.SL - 1;
+    w.nS = s1 - w.s0 + 1;
+    return w;
+}
+
+// Axis of the lines for a box: each tile-to-tile hand-off on the critical path
+// costs about as much as 20 steps, so the lines run along the axis that leaves
+// the fewest tiles across the box's two other extents (ties: along i).  Lines
+// along k need natural-layout offsets of a whole grid within 31 bits.
+int w8_choose_axis(const Geo& g, const int* box) {
+    long best = -1;
+    int ax = 0;
+    for (int a : {0, 2}) {
+        if (a == 2 && g.N * 8 >= 0x7fff0000L) continue;
+        const W8Box w = w8_box(w8_geo(g, a), box);
+        const long cost = 20L * (w.ntjb + w.ntkb - 2) + w.nS;
+        if (best < 0 || cost < best) { best = cost; ax = a; }
+    }
+    return ax;
+}
+
 bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double sigma, const SweepSkew<double>& sk,
              int* flags, unsigned long long* epoch, hipStream_t st) {
-    int ntj, ntk;
-    w8_tiles(g, &ntj, &ntk);
+    const W8Geo G = w8_geo(g, sk.ax);
     const long n = sweep_w8_size(g);
-    const long ebytes = (long)ntj * ntk * w8_erows(g) * 16 * 16;
+    const long ebytes = (long)G.ntA * G.ntB * w8_erows(G) * 16 * 16;
     uint32_t tag;
-    if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, sk, &tag, st)) return false;
-    k_mic_records<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, sk.oRec);
-    const int J0 = sk.box[2] / 8, J1 = sk.box[3] / 8, K0 = sk.box[4] / 8, K1 = sk.box[5] / 8;
-    const int s0 = sk.box[0], s1 = sk.box[1] + W8_PAD < w8_slots(g) ? sk.box[1] + W8_PAD : w8_slots(g) - 1;
+    if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, G, sk, &tag, st)) return false;
+    k_mic_records<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, G, n, code, sk.oRec);
+    const W8Box w = w8_box(G, sk.box);
     const W8Planes<double> P{nullptr, sk.oQ, sk.oRec, sk.oEdge, nullptr, nullptr};
-    k_sweep_w8<double, true, true><<<(J1 - J0 + 1) * (K1 - K0 + 1), 192, 0, st>>>(
-        g, ntk, J0, K0, J1 - J0 + 1, K1 - K0 + 1, s0, s1 - s0 + 1, P, nullptr, tag, flags, tau, sigma);
+    auto k = sk.ax == 2 ? k_sweep_w8<double, true, true, 2> : k_sweep_w8<double, true, true, 0>;
+    k<<<w.ntjb * w.ntkb, 192, 0, st>>>(G, w.J0, w.K0, w.ntjb, w.ntkb, w.s0, w.nS, P, nullptr, tag, flags, tau, sigma);
     const Box8 bx{sk.box[0], sk.box[1], sk.box[2], sk.box[3], sk.box[4], sk.box[5]};
-    k_from_oct<double><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, ntk, sk.oQ, inv, nullptr, bx);
+    k_from_oct<double><<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, G, sk.oQ, inv, nullptr, bx);
     return true;
 }
 
 template <class T>
 void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st) {
-    int ntj, ntk;
-    w8_tiles(g, &ntj, &ntk);
     const long n = sweep_w8_size(g);
-    k_w8_records<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, ntk, n, code, inv, sk.oRec);
+    k_w8_records<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, w8_geo(g, sk.ax), n, code, inv, sk.oRec);
 }
 
-// whether all `tiles` workgroups of k_sweep_w8_fb<T, DEEP> can be resident at once
+// whether all `tiles` workgroups of k_sweep_w8_fb<T, DEEP, *> can be resident at once
 template <class T, bool DEEP>
 static bool w8_fused_fits(int tiles) {
     static int cap = -1;
@@ -933,80 +994,74 @@ static bool w8_fused_fits(int tiles) {
         int dev = 0, cus = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep_w8_fb<T, DEEP>, 192, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep_w8_fb<T, DEEP, 0>, 192, 0) != hipSuccess)
             cus = per = 0;
         cap = cus * per;
     }
     return tiles <= cap;
 }
 
-// the box's tiles and slots: tiles [J0, J0 + ntjb) x [K0, K0 + ntkb), slots [s0, s0 + nS)
-struct W8Box { int J0, K0, ntjb, ntkb, s0, nS; };
-static W8Box w8_box(const Geo& g, const int* box) {
-    W8Box w;
-    w.J0 = box[2] / 8;
-    w.K0 = box[4] / 8;
-    w.ntjb = box[3] / 8 - w.J0 + 1;
-    w.ntkb = box[5] / 8 - w.K0 + 1;
-    w.s0 = box[0];
-    const int s1 = box[1] + W8_PAD < w8_slots(g) ? box[1] + W8_PAD : w8_slots(g) - 1;
-    w.nS = s1 - w.s0 + 1;
-    return w;
-}
-
-int w8_box_tiles(const Geo& g, const int* box) {
-    const W8Box w = w8_box(g, box);
+int w8_box_tiles(const Geo& g, int ax, const int* box) {
+    const W8Box w = w8_box(w8_geo(g, ax), box);
     return w.ntjb * w.ntkb;
 }
 
 template <class T>
-void w8_dot_tiles(const Geo& g, const int* box, const T* z, const T* r, double* partial, const PcgScalars* S,
+void w8_dot_tiles(const Geo& g, int ax, const int* box, const T* z, const T* r, double* partial, const PcgScalars* S,
                   hipStream_t st) {
-    const W8Box w = w8_box(g, box);
-    k_dot_tiles<T><<<w.ntjb * w.ntkb, 64, 0, st>>>(g, w.J0, w.K0, w.ntkb, w.s0, w.nS, z, r, partial, S);
+    const W8Geo G = w8_geo(g, ax);
+    const W8Box w = w8_box(G, box);
+    k_dot_tiles<T><<<w.ntjb * w.ntkb, 64, 0, st>>>(G, w.J0, w.K0, w.ntkb, w.s0, w.nS, z, r, partial, S);
 }
 
-template <class T>
-bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
-                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
-                      FinRes* fin) {
-    int ntj, ntk;
-    w8_tiles(g, &ntj, &ntk);
-    const long n = sweep_w8_size(g);
-    const long ebytes = (long)ntj * ntk * w8_erows(g) * 16 * 8 * (long)(sizeof(T) / 4);
-    uint32_t tag1, tag2;
-    if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, sk, &tag1, st) ||
-        !w8_next_tag(g, sk, &tag2, st))
-        return false;
-    const W8Box w = w8_box(g, sk.box);
-    const int J0 = w.J0, K0 = w.K0, ntjb = w.ntjb, ntkb = w.ntkb, s0 = w.s0, nS = w.nS;
-    // forward: r (natural) -> (q, r) pairs; backward: pairs -> z (natural) and the
-    // tiles' dot(z, r) partials
-    const W8Planes<T> f{r, sk.oQ, sk.oRec, sk.oEdge, nullptr, nullptr};
-    const W8Planes<T> b{sk.oQ, nullptr, sk.oRec + 4 * n, sk.oEdge, z, dot_partial};
-    const int tiles = ntjb * ntkb;
+template <class T, int AX>
+static void w8_launch(const W8Geo& G, const W8Box& w, bool fused, bool deep, const W8Planes<T>& f,
+                      const W8Planes<T>& b, const PcgScalars* S, uint32_t tag1, uint32_t tag2, int* flags,
+                      FinRes* fin, hipEvent_t t0, hipEvent_t t1, hipStream_t st) {
+    const int tiles = w.ntjb * w.ntkb;
     // t0 / t1 ride on the dispatch packets themselves, so they time the kernels and not
     // the marker packets (and their cache release) that hipEventRecord would add
-    // (FP32: the deep ring when every tile gets a CU of its own)
-    const bool deep = sizeof(T) == 4 && w8_fused_fits<T, sizeof(T) == 4>(tiles);
-    sk.fused = deep || w8_fused_fits<T, false>(tiles) ? 1 : 0;
-    if (sk.fused) {
+    if (fused) {
         const double* fp = fin ? fin->partial : nullptr;
-        hipExtLaunchKernelGGL(deep ? k_sweep_w8_fb<T, sizeof(T) == 4> : k_sweep_w8_fb<T, false>, dim3(tiles),
-                              dim3(192), 0, st, t0, t1, 0, g, ntk, J0, K0, ntjb, ntkb, s0, nS, f, b, S, tag1, tag2,
-                              flags, fp, fin ? fin->n : 0, fin ? fin->it : 0, fin ? fin->S : (PcgScalars*)nullptr,
-                              fin ? fin->mirror : (PcgScalars*)nullptr);
+        hipExtLaunchKernelGGL(deep ? k_sweep_w8_fb<T, sizeof(T) == 4, AX> : k_sweep_w8_fb<T, false, AX>, dim3(tiles),
+                              dim3(192), 0, st, t0, t1, 0, G, w.J0, w.K0, w.ntjb, w.ntkb, w.s0, w.nS, f, b, S, tag1,
+                              tag2, flags, fp, fin ? fin->n : 0, fin ? fin->it : 0,
+                              fin ? fin->S : (PcgScalars*)nullptr, fin ? fin->mirror : (PcgScalars*)nullptr);
         if (fin) fin->ran = true;
     } else {
         if (fin) {
             launch_finish_res(*fin, st);
             fin->ran = true;
         }
-        hipExtLaunchKernelGGL(k_sweep_w8<T, true>, dim3(tiles), dim3(192), 0, st, t0, (hipEvent_t) nullptr, 0, g,
-                              ntk, J0, K0, ntjb, ntkb, s0, nS, f, S, tag1, flags, 0.0, 0.0);
-        hipExtLaunchKernelGGL(k_sweep_w8<T, false>, dim3(tiles), dim3(192), 0, st, (hipEvent_t) nullptr, t1, 0, g,
-                              ntk, J0, K0, ntjb, ntkb, s0, nS, b, S, tag2, flags, 0.0, 0.0);
+        hipExtLaunchKernelGGL(k_sweep_w8<T, true, false, AX>, dim3(tiles), dim3(192), 0, st, t0, (hipEvent_t) nullptr,
+                              0, G, w.J0, w.K0, w.ntjb, w.ntkb, w.s0, w.nS, f, S, tag1, flags, 0.0, 0.0);
+        hipExtLaunchKernelGGL(k_sweep_w8<T, false, false, AX>, dim3(tiles), dim3(192), 0, st, (hipEvent_t) nullptr, t1,
+                              0, G, w.J0, w.K0, w.ntjb, w.ntkb, w.s0, w.nS, b, S, tag2, flags, 0.0, 0.0);
     }
+}
+
+template <class T>
+bool sweep_precond_w8(const Geo& g, const SweepSkew<T>& sk, const T* r, T* z, const PcgScalars* S, int* flags,
+                      unsigned long long* epoch, hipEvent_t t0, hipEvent_t t1, hipStream_t st, double* dot_partial,
+                      FinRes* fin) {
+    const W8Geo G = w8_geo(g, sk.ax);
+    const long n = sweep_w8_size(g);
+    const long ebytes = (long)G.ntA * G.ntB * w8_erows(G) * 16 * 8 * (long)(sizeof(T) / 4);
+    uint32_t tag1, tag2;
+    if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, G, sk, &tag1, st) ||
+        !w8_next_tag(g, G, sk, &tag2, st))
+        return false;
+    const W8Box w = w8_box(G, sk.box);
+    // forward: r (natural) -> (q, r) pairs; backward: pairs -> z (natural) and the
+    // tiles' dot(z, r) partials
+    const W8Planes<T> f{r, sk.oQ, sk.oRec, sk.oEdge, nullptr, nullptr};
+    const W8Planes<T> b{sk.oQ, nullptr, sk.oRec + 4 * n, sk.oEdge, z, dot_partial};
+    const int tiles = w.ntjb * w.ntkb;
+    // (FP32: the deep ring when every tile gets a CU of its own)
+    const bool deep = sizeof(T) == 4 && w8_fused_fits<T, sizeof(T) == 4>(tiles);
+    sk.fused = deep || w8_fused_fits<T, false>(tiles) ? 1 : 0;
+    if (sk.ax == 2) w8_launch<T, 2>(G, w, sk.fused, deep, f, b, S, tag1, tag2, flags, fin, t0, t1, st);
+    else w8_launch<T, 0>(G, w, sk.fused, deep, f, b, S, tag1, tag2, flags, fin, t0, t1, st);
     return true;
 }
 
@@ -1018,7 +1073,7 @@ template bool sweep_precond_w8<double>(const Geo&, const SweepSkew<double>&, con
 template bool sweep_precond_w8<float>(const Geo&, const SweepSkew<float>&, const float*, float*, const PcgScalars*,
                                       int*, unsigned long long*, hipEvent_t, hipEvent_t, hipStream_t, double*,
                                       FinRes*);
-template void w8_dot_tiles<double>(const Geo&, const int*, const double*, const double*, double*, const PcgScalars*,
-                                   hipStream_t);
-template void w8_dot_tiles<float>(const Geo&, const int*, const float*, const float*, double*, const PcgScalars*,
+template void w8_dot_tiles<double>(const Geo&, int, const int*, const double*, const double*, double*,
+                                   const PcgScalars*, hipStream_t);
+template void w8_dot_tiles<float>(const Geo&, int, const int*, const float*, const float*, double*, const PcgScalars*,
                                   hipStream_t);
