@@ -106,20 +106,6 @@ __device__ inline void lds_st64(uint32_t a, uint64_t v) {
     asm volatile("ds_write_b64 %0, %1\n\ts_waitcnt lgkmcnt(0)" ::"v"(a), "v"(v) : "memory");
 }
 
-template <class T>
-__device__ inline T wv_sel(uint32_t m, T keep, T take);
-template <>
-__device__ inline float wv_sel<float>(uint32_t m, float keep, float take) {
-    return __uint_as_float((m & __float_as_uint(keep)) | (~m & __float_as_uint(take)));
-}
-template <>
-__device__ inline double wv_sel<double>(uint32_t m, double keep, double take) {
-    const uint64_t a = (uint64_t)__double_as_longlong(keep), b = (uint64_t)__double_as_longlong(take);
-    const uint32_t lo = (m & (uint32_t)a) | (~m & (uint32_t)b);
-    const uint32_t hi = (m & (uint32_t)(a >> 32)) | (~m & (uint32_t)(b >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
 // lane-1 (forward) / lane+1 (backward) inside each 16-lane row
 template <bool FWD>
 __device__ inline int wv_dpp(int x) {
@@ -160,8 +146,7 @@ typedef uint32_t wv_u32x2 __attribute__((ext_vector_type(2)));
 // lane, lane = jl + 8*kl, SL = ni + 14; cell (i, 8J + jl, 8K + kl) with
 // i = slot - jl - kl.  Records: four values per element, {L0, L1, L2, pivot}
 // (forward -k, -j, -i; backward +i, +j, +k).  An absent entry is stored as +0
-// and a present one as A_cm * inv (never positive: its sign bit marks the
-// entry present).  Edge rows: 16 words per tile and step, 0-7 the edge column
+// (its term then leaves the sum's value as it is) and a present one as A_cm * inv.  Edge rows: 16 words per tile and step, 0-7 the edge column
 // by kl, 8-15 the edge row by jl, whose corner cell is not repeated there
 // (readers take it from the column word).  An edge word's tag is the sweep's
 // tag base plus the producing step (w8_next_tag), so a ring slot's copy of an
@@ -313,10 +298,6 @@ struct W8Planes {
     double* dot;     // backward: per-tile partials of dot(z, r), or null
 };
 
-
-// all-ones where the record's coefficient is present (sign bit set)
-__device__ inline uint32_t w8_present(float L) { return (uint32_t)(__float_as_int(L) >> 31); }
-__device__ inline uint32_t w8_present(double L) { return (uint32_t)((int)(__double_as_longlong(L) >> 32) >> 31); }
 
 // edge-row word holding the upstream's value for lane jl of the edge row
 template <bool FWD>
@@ -673,7 +654,7 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
         }
         qj = eJ ? ew_val(wj) : qj;
         qk = eK ? ew_val(wk) : qk;
-        T v = cur.rin, w, q;
+        T v = cur.rin, q;
         // the neighbour values in the records' direction order (forward -k, -j, -i,
         // backward +i, +j, +k): qp is the line's previous cell, qj the A neighbour,
         // qk the B neighbour
@@ -699,20 +680,24 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
             // neighbour (permuted across rows, the last to arrive) last;
             // k_fwd_plane / k_bwd_plane<float> use the same order, so the fallback
             // gives the same bits.  (Record index of the own, A and B entries.)
+            // An absent entry is stored as +0 and every neighbour value here is
+            // finite (zeros outside the grid and the box), so its term leaves v's
+            // value as it is: no test for presence.
             constexpr int IO = FWD ? (AX == 2 ? 0 : 2) : (AX == 2 ? 2 : 0);
             constexpr int IA = FWD ? (AX == 2 ? 2 : 1) : (AX == 2 ? 0 : 1);
             constexpr int IB = 3 - IO - IA;
             const T Ls[3] = {cur.L0, cur.L1, cur.L2};
-            v = wv_sel<T>(w8_present(Ls[IO]), __builtin_fmaf(-Ls[IO], qp, v), v);
-            v = wv_sel<T>(w8_present(Ls[IA]), __builtin_fmaf(-Ls[IA], qj, v), v);
-            v = wv_sel<T>(w8_present(Ls[IB]), __builtin_fmaf(-Ls[IB], qk, v), v);
+            v = __builtin_fmaf(-Ls[IO], qp, v);
+            v = __builtin_fmaf(-Ls[IA], qj, v);
+            v = __builtin_fmaf(-Ls[IB], qk, v);
             q = v * cur.ic;
         } else {
             // FP64: the reference's order (forward c-nn, c-ni, c-1; backward c+1,
-            // c+ni, c+nn)
-            w = v - cur.L0 * n0; v = wv_sel<T>(w8_present(cur.L0), w, v);
-            w = v - cur.L1 * n1; v = wv_sel<T>(w8_present(cur.L1), w, v);
-            w = v - cur.L2 * n2; v = wv_sel<T>(w8_present(cur.L2), w, v);
+            // c+ni, c+nn), each term rounded as there; an absent entry (+0, with a
+            // finite neighbour value) leaves v's value as it is
+            v = v - cur.L0 * n0;
+            v = v - cur.L1 * n1;
+            v = v - cur.L2 * n2;
             q = v * cur.ic;
         }
         const uint64_t qb = NS == 1 ? (uint64_t)__float_as_uint((float)q) : (uint64_t)__double_as_longlong((double)q);
