@@ -701,6 +701,15 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
             q = v * cur.ic;
         }
         const uint64_t qb = NS == 1 ? (uint64_t)__float_as_uint((float)q) : (uint64_t)__double_as_longlong((double)q);
+        // edge words first (the downstream tiles wait for them; the output store can
+        // wait): every lane stores, those off the edge out of the buffer's range
+        const uint32_t t = tagbase + (uint32_t)s;
+        if constexpr (NS == 1) {
+            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, t}, erc, estore + u * ROWB, sb * ROWB, WV_SC1);
+        } else {
+            __builtin_amdgcn_raw_buffer_store_b128(wv_u32x4{(uint32_t)qb, t, (uint32_t)(qb >> 32), t}, erc,
+                                                   estore + u * ROWB, sb * ROWB, WV_SC1);
+        }
         if constexpr (FWD) {
             // (q, r) pairs (the pivots: q), step by step in the tile's plane
             constexpr int SB = 64 * OPC * (int)sizeof(T);
@@ -724,14 +733,6 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
             if constexpr (NS == 1) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)qb, zrc, zo, 0, 0);
             else __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, (uint32_t)(qb >> 32)}, zrc, zo, 0, 0);
             dacc = dacc + (double)q * (double)cur.rd;
-        }
-        // edge words: every lane stores, those off the edge out of the buffer's range
-        const uint32_t t = tagbase + (uint32_t)s;
-        if constexpr (NS == 1) {
-            __builtin_amdgcn_raw_buffer_store_b64(wv_u32x2{(uint32_t)qb, t}, erc, estore + u * ROWB, sb * ROWB, WV_SC1);
-        } else {
-            __builtin_amdgcn_raw_buffer_store_b128(wv_u32x4{(uint32_t)qb, t, (uint32_t)(qb >> 32), t}, erc,
-                                                   estore + u * ROWB, sb * ROWB, WV_SC1);
         }
         qp = q;
     };
