@@ -385,10 +385,10 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
             const double s2 = ex * ex + ey * ey + ez * ez;
             cw = wyvill(W, s2);
         }
-        if (cw != 0.0) {
-            wsum += cw;
-            vsum += v * cw;
-        }
+        // (a pair outside the support has weight +0, which leaves both sums' values as
+        // they are: no branch)
+        wsum += cw;
+        vsum += v * cw;
     };
     const int nrow = chi[1] - clo[1] + 1, ncell = chi[0] - clo[0] + 1, nsl = chi[2] - clo[2] + 1;
     // the cell starts of every candidate slab row
@@ -444,9 +444,18 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
                 if (fits) {
                     const int base = roff[ks][jr] - raw[ks][jr][0];
                     const int pa = base + raw[ks][jr][lo[0] - clo[0]], pb = base + raw[ks][jr][hi[0] + 1 - clo[0]];
-                    for (int p = pa; p < pb; ++p) {
-                        const d2 a = lp[2 * p], b = lp[2 * p + 1];
-                        pair(a.x, a.y, b.x, b.y);
+                    // four particles' LDS reads in flight at once, then added in index order
+                    int p = pa;
+                    for (; p + 4 <= pb; p += 4) {
+                        d2 a[4], c[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) { a[u] = lp[2 * (p + u)]; c[u] = lp[2 * (p + u) + 1]; }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) pair(a[u].x, a[u].y, c[u].x, c[u].y);
+                    }
+                    for (; p < pb; ++p) {
+                        const d2 a = lp[2 * p], c = lp[2 * p + 1];
+                        pair(a.x, a.y, c.x, c.y);
                     }
                 } else {
                     const long row = (long)g.ni * j + g.nn * k;
