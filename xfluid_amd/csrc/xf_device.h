@@ -196,37 +196,46 @@ __device__ inline void sample(const Geo& g, const Faces& F, const double* p, dou
     out[2] = trilerp8(t, x, y, z);
 }
 
-// The cells of the planes k0..k1 and rows j0..j1 (all i) of the active rows
-// bounding box, walked by a grid-stride loop without a division per cell: cell
-// t is base + (t / blk) nn + t % blk; a thread steps t by `stride` (stride / blk
-// and stride % blk precomputed).  Outside these rows every PCG vector is zero.
+// The cells of the active rows' bounding box (i0..i1, j0..j1, k0..k1), walked
+// by a grid-stride loop.  Outside the box every PCG vector is zero.
 struct BoxRange {
-    long base, nn;
-    unsigned blk, n, sdiv, smod;
+    long base, ni, nn;                // first cell (i0, j0, k0) and the grid's strides
+    unsigned iw, jw, n, si, sj, sk;   // box extents in i and j, its cells, the stride in (i, j, k) digits
 };
 __host__ __device__ inline BoxRange box_range(long ni, long nj, long nk, const int box[6], unsigned stride) {
     (void)nk;
     BoxRange r;
+    r.ni = ni;
     r.nn = ni * nj;
-    r.base = (long)box[4] * r.nn + (long)box[2] * ni;
-    r.blk = (unsigned)((box[3] - box[2] + 1) * ni);
-    r.n = r.blk * (unsigned)(box[5] - box[4] + 1);
-    r.sdiv = stride / r.blk;
-    r.smod = stride % r.blk;
+    r.base = (long)box[4] * r.nn + (long)box[2] * ni + box[0];
+    r.iw = (unsigned)(box[1] - box[0] + 1);
+    r.jw = (unsigned)(box[3] - box[2] + 1);
+    r.n = r.iw * r.jw * (unsigned)(box[5] - box[4] + 1);
+    r.si = stride % r.iw;
+    r.sj = (stride / r.iw) % r.jw;
+    r.sk = stride / (r.iw * r.jw);
     return r;
 }
-
-// walk of a BoxRange: position t as (t / blk, t % blk)
-__device__ inline void bw_init(const BoxRange& b, unsigned t, unsigned& kq, unsigned& off) {
-    kq = t / b.blk;
-    off = t - kq * b.blk;
+// The walk over the box's cells in (k, j, i) order: sequence number t is the
+// mixed-radix number (kq, jq, iq); a grid-stride step adds the stride's digits
+// with carries (no division per cell).
+struct BoxWalk { unsigned kq, jq, iq; };
+__device__ inline void bw_init(const BoxRange& b, unsigned t, BoxWalk& w) {
+    w.iq = t % b.iw;
+    const unsigned q = t / b.iw;
+    w.jq = q % b.jw;
+    w.kq = q / b.jw;
 }
-__device__ inline void bw_next(const BoxRange& b, unsigned& kq, unsigned& off) {
-    kq += b.sdiv;
-    off += b.smod;
-    if (off >= b.blk) { off -= b.blk; ++kq; }
+__device__ inline void bw_next(const BoxRange& b, BoxWalk& w) {
+    w.iq += b.si;
+    w.jq += b.sj;
+    w.kq += b.sk;
+    if (w.iq >= b.iw) { w.iq -= b.iw; ++w.jq; }
+    if (w.jq >= b.jw) { w.jq -= b.jw; ++w.kq; }
 }
-__device__ inline long bw_cell(const BoxRange& b, unsigned kq, unsigned off) { return b.base + (long)kq * b.nn + off; }
+__device__ inline long bw_cell(const BoxRange& b, const BoxWalk& w) {
+    return b.base + (long)w.kq * b.nn + (long)w.jq * b.ni + w.iq;
+}
 
 // collision.cpp:293-385 (voxel walk to the first solid cell)
 __device__ inline bool voxel_hit(const Geo& g, const uint8_t* mat, const double* a, const double* b, int* vox) {

@@ -335,8 +335,10 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
     if (S && S->done) return;
     double acc = 0.0;
     const unsigned stride = gridDim.x * blockDim.x;
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
-    bw_init(bx, t, kq, off);
+    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
+    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    BoxWalk w;
+    bw_init(bx, t, w);
     // four cells of the grid-stride sequence at a time (the same cells, in the
     // same order, as one at a time)
     for (; t + 3 * stride < bx.n; t += 4 * stride) {
@@ -345,8 +347,8 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
         T v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            cs[u] = bw_cell(bx, kq, off);
-            bw_next(bx, kq, off);
+            cs[u] = bw_cell(bx, w);
+            bw_next(bx, w);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) cc[u] = code[cs[u]];
@@ -359,8 +361,8 @@ __global__ void k_spmv_dot(Geo g, const uint16_t* __restrict__ code, const T* __
         }
     }
     for (; t < bx.n; t += stride) {
-        const long c = bw_cell(bx, kq, off);
-        bw_next(bx, kq, off);
+        const long c = bw_cell(bx, w);
+        bw_next(bx, w);
         T v = spmv_cell<T>(g, code[c], c, x);
         y[c] = v;
         acc += (double)x[c] * (double)v;
@@ -434,11 +436,13 @@ __global__ void k_axpy_max(BoxRange bx, T* __restrict__ x, T* __restrict__ r, co
     const T a = (T)alpha, na = (T)(-alpha);
     double m = 0.0;
     const unsigned stride = gridDim.x * blockDim.x;
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
-    bw_init(bx, t, kq, off);
+    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
+    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    BoxWalk w;
+    bw_init(bx, t, w);
     for (; t < bx.n; t += stride) {
-        const long c = bw_cell(bx, kq, off);
-        bw_next(bx, kq, off);
+        const long c = bw_cell(bx, w);
+        bw_next(bx, w);
         x[c] += a * s[c];
         T rv = r[c] + na * z[c];
         r[c] = rv;
@@ -456,11 +460,13 @@ __global__ void k_dot(BoxRange bx, const T* __restrict__ a, const T* __restrict_
     if (S && S->done) return;
     double acc = 0.0;
     const unsigned stride = gridDim.x * blockDim.x;
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
-    bw_init(bx, t, kq, off);
+    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
+    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    BoxWalk w;
+    bw_init(bx, t, w);
     for (; t < bx.n; t += stride) {
-        const long c = bw_cell(bx, kq, off);
-        bw_next(bx, kq, off);
+        const long c = bw_cell(bx, w);
+        bw_next(bx, w);
         acc += (double)a[c] * (double)b[c];
     }
     acc = block_reduce<double, false>(acc, sh);
@@ -503,16 +509,18 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
     auto sv = [=](long e) __attribute__((always_inline)) { return z[e] + beta * s_old[e]; };
     double acc = 0.0;
     const unsigned stride = gridDim.x * blockDim.x;
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x, kq, off;
-    bw_init(bx, t, kq, off);
+    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
+    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    BoxWalk w;
+    bw_init(bx, t, w);
     for (; t + 3 * stride < bx.n; t += 4 * stride) {
         long cs[4];
         uint16_t cc[4];
         T yv[4], sc[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            cs[u] = bw_cell(bx, kq, off);
-            bw_next(bx, kq, off);
+            cs[u] = bw_cell(bx, w);
+            bw_next(bx, w);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) cc[u] = code[cs[u]];
@@ -529,8 +537,8 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
         }
     }
     for (; t < bx.n; t += stride) {
-        const long c = bw_cell(bx, kq, off);
-        bw_next(bx, kq, off);
+        const long c = bw_cell(bx, w);
+        bw_next(bx, w);
         const T yv = spmv_cell_g<T>(g, code[c], c, sv), sc = sv(c);
         s_new[c] = sc;
         y[c] = yv;
