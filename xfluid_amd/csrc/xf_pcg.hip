@@ -94,16 +94,18 @@ __global__ void k_fill(long n, T* __restrict__ d, T v);
 // Active rows' bounding box: one block-level min/max per block, then atomics.
 __global__ void k_active_box(Geo g, const uint16_t* __restrict__ code, int* box) {
     __shared__ int red[6][XF_BLOCK / 64];
-    const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     int v[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
-    if (c < g.N) {
+    // grid-stride: few blocks, so few atomics
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
         const uint16_t cc = code[c];
         bool act = code_diag(cc) != 0.0;
         for (int d = 0; d < 6; ++d) act = act || present(code_dir(cc, d));
         if (act) {  // (32-bit index arithmetic: N < 2^30)
             const unsigned cu = (unsigned)c, row = cu / (unsigned)g.ni;
             const int i = (int)(cu - row * (unsigned)g.ni), j = (int)(row % (unsigned)g.nj), k = (int)(row / (unsigned)g.nj);
-            v[0] = v[1] = i; v[2] = v[3] = j; v[4] = v[5] = k;
+            v[0] = i < v[0] ? i : v[0]; v[1] = i > v[1] ? i : v[1];
+            v[2] = j < v[2] ? j : v[2]; v[3] = j > v[3] ? j : v[3];
+            v[4] = k < v[4] ? k : v[4]; v[5] = k > v[5] ? k : v[5];
         }
     }
     for (int o = 32; o > 0; o >>= 1)
@@ -134,7 +136,7 @@ __global__ void k_box_init(int* box) {
 
 bool pcg_active_box(const Geo& g, const uint16_t* code, PcgWork& w, int box[6], hipStream_t st) {
     k_box_init<<<1, 64, 0, st>>>(w.box_dev);
-    k_active_box<<<xf_blocks(g.N), XF_BLOCK, 0, st>>>(g, code, w.box_dev);
+    k_active_box<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.box_dev);
     hipMemcpyAsync(w.box_host, w.box_dev, 6 * sizeof(int), hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
     for (int d = 0; d < 6; ++d) box[d] = w.box_host[d];

@@ -217,6 +217,20 @@ __device__ inline bool w8_cell(const W8Geo& G, long e, int* i, int* j, int* k) {
 
 struct Box8 { int i0, i1, j0, j1, k0, k1; };
 // cells outside the active box are zero (the sweep did not visit their slots)
+// The elements the sweeps over a box read: its tiles' slots [s0, s0 + nS).
+// Thread t of a launch over ntjb * ntkb * nS * 64 threads -> element e.
+struct W8BoxArgs { int J0, K0, ntkb, s0, nS, ntjb; };
+__device__ inline bool w8_box_element(const W8Geo& G, const W8BoxArgs& b, long t, long* e) {
+    const long total = (long)b.ntjb * b.ntkb * b.nS * 64;
+    if (t >= total) return false;
+    const int lane = (int)(t & 63);
+    const long r = t >> 6;
+    const int sr = (int)(r % b.nS), tb = (int)(r / b.nS);
+    const long tile = (long)(b.J0 + tb / b.ntkb) * G.ntB + (b.K0 + tb % b.ntkb);
+    *e = (tile * G.SL + b.s0 + sr) * 64 + lane;
+    return true;
+}
+
 template <class T>
 __global__ void k_from_oct(Geo g, W8Geo G, const T* __restrict__ src, T* __restrict__ dst, const PcgScalars* S,
                            Box8 b) {
@@ -259,10 +273,10 @@ __global__ __launch_bounds__(64) void k_dot_tiles(W8Geo G, int J0, int K0, int n
 
 // records: rec[4e..4e+3] forward, rec[4n + 4e..] backward
 template <class T>
-__global__ void k_w8_records(Geo g, W8Geo G, long n, const uint16_t* __restrict__ code, const T* __restrict__ inv,
-                             T* __restrict__ rec) {
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e >= n) return;
+__global__ void k_w8_records(Geo g, W8Geo G, long n, W8BoxArgs bx, const uint16_t* __restrict__ code,
+                             const T* __restrict__ inv, T* __restrict__ rec) {
+    long e;
+    if (!w8_box_element(G, bx, blockIdx.x * (long)blockDim.x + threadIdx.x, &e)) return;
     int i, j, k;
     T lf[3] = {0, 0, 0}, lb[3] = {0, 0, 0}, ic = 0;
     if (w8_cell(G, e, &i, &j, &k)) {
@@ -862,9 +876,10 @@ __global__ __launch_bounds__(192) void k_sweep_w8_fb(W8Geo G, int J0, int K0, in
 // cM2, 0} per element, directions -k, -j, -i; cM sums the neighbour's
 // coefficients in the two directions k_mic_plane drops (exact halves, so
 // cM * inv rounds like the sum of the two products).
-__global__ void k_mic_records(Geo g, W8Geo G, long n, const uint16_t* __restrict__ code, double* __restrict__ rec) {
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e >= n) return;
+__global__ void k_mic_records(Geo g, W8Geo G, W8BoxArgs bx, const uint16_t* __restrict__ code,
+                              double* __restrict__ rec) {
+    long e;
+    if (!w8_box_element(G, bx, blockIdx.x * (long)blockDim.x + threadIdx.x, &e)) return;
     int i, j, k;
     double cL[3] = {0, 0, 0}, cM[3] = {0, 0, 0}, diag = 0;
     if (w8_cell(G, e, &i, &j, &k)) {
@@ -956,8 +971,9 @@ bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double
     const long ebytes = (long)G.ntA * G.ntB * w8_erows(G) * 16 * 16;
     uint32_t tag;
     if (!flags || !sk.oRec || ebytes >= 0x7fffffffL || !w8_next_tag(g, G, sk, &tag, st)) return false;
-    k_mic_records<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, G, n, code, sk.oRec);
     const W8Box w = w8_box(G, sk.box);
+    const W8BoxArgs ba{w.J0, w.K0, w.ntkb, w.s0, w.nS, w.ntjb};
+    k_mic_records<<<xf_blocks((long)w.ntjb * w.ntkb * w.nS * 64), XF_BLOCK, 0, st>>>(g, G, ba, code, sk.oRec);
     const W8Planes<double> P{nullptr, sk.oQ, sk.oRec, sk.oEdge, nullptr, nullptr};
     auto k = sk.ax == 2 ? k_sweep_w8<double, true, true, 2> : k_sweep_w8<double, true, true, 0>;
     k<<<w.ntjb * w.ntkb, 192, 0, st>>>(G, w.J0, w.K0, w.ntjb, w.ntkb, w.s0, w.nS, P, nullptr, tag, flags, tau, sigma);
@@ -968,8 +984,13 @@ bool mic0_w8(const Geo& g, const uint16_t* code, double* inv, double tau, double
 
 template <class T>
 void sweep_prepare_w8(const Geo& g, const uint16_t* code, const T* inv, const SweepSkew<T>& sk, hipStream_t st) {
+    // (only the elements of the box's tiles and slots: the sweeps read no others)
     const long n = sweep_w8_size(g);
-    k_w8_records<T><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, w8_geo(g, sk.ax), n, code, inv, sk.oRec);
+    const W8Geo G = w8_geo(g, sk.ax);
+    const W8Box w = w8_box(G, sk.box);
+    const W8BoxArgs ba{w.J0, w.K0, w.ntkb, w.s0, w.nS, w.ntjb};
+    k_w8_records<T><<<xf_blocks((long)w.ntjb * w.ntkb * w.nS * 64), XF_BLOCK, 0, st>>>(g, G, n, ba, code, inv,
+                                                                                       sk.oRec);
 }
 
 // whether all `tiles` workgroups of k_sweep_w8_fb<T, DEEP, *> can be resident at once
