@@ -84,15 +84,18 @@ def run_gpu(xf, cfg, precision, steps, warmup, dt, sweep_kernel=0, pressure=0):
 
 
 def measured_traffic(cfg):
-    """HBM bytes per sweep launch from the committed counter summary (FETCH_SIZE
-    and WRITE_SIZE of the 128^3 run, scaled by a plain copy), or None."""
+    """(HBM bytes per active sweep launch, mean box cells of the counted run) from
+    the committed counter summary (FETCH_SIZE and WRITE_SIZE of the 128^3 run,
+    scaled by a plain copy), or (None, None)."""
     path = os.path.join(ROOT, "profiles", "sweep_traffic_dam128.md")
     if cfg != "dam128" or not os.path.exists(path):
-        return None
+        return None, None
+    got = {}
     for line in open(path):
-        if line.startswith("sweep_traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
+        for key in ("sweep_traffic_bytes_per_launch", "sweep_traffic_box_cells"):
+            if line.startswith(key + ":"):
+                got[key] = int(line.split(":")[1])
+    return got.get("sweep_traffic_bytes_per_launch"), got.get("sweep_traffic_box_cells")
 
 
 def host_cores():
@@ -576,6 +579,10 @@ def main():
     box = mixed["box_cells"]
     ach_sweep = 28.0 * box / (sweep_us * 1e-6) / 1e9 if apps else 0.0
     ach_sweep_full = 28.0 * N / (sweep_us * 1e-6) / 1e9 if apps else 0.0
+    # Counted bytes scale with the active box (tiles, slots and padding all do):
+    # the counter run's bytes per box cell times this run's mean box.
+    traffic_counted, traffic_box = measured_traffic(args.config) if args.sweep in (0, 9) else (None, None)
+    traffic = int(traffic_counted * box / traffic_box) if traffic_counted and traffic_box and box else None
     # whole-solve rate (algorithmic bytes per PCG iteration: FP64 131 B/cell,
     # FP32 bulk 67 B/cell over the active box)
     it_m = max(mixed["iters"], 1)
@@ -605,7 +612,7 @@ def main():
                      "kernel": "k_sweep_w8_fb (forward + backward MIC(0) sweep, FP32 in the mixed solve)",
                      "achieved": round(ach_sweep, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_sweep / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(args.config) if args.sweep in (0, 9) else None,
+                     "traffic": traffic, "traffic_counted": traffic_counted, "traffic_counted_box_cells": traffic_box,
                      "bytes_per_launch": int(28 * box), "box_cells": int(box), "avg_launch_us": round(sweep_us, 2),
                      "launches_timed": apps,
                      "achieved_full_grid": round(ach_sweep_full, 2), "bytes_per_launch_full_grid": 28 * N},
