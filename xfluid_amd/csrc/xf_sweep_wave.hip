@@ -341,21 +341,25 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
 // tile's last 7 steps wait for those, for cells outside the box whose entries
 // are absent, so every step checks its edge words the same way.
 //
-// The ring holds R steps, the records of LEAD steps in flight (32 and 24 for
-// the FP32 sweeps, 32 and 16 for FP64, 16 and 12 for the pivots, whose records
-// are twice as long).  Record loads take some 4-8 us under the sweep's load, so
-// with 32 slots the record wave, which may run at most R - 1 steps ahead of the
-// compute wave, holds an FP32 tile back to about 0.2 us a step.  The deep ring
-// (DEEP: 64 slots, 31 steps in flight, the most one wave's 6-bit vmcnt allows
-// at two loads a step) takes 99 KB of LDS, so one workgroup per CU; the host
-// uses it when every tile fits.  The compute wave runs in blocks
-// of W8_B steps unrolled so that every ring slot sits at a fixed LDS offset
-// from the block's base.  With one wave per SIMD a step costs about
-// its instruction count times the issue interval, so the step is kept short:
-// lanes without an upstream edge read the slot's dummy word (value 0, the
-// step's tag), which makes the tag check two compares for every lane; the
-// record count is checked every 4 steps and the consumed count published every
-// 4 steps (which needs R - LEAD >= 4 slots of slack).
+// The ring holds R steps (32; 16 for the pivots, whose records are twice as
+// long; 64 for the fused FP32 sweep when every tile has a CU of its own, DEEP,
+// 99 KB of LDS, one workgroup per CU), the records of LEAD = 8 steps in
+// flight.  The record wave issues a step's requests in about 0.1 us, so the
+// first step's records arrive some 0.4 us + 0.1 us x LEAD after the wave
+// starts (device-clock traces: 1.2 us at 8, 3.5 us at 31 steps in flight),
+// while 8 steps in flight keep the records ahead of a compute wave stepping
+// at 0.16 us (fewer in flight measured the same step time).  The compute wave
+// runs in blocks of W8_B steps unrolled so that every ring slot sits at a
+// fixed LDS offset from the block's base.  A step is bound by latency (an LDS
+// exchange and the dependent multiply-add chain), not by issue: eight more
+// independent instructions per step left its time unchanged.  Lanes without an
+// upstream edge read the slot's dummy word (value 0, the step's tag), which
+// makes the tag check two compares for every lane; the record count is checked
+// every 4 steps and the consumed count published every 4 steps (which needs
+// R - LEAD >= 4 slots of slack).  The record wave's waits are the explicit
+// vm_wait counts: a compiler-inserted full vmcnt wait before its publishing
+// LDS store (a register allocation that reuses a load's registers can cause
+// one) would drain every step's requests and double the step time.
 //
 // MIC: the same pipeline computes the MIC(0) pivots (factor_modified_incomplete_
 // cholesky0, pcg_solver.h:85-186, in the order of k_mic_plane): the value
@@ -375,7 +379,9 @@ struct W8Ring {
 
 // One sweep of tile (J, K) by the workgroup's three waves (ring_ and ctr_ are
 // the workgroup's LDS).
-template <class T, bool FWD, bool MIC, bool DEEP, int AX>
+// PRE: the record wave's loads of steps 0 .. LEAD - 1 were issued already (by
+// w8_prefetch with the same arguments), so its loop starts at step 0.
+template <class T, bool FWD, bool MIC, bool DEEP, int AX, bool PRE = false>
 __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, int J, int K, int J0, int K0, int ntjb,
                                                                int ntkb, int s0, int nS, W8Planes<T> P,
                                                                uint32_t tagbase, int* err, double tau, double sigma,
@@ -383,7 +389,7 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
     static_assert(!MIC || (FWD && sizeof(T) == 8), "pivots: forward, FP64");
     constexpr int NS = sizeof(T) / 4;
     constexpr int R = W8Ring<T, MIC, DEEP>::R;                      // ring depth in steps
-    constexpr int LEAD = MIC ? 12 : DEEP ? 31 : NS == 1 ? 24 : 16;  // steps of record loads in flight
+    constexpr int LEAD = 8;  // steps of record loads in flight
     static_assert((R & (R - 1)) == 0 && R % W8_B == 0 && LEAD + 4 <= R, "ring depth");
     constexpr int EWB = 8 * NS;                  // bytes per tagged edge word
     constexpr int ROWB = 16 * EWB;               // bytes per edge row
@@ -432,7 +438,7 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
         // step's part sL * t is wave-uniform, so no per-lane multiply
         const T* rrow = P.in + (rjk ? G.sA * (8 * J + rjl) + G.sB * (8 * K + rkl) - G.sL * (rjl + rkl) : 0);
         const int rl0 = rjl + rkl;
-        for (int x = -LEAD; x < nS; ++x) {
+        for (int x = PRE ? 0 : -LEAD; x < nS; ++x) {
             if (x >= 0) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 // (steps x + 1 .. x + LEAD - 1 may stay in flight)
@@ -813,6 +819,36 @@ __global__ __launch_bounds__(192) void k_sweep_w8(W8Geo G, int J0, int K0, int n
                                      ctr_);
 }
 
+// The forward sweep's record wave requests for steps 0 .. LEAD - 1 (records and
+// natural-layout input into their ring slots), as its loop in w8_sweep issues
+// them; a kernel issues these before its own prologue and then runs
+// w8_sweep<..., PRE = true>.  Nothing the sweep computes is read.
+template <class T, bool DEEP>
+__device__ __attribute__((always_inline)) inline void w8_prefetch(const W8Geo& G, int J, int K, int s0, int nS,
+                                                                  const W8Planes<T>& P, unsigned char* ring_) {
+    constexpr int NS = sizeof(T) / 4, LEAD = 8, RECB = 64 * 4 * sizeof(T);
+    constexpr int R = W8Ring<T, false, DEEP>::R, SLOT = W8Ring<T, false, DEEP>::SLOT;
+    if ((threadIdx.x >> 6) != 1) return;
+    const int lane = threadIdx.x & 63, rjl = lane & 7, rkl = lane >> 3, rl0 = rjl + rkl;
+    const bool rjk = 8 * J + rjl < G.nA && 8 * K + rkl < G.nB;
+    const T* rrow = P.in + (rjk ? G.sA * (8 * J + rjl) + G.sB * (8 * K + rkl) - G.sL * (rjl + rkl) : 0);
+    const long myTile = (long)J * G.ntB + K;
+    lds_u8* ring = (lds_u8*)ring_;
+    for (int st = 0; st < LEAD && st < nS; ++st) {
+        const int t = s0 + st;
+        const long e0 = (myTile * G.SL + t) * 64;
+        lds_u8* dst = ring + (st % R) * SLOT;
+#pragma unroll
+        for (int h = 0; h < RECB / 1024; ++h)
+            __builtin_amdgcn_global_load_lds((const unsigned char*)(P.rec + 4 * e0) + h * 1024 + lane * 16,
+                                             dst + h * 1024, 16, 0, 0);
+        const bool ok = rjk & ((unsigned)(t - rl0) < (unsigned)G.nL);
+        const unsigned char* a = (const unsigned char*)(ok ? rrow + G.sL * t : P.in);
+        __builtin_amdgcn_global_load_lds(a, dst + RECB, 4, 0, 0);
+        if (NS == 2) __builtin_amdgcn_global_load_lds(a + 4, dst + RECB + 256, 4, 0, 0);
+    }
+}
+
 // Forward and backward sweep in one launch: each workgroup sweeps its tile
 // forward, then the same tile backward (the tile that ends the forward sweep
 // begins the backward one, with no launch in between).  Every tile waits on
@@ -825,13 +861,20 @@ __global__ __launch_bounds__(192) void k_sweep_w8_fb(W8Geo G, int J0, int K0, in
                                                      int fin_n, int fin_it, PcgScalars* Sw, PcgScalars* mirror) {
     __shared__ __attribute__((aligned(16))) unsigned char ring_[W8Ring<T, false, DEEP>::R * W8Ring<T, false, DEEP>::SLOT];
     __shared__ int ctr_[2];
+    int J, K;
+    w8_tile_xcd(blockIdx.x, ntjb, ntkb, &J, &K);
+    // the forward sweep's first records and input are requested before the
+    // residual test, which they do not depend on: their latency overlaps its own
+    w8_prefetch<T, DEEP>(G, J + J0, K + K0, s0, nS, Pf, ring_);
     if (fin_partial) {
         // the iteration's residual test (k_finish FIN_RES): every workgroup reduces the
-        // partials in the same order and agrees on `stop`; workgroup 0 records it
+        // partials in the same order and agrees on `stop`; the workgroup of the tile
+        // that starts last (it waits on every other tile anyway) records it
         __shared__ double sh[16];
         __shared__ int stop;
-        // all of a thread's loads in flight at once (the partials were just written
-        // by another kernel, so each load is a round trip to memory)
+        // all of a thread's loads in flight at once, the solver state's among them
+        // (the partials were just written by another kernel, so each load is a
+        // round trip to memory)
         constexpr int FU = 6;
         double v[FU];
 #pragma unroll
@@ -839,31 +882,38 @@ __global__ __launch_bounds__(192) void k_sweep_w8_fb(W8Geo G, int J0, int K0, in
             const int q = threadIdx.x + u * 192;
             v[u] = q < fin_n ? fin_partial[q] : 0.0;
         }
+        int was = 0;
+        double tol = 0.0;
+        if (threadIdx.x == 0) {
+            was = Sw->done;
+            tol = Sw->tol;
+        }
         double m = 0.0;
 #pragma unroll
         for (int u = 0; u < FU; ++u) m = v[u] > m ? v[u] : m;
         for (int q = threadIdx.x + FU * 192; q < fin_n; q += 192) m = fin_partial[q] > m ? fin_partial[q] : m;
         m = block_reduce<double, true>(m, sh);
         if (threadIdx.x == 0) {
-            const bool was = Sw->done != 0;
-            stop = was || m <= Sw->tol;
-            if (blockIdx.x == 0) {
+            stop = was != 0 || m <= tol;
+            if (J == ntjb - 1 && K == ntkb - 1) {
                 if (!was) {
                     Sw->res = m;
                     Sw->it = fin_it + 1;
-                    if (m <= Sw->tol) { Sw->done = 1; Sw->iters = fin_it + 1; }
+                    if (m <= tol) { Sw->done = 1; Sw->iters = fin_it + 1; }
                 }
                 if (mirror) *mirror = *Sw;
             }
         }
         __syncthreads();
-        if (stop) return;
+        if (stop) {
+            vm_wait<0>();  // (the requests land before the workgroup ends)
+            return;
+        }
     } else if (S && S->done) {
+        vm_wait<0>();
         return;
     }
-    int J, K;
-    w8_tile_xcd(blockIdx.x, ntjb, ntkb, &J, &K);
-    w8_sweep<T, true, false, DEEP, AX>(G, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_,
+    w8_sweep<T, true, false, DEEP, AX, true>(G, J + J0, K + K0, J0, K0, ntjb, ntkb, s0, nS, Pf, tag1, err, 0.0, 0.0, ring_,
                                    ctr_);
     // this tile's forward output is its backward input: stores complete, LDS free
     __builtin_amdgcn_s_waitcnt(0);
