@@ -339,8 +339,11 @@ void launch_source(const Geo& g, Particles& P, uint8_t* mat, const int32_t* cell
 #endif
 #define P2G_BI 8
 #define P2G_MAXC 12  // cells along i in a slab row (8 faces + support)
+// (KERN: 0 Wyvill, 1 trilinear; one kernel each, so the pair loop carries only
+// its own weight's registers)
+template <int KERN>
 __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
-    Geo g, int axis, int kern, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
+    Geo g, int axis, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
     const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
     double* __restrict__ out) {
 #pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
@@ -355,7 +358,7 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     const int tid = threadIdx.x;
     const int fi = F[0] + tid % P2G_BI, fj = F[1] + (tid / P2G_BI) % P2G_BJ, fk = F[2] + tid / (P2G_BI * P2G_BJ);
     const bool valid = fi < si && fj < sj && fk < sk;
-    const bool tri = kern == 1;
+    constexpr bool tri = KERN == 1;
     const double dx = g.dx;
     const double r = tri ? dx : 1.5 * dx;
     const Wyvill W = make_wyvill(r);
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     auto pair = [&](double qx, double qy, double qz, double v) __attribute__((always_inline)) {
         const double ex = (qx + sx) - cx, ey = (qy + sy) - cy, ez = (qz + sz) - cz;
         double cw;
-        if (tri) {
+        if constexpr (tri) {
             const double ax = fabs(ex) / dx, ay = fabs(ey) / dx, az = fabs(ez) / dx;
             cw = (ax < 1.0 && ay < 1.0 && az < 1.0) ? (1.0 - ax) * (1.0 - ay) * (1.0 - az) : 0.0;
         } else {
@@ -474,8 +477,8 @@ void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f
     const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
     const double* vc = axis == 0 ? P.vx : axis == 1 ? P.vy : P.vz;
     const int nbi = (si + P2G_BI - 1) / P2G_BI, nbj = (sj + P2G_BJ - 1) / P2G_BJ, nbk = (sk + P2G_BK - 1) / P2G_BK;
-    k_p2g_brick<<<nbi * nbj * nbk, P2G_BI * P2G_BJ * P2G_BK, 0, st>>>(g, axis, kernel, nbi, nbj, P.px, P.py, P.pz,
-                                                                       vc, ci.start, f);
+    auto k = kernel == 1 ? k_p2g_brick<1> : k_p2g_brick<0>;
+    k<<<nbi * nbj * nbk, P2G_BI * P2G_BJ * P2G_BK, 0, st>>>(g, axis, nbi, nbj, P.px, P.py, P.pz, vc, ci.start, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -556,11 +559,190 @@ __global__ void k_density(Geo g, const double* __restrict__ px, const double* __
     (void)vk;
 }
 
+// The same density gather over bricks of 8x8x4 coarse cells (one thread per
+// coarse cell, as k_density): the workgroup first reads the cell starts of
+// every k-slab its cells' candidate ranges touch, then walks those slabs, copying
+// each slab's particles (one contiguous run per cell row of the sorted arrays)
+// into LDS, from which every thread adds the rows inside its own range.  Each
+// particle is fetched from memory once per brick instead of once per thread.
+// Sums run in the same order as k_density (k, j, particle index).  A brick whose
+// candidate range exceeds the LDS row tables (blob radii of more than about 2
+// cells) runs k_density's loop; a slab with more particles than fit is read
+// from global memory in the same order.
+#define DB_BI 8
+#define DB_BJ 8
+#define DB_BK 4
+#define DB_CAP 1024  // staged particles: 24 KB
+#define DB_MAXC 12   // candidate cells per row (i) and rows per slab (j)
+#define DB_MAXS 8    // candidate slabs (k)
+template <int D>
+__global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
+    Geo g, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi, double R) {
+#pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    __shared__ d2 lxy[DB_CAP];
+    __shared__ double lz[DB_CAP];
+    __shared__ int raw[DB_MAXS][DB_MAXC][DB_MAXC + 1];
+    __shared__ int roff[DB_MAXS][DB_MAXC + 1];
+    const long b = xcd_block(blockIdx.x, gridDim.x);
+    const int bi = (int)(b % nbi), bj = (int)((b / nbi) % nbj), bk = (int)(b / ((long)nbi * nbj));
+    const int F[3] = {bi * DB_BI, bj * DB_BJ, bk * DB_BK}, BS[3] = {DB_BI, DB_BJ, DB_BK};
+    const int tid = threadIdx.x;
+    const int ci = F[0] + tid % DB_BI, cj = F[1] + (tid / DB_BI) % DB_BJ, ck = F[2] + tid / (DB_BI * DB_BJ);
+    const int nc[3] = {g.ni, g.nj, g.nk};
+    const bool valid = ci <= g.ni && cj <= g.nj && ck <= g.nk;
+    const int vi = D * g.ni + 1, vj = D * g.nj + 1;
+    const int ni_v = (ci == g.ni) ? 1 : D, nj_v = (cj == g.nj) ? 1 : D, nk_v = (ck == g.nk) ? 1 : D;
+    const double dx = g.dx, subdx = dx / (double)D;
+    const Wyvill W = make_wyvill(R);
+    double xs[D], ys[D], zs[D];
+    for (int a = 0; a < D; ++a) {
+        xs[a] = idx2pos(D * ci + a, subdx);
+        ys[a] = idx2pos(D * cj + a, subdx);
+        zs[a] = idx2pos(D * ck + a, subdx);
+    }
+    // candidate cells of coarse cell c along one axis: those overlapping
+    // [x_lo - R, x_hi + R] (k_density's rule); monotone in c
+    auto range = [&](int c, int n, int* lo, int* hi) {
+        const int nv = c == n ? 1 : D;
+        const double lo_pos = idx2pos(D * c, subdx), hi_pos = idx2pos(D * c + nv - 1, subdx);
+        *lo = (int)floor((lo_pos - R) / dx - 1e-7);
+        *hi = (int)floor((hi_pos + R) / dx + 1e-7);
+        if (*lo < 0) *lo = 0;
+        if (*hi > n - 1) *hi = n - 1;
+    };
+    int lo[3], hi[3], blo[3], bhi[3];
+    const int cc[3] = {ci, cj, ck};
+    for (int a = 0; a < 3; ++a) {
+        int t;
+        range(cc[a] <= nc[a] ? cc[a] : nc[a], nc[a], &lo[a], &hi[a]);
+        range(F[a], nc[a], &blo[a], &t);
+        const int last = F[a] + BS[a] - 1 < nc[a] ? F[a] + BS[a] - 1 : nc[a];
+        range(last, nc[a], &t, &bhi[a]);
+    }
+    double acc[D][D][D];
+    for (int a = 0; a < D; ++a) for (int c = 0; c < D; ++c) for (int e = 0; e < D; ++e) acc[a][c][e] = 0.0;
+    auto one = [&](double x, double y, double z) __attribute__((always_inline)) {
+        double ex2[D], ey2[D], ez2[D];
+        for (int a = 0; a < D; ++a) {
+            double e = xs[a] - x; ex2[a] = e * e;
+            e = ys[a] - y; ey2[a] = e * e;
+            e = zs[a] - z; ez2[a] = e * e;
+        }
+        for (int c = 0; c < D; ++c)
+            for (int bb = 0; bb < D; ++bb)
+                for (int a = 0; a < D; ++a) {
+                    double s = ex2[a] + ey2[bb] + ez2[c];
+                    if (s < W.r2) acc[c][bb][a] += wyvill(W, s);
+                }
+    };
+    // the particles of cells [i0, i1] in row (j, k), from global memory
+    auto row_global = [&](int j, int k, int i0, int i1) __attribute__((always_inline)) {
+        const long row = (long)g.ni * j + g.nn * k;
+        const int a0 = start[row + i0], b0 = start[row + i1 + 1];
+        int p = a0;
+        for (; p + 4 <= b0; p += 4) {
+            double X[4], Y[4], Z[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { X[u] = px[p + u]; Y[u] = py[p + u]; Z[u] = pz[p + u]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) one(X[u], Y[u], Z[u]);
+        }
+        for (; p < b0; ++p) one(px[p], py[p], pz[p]);
+    };
+    const int ncell = bhi[0] - blo[0] + 1, nrow = bhi[1] - blo[1] + 1, nsl = bhi[2] - blo[2] + 1;
+    if (ncell > DB_MAXC || nrow > DB_MAXC || nsl > DB_MAXS) {
+        // (uniform) the row tables cannot hold this brick's range
+        if (valid)
+            for (int k = lo[2]; k <= hi[2]; ++k)
+                for (int j = lo[1]; j <= hi[1]; ++j) row_global(j, k, lo[0], hi[0]);
+    } else {
+        const int per = nrow * (ncell + 1);
+        for (int q = tid; q < nsl * per; q += blockDim.x) {
+            const int ks = q / per, rem = q - ks * per, jr = rem / (ncell + 1), ic = rem - jr * (ncell + 1);
+            raw[ks][jr][ic] = start[(long)g.ni * (blo[1] + jr) + g.nn * (blo[2] + ks) + blo[0] + ic];
+        }
+        __syncthreads();
+        if (tid < nsl) {
+            int a = 0;
+            for (int jr = 0; jr < nrow; ++jr) {
+                roff[tid][jr] = a;
+                a += raw[tid][jr][ncell] - raw[tid][jr][0];
+            }
+            roff[tid][nrow] = a;
+        }
+        __syncthreads();
+        constexpr int NB = DB_BI * DB_BJ * DB_BK, UNR = 4;
+        for (int ks = 0; ks < nsl; ++ks) {
+            const int k = blo[2] + ks, m = roff[ks][nrow];
+            if (m == 0) continue;  // (uniform) no particles in the slab
+            const bool fits = m <= DB_CAP;
+            if (fits) {
+                // slab particle q: row jr with roff[jr] <= q < roff[jr + 1]
+                for (int q0 = tid; q0 < m; q0 += UNR * NB) {
+                    double X[UNR], Y[UNR], Z[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        const int q = q0 + u * NB;
+                        if (q < m) {
+                            int jr = 0;
+                            while (jr + 1 < nrow && roff[ks][jr + 1] <= q) ++jr;
+                            const int a = raw[ks][jr][0] + (q - roff[ks][jr]);
+                            X[u] = px[a]; Y[u] = py[a]; Z[u] = pz[a];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        const int q = q0 + u * NB;
+                        if (q < m) { lxy[q] = d2{X[u], Y[u]}; lz[q] = Z[u]; }
+                    }
+                }
+            }
+            __syncthreads();
+            if (valid && k >= lo[2] && k <= hi[2]) {
+                for (int j = lo[1]; j <= hi[1]; ++j) {
+                    if (!fits) { row_global(j, k, lo[0], hi[0]); continue; }
+                    const int jr = j - blo[1];
+                    const int base = roff[ks][jr] - raw[ks][jr][0];
+                    const int pa = base + raw[ks][jr][lo[0] - blo[0]], pb = base + raw[ks][jr][hi[0] + 1 - blo[0]];
+                    int p = pa;
+                    for (; p + 4 <= pb; p += 4) {
+                        d2 a[4];
+                        double z[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) { a[u] = lxy[p + u]; z[u] = lz[p + u]; }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) one(a[u].x, a[u].y, z[u]);
+                    }
+                    for (; p < pb; ++p) {
+                        const d2 a = lxy[p];
+                        one(a.x, a.y, lz[p]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (!valid) return;
+    for (int c = 0; c < nk_v; ++c)
+        for (int bb = 0; bb < nj_v; ++bb)
+            for (int a = 0; a < ni_v; ++a) {
+                long v = (D * ci + a) + (long)vi * ((D * cj + bb) + (long)vj * (D * ck + c));
+                phi[v] = acc[c][bb][a];
+            }
+}
+
 void launch_density(const Geo& g, const Particles& P, const CellIndex& ci, double* phi, int d, double radius,
                     hipStream_t st) {
     const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
-    if (d == 2) k_density<2><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
-    else if (d == 1) k_density<1><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
+    if (d == 2) {
+        const int nbi = (g.ni + DB_BI) / DB_BI, nbj = (g.nj + DB_BJ) / DB_BJ, nbk = (g.nk + DB_BK) / DB_BK;
+        k_density_brick<2><<<nbi * nbj * nbk, DB_BI * DB_BJ * DB_BK, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, ci.start,
+                                                                           phi, radius);
+        return;
+    }
+    if (d == 1) k_density<1><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
     else if (d == 3) k_density<3><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
     else k_density<4><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
 }
