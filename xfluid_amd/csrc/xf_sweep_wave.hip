@@ -319,8 +319,14 @@ __device__ inline int w8_kword(int jl) {
     constexpr int CJ = FWD ? 7 : 0;
     return jl == CJ ? CJ : 8 + jl;
 }
-// wait until ring slot s is free (step s - R consumed); cons = last count seen
-template <int R>
+// wait until ring slot s is free (step s - R consumed); cons = last count seen.
+// POLL: also give up once another wave has set the error word.  The record
+// wave does not poll it: a load with a register result in its loop, even on a
+// path it rarely takes, lets the compiler put a full vmcnt wait before its
+// publishing store, which would drain the record requests in flight (the
+// compute wave it waits on keeps publishing after any failed wait of its own,
+// so the spin limit alone bounds this wait).
+template <int R, bool POLL = true>
 __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) {
     if (s < R) return;
     int spins = 0;
@@ -328,7 +334,8 @@ __device__ inline void w8_wait_slot(int s, int& cons, uint32_t consA, int* err) 
         cons = lds_ld32(consA);
         if (cons >= s - R + 1) break;
         __builtin_amdgcn_s_sleep(1);
-        if ((++spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) cons = s;
+        ++spins;
+        if (POLL && (spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) cons = s;
         else if (spins > WV_SPIN) { atomicOr(err, 4); cons = s; }
     }
 }
@@ -449,7 +456,7 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
             }
             const int s = x + LEAD;
             if (s < nS) {
-                w8_wait_slot<R>(s, cons_seen, ctrA + 4, err);
+                w8_wait_slot<R, false>(s, cons_seen, ctrA + 4, err);
                 const long e0 = (myTile * SL + slot_of(s)) * 64;
                 lds_u8* dst = ring + (s % R) * SLOT;
 #pragma unroll
@@ -480,10 +487,14 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
         // -------------------------------- edge wave --------------------------------
         // Lanes 0-15 each follow one upstream edge word (0-7: the j-upstream's
         // column word kl, 8-15: the k-upstream's row word for jl): the word of
-        // step x + 1 is loaded while step x is checked; a stale word (older tag)
-        // is re-read until it carries the tag of its step, then copied into the
-        // ring slot, where the compute wave checks the tag once more.  Lane 16
-        // writes the slot's dummy word.
+        // step x + 2 is loaded while step x is checked; a stale word (older tag)
+        // is re-read at once until it carries the tag of its step, then copied
+        // into the ring slot, where the compute wave checks the tag once more.
+        // Lane 16 writes the slot's dummy word.  (With the words of two steps in
+        // flight the wave takes three steps per memory round trip; one step ahead
+        // capped a tile that waits on its upstream at two steps per round trip,
+        // about 0.2 us a step; three ahead return stale more often: traced
+        // slower.)
         const bool mine = lane < 16, sideJ = lane < 8, dummy = lane == 16;
         const bool has = mine && (sideJ ? hasJ : hasK);
         const long ut = has ? (sideJ ? tJ : tK) : myTile;
@@ -494,11 +505,11 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
             return (const uint64_t*)((const unsigned char*)P.edge + (ut * ER + x + 7) * ROWB + w * EWB);
         };
         int cons_seen = 0;
-        // one step: load the words of step x + 1 into (n0, n1) while checking and
-        // placing those of step x (c0, c1); two register sets alternate, so no
-        // copy makes the compiler wait for the load just issued
+        // one step: load the words of step x + 2 into (n0, n1) while checking and
+        // placing those of step x (c0, c1); three register sets rotate, so no
+        // copy makes the compiler wait for a load just issued
         auto step = [&](int x, uint64_t& c0, uint64_t& c1, uint64_t& n0, uint64_t& n1) __attribute__((always_inline)) {
-            const int xn = x + 1 < nS ? x + 1 : x;
+            const int xn = x + 2 < nS ? x + 2 : nS - 1;
             n0 = __hip_atomic_load(src(xn), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (NS == 2) n1 = __hip_atomic_load(src(xn) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t want = tagbase + (uint32_t)(x + 7);
@@ -507,7 +518,6 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
                 while (!((uint32_t)(c0 >> 32) == want && (NS == 1 || (uint32_t)(c1 >> 32) == want))) {
                     if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                     if (++spins > WV_SPIN_EDGE) { atomicOr(err, 2); break; }
-                    __builtin_amdgcn_s_sleep(2);
                     c0 = __hip_atomic_load(src(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (NS == 2) c1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
@@ -520,13 +530,18 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
                 if (NS == 2) asm volatile("ds_write_b64 %0, %1 offset:8" ::"v"(ea), "v"(dummy ? d : c1) : "memory");
             }
         };
-        uint64_t a0 = __hip_atomic_load(src(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), a1 = 0, b0 = 0, b1 = 0;
+        uint64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0, c0 = 0, c1 = 0;
+        const int x1 = 1 < nS ? 1 : nS - 1;
+        a0 = __hip_atomic_load(src(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (NS == 2) a1 = __hip_atomic_load(src(0) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // (an even number of steps keeps the register state the same at the loop
-        // head; a step x >= nS only issues a harmless load)
-        for (int x = 0; x < nS; x += 2) {
-            step(x, a0, a1, b0, b1);
+        b0 = __hip_atomic_load(src(x1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (NS == 2) b1 = __hip_atomic_load(src(x1) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // (three steps per trip keep the register sets in place at the loop head;
+        // a step x >= nS only issues a harmless load of step nS - 1's word)
+        for (int x = 0; x < nS; x += 3) {
+            step(x, a0, a1, c0, c1);
             step(x + 1, b0, b1, a0, a1);
+            step(x + 2, c0, c1, b0, b1);
         }
         return;
     }
