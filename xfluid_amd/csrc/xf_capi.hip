@@ -314,10 +314,11 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
     if (sweep >= 2) {
-        int errw = 0;
-        HIPCHK(hipMemcpyAsync(&errw, s->pw.sweep_flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
+        // (a pinned destination: the copy joins the stream, and one wait covers the
+        // solve's last kernels and the copy)
+        HIPCHK(hipMemcpyAsync(s->pw.err_host, s->pw.sweep_flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
-        if (errw) {  // a pipelined wait timed out: redo the solve with per-hyperplane launches
+        if (*(volatile int*)s->pw.err_host) {  // a pipelined wait timed out: redo the solve with per-hyperplane launches
             fprintf(stderr, "xfluid: pipelined sweep wait timed out; solving with per-plane sweeps\n");
             HIPCHK(hipMemsetAsync(s->pw.sweep_flags, 0, sizeof(int), s->st));
             tm.used = 0;
@@ -491,7 +492,8 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
     if (hipHostMalloc((void**)&s->pw.S_host, 3 * sizeof(PcgScalars)) != hipSuccess ||
         hipEventCreateWithFlags(&s->pw.chunk_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s->pw.chunk_ev[1], hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void**)&s->h_red, RED_N * sizeof(double)) != hipSuccess)
+        hipHostMalloc((void**)&s->h_red, RED_N * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void**)&s->pw.err_host, sizeof(int)) != hipSuccess)
         return fail(set_err(-2, "pinned alloc"));
     std::memset(s->pw.S_host, 0, 3 * sizeof(PcgScalars));
     hipMemsetAsync(s->u, 0, s->nu() * 8, s->st);
@@ -538,6 +540,7 @@ void xf_sim_destroy(xf_sim* s) {
         delete[] s->pw.timer.ev;
     }
     if (s->pw.S_host) hipHostFree(s->pw.S_host);
+    if (s->pw.err_host) hipHostFree(s->pw.err_host);
     for (hipEvent_t e : s->pw.chunk_ev)
         if (e) hipEventDestroy(e);
     if (s->h_red) hipHostFree(s->h_red);

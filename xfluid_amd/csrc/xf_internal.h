@@ -83,6 +83,7 @@ struct PcgWork {
     PcgScalars* S_host;  // pinned mirror: [0] inner solve, [1] outer, [2] inner (odd chunks)
     hipEvent_t chunk_ev[2];  // ends of the PCG's iteration chunks in flight
     int* sweep_flags;    // error word + progress counters of the pipelined sweep
+    int* err_host;       // pinned copy of the error word, read after a solve
     int nflags;
     SweepSkew<double> sk64;  // pipelined sweep buffers (FP64 solve)
     SweepSkew<float> sk32;   // (FP32 inner solve)

@@ -617,8 +617,12 @@ void launch_finish_res(const FinRes& f, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // One PCG solve (pcg_solver.h:243-285) of A x = rhs with T-precision vectors.
 // x must be zeroed by the caller; rhs is consumed as the initial residual r.
+// Returns the solver state after the converged (or last) iteration.  A
+// converged solve returns as soon as the host has seen its status: the
+// iterations enqueued after it, which return at once, may still be in flight
+// on the stream, ahead of whatever the caller enqueues next.
 template <class T>
-static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* s2, T* z, T* q,
+static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* s2, T* z, T* q,
                      double* partial, int half, PcgScalars* S, PcgScalars* S_host, hipEvent_t* chunk_ev,
                      double tol_factor,
                      int max_iter, int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
@@ -688,22 +692,23 @@ static void pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* 
         if (hs.done || chunk * CH + CH >= max_iter) break;
         ++chunk;
     }
-    hipStreamSynchronize(st);
     if (!hs.done) {
+        // (the stream is idle after this wait: no kernel writes the slots then)
+        hipStreamSynchronize(st);
         hipMemcpyAsync(S_host, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
         hs = *S_host;
         if (!hs.done) { hs.status = 1; hs.iters = max_iter; }
     }
-    *S_host = hs;  // (the stream is idle: no kernel writes the slots now)
     if (tm && tm->used > tm_start) {
         // keep the timings of the applications that ran: the initial one and one per
         // iteration before the converged one (later launches returned at once);
         // applications 0, stride, 2 stride, ... were timed
-        const int apps = S_host->res0 == 0 ? 0 : S_host->status == 1 ? 1 + max_iter : S_host->iters;
+        const int apps = hs.res0 == 0 ? 0 : hs.status == 1 ? 1 + max_iter : hs.iters;
         const int timed = apps > 0 ? (apps - 1) / SWEEP_TIMER_STRIDE + 1 : 0;
         if (tm->used > tm_start + 2 * timed) tm->used = tm_start + 2 * timed;
     }
+    return hs;
 }
 
 PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,
@@ -725,14 +730,14 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (precision == 0) {
         k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
         k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
-        pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.npartial, w.S, w.S_host, w.chunk_ev, tol_factor,
+        const PcgScalars hs = pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.npartial, w.S, w.S_host, w.chunk_ev, tol_factor,
                          max_iter,
                          sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, &w.timer, st);
-        res.status = w.S_host->status;
-        res.iterations = w.S_host->iters;
+        res.status = hs.status;
+        res.iterations = hs.iters;
         res.outer = 1;
-        res.residual = w.S_host->res;
-        res.rhs_norm = w.S_host->res0;
+        res.residual = hs.res;
+        res.rhs_norm = hs.res0;
         return res;
     }
     // Mixed precision: FP32 inner PCG (SpMV, sweeps, axpy in FP32; dots in
@@ -762,10 +767,10 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
         k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
         k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
-        pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.npartial, w.S, w.S_host,
+        const PcgScalars hs = pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.npartial, w.S, w.S_host,
                         w.chunk_ev, itol,
                         max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
-        total += w.S_host->iters;
+        total += hs.iters;
         ++outer;
         k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
         k_residual<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.b, w.x, w.r, w.partial);
@@ -773,7 +778,7 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         hipStreamSynchronize(st);
         res.residual = S_outer_h->res;
         if (S_outer_h->res <= tol) { res.status = 0; break; }
-        if (w.S_host->status == 2) { res.status = 2; break; }
+        if (hs.status == 2) { res.status = 2; break; }
     }
     res.iterations = total;
     res.outer = outer;
