@@ -514,12 +514,30 @@ __device__ __attribute__((always_inline)) inline void w8_sweep(const W8Geo& G, i
             if (NS == 2) n1 = __hip_atomic_load(src(xn) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t want = tagbase + (uint32_t)(x + 7);
             if (has && x < nS) {
-                int spins = 0;
-                while (!((uint32_t)(c0 >> 32) == want && (NS == 1 || (uint32_t)(c1 >> 32) == want))) {
-                    if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                    if (++spins > WV_SPIN_EDGE) { atomicOr(err, 2); break; }
+                auto fresh = [&](uint64_t v0, uint64_t v1) __attribute__((always_inline)) {
+                    return (uint32_t)(v0 >> 32) == want && (NS == 1 || (uint32_t)(v1 >> 32) == want);
+                };
+                if (!fresh(c0, c1)) {
+                    // stale: two re-reads in flight, the second issued about half a
+                    // round trip after the first, each re-issued as soon as it returns
+                    // stale, so a word is seen within about half a trip of landing
+                    int spins = 0;
+                    uint64_t e0 = 0, e1 = 0;
                     c0 = __hip_atomic_load(src(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (NS == 2) c1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __builtin_amdgcn_s_sleep(4);
+                    e0 = __hip_atomic_load(src(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (NS == 2) e1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (;;) {
+                        if (fresh(c0, c1)) break;
+                        if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                        if (++spins > WV_SPIN_EDGE) { atomicOr(err, 2); break; }
+                        c0 = __hip_atomic_load(src(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (NS == 2) c1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (fresh(e0, e1)) { c0 = e0; c1 = e1; break; }
+                        e0 = __hip_atomic_load(src(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (NS == 2) e1 = __hip_atomic_load(src(x) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 }
             }
             w8_wait_slot<R>(x, cons_seen, ctrA + 4, err);
