@@ -339,11 +339,11 @@ void launch_source(const Geo& g, Particles& P, uint8_t* mat, const int32_t* cell
 #endif
 #define P2G_BI 8
 #define P2G_MAXC 12  // cells along i in a slab row (8 faces + support)
-// (KERN: 0 Wyvill, 1 trilinear; one kernel each, so the pair loop carries only
-// its own weight's registers)
-template <int KERN>
+// (KERN: 0 Wyvill, 1 trilinear; AX: the face axis; one kernel each, so the pair
+// loop carries only its own weight's registers and shifts only the axis component)
+template <int KERN, int AX>
 __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
-    Geo g, int axis, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
+    Geo g, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
     const double* __restrict__ pz, const double* __restrict__ vc, const int32_t* __restrict__ start,
     double* __restrict__ out) {
 #pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
@@ -351,6 +351,7 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     __shared__ d2 lp[2 * P2G_CAP];  // slab particle q: {x, y} at 2q, {z, v} at 2q + 1
     __shared__ int raw[P2G_BK + 3][P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slabs' cells, per row
     __shared__ int roff[P2G_BK + 3][P2G_BJ + 4];               // LDS offset of each slab row, then the slab's total
+    constexpr int axis = AX;
     const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const int bi = (int)(b % nbi), bj = (int)((b / nbi) % nbj), bk = (int)(b / ((long)nbi * nbj));
@@ -376,10 +377,12 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
         if (chi[a] > n[a] - 1) chi[a] = n[a] - 1;
     }
     const double cx = idx2pos(fi, dx), cy = idx2pos(fj, dx), cz = idx2pos(fk, dx);
-    const double sx = axis == 0 ? 0.5 * dx : 0.0, sy = axis == 1 ? 0.5 * dx : 0.0, sz = axis == 2 ? 0.5 * dx : 0.0;
+    const double hdx = 0.5 * dx;
     double wsum = 0.0, vsum = 0.0;
     auto pair = [&](double qx, double qy, double qz, double v) __attribute__((always_inline)) {
-        const double ex = (qx + sx) - cx, ey = (qy + sy) - cy, ez = (qz + sz) - cz;
+        // q = p + dx/2 e_axis (the other components' + 0 changes no value)
+        const double ex = (AX == 0 ? qx + hdx : qx) - cx, ey = (AX == 1 ? qy + hdx : qy) - cy,
+                     ez = (AX == 2 ? qz + hdx : qz) - cz;
         double cw;
         if constexpr (tri) {
             const double ax = fabs(ex) / dx, ay = fabs(ey) / dx, az = fabs(ez) / dx;
@@ -477,8 +480,11 @@ void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f
     const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
     const double* vc = axis == 0 ? P.vx : axis == 1 ? P.vy : P.vz;
     const int nbi = (si + P2G_BI - 1) / P2G_BI, nbj = (sj + P2G_BJ - 1) / P2G_BJ, nbk = (sk + P2G_BK - 1) / P2G_BK;
-    auto k = kernel == 1 ? k_p2g_brick<1> : k_p2g_brick<0>;
-    k<<<nbi * nbj * nbk, P2G_BI * P2G_BJ * P2G_BK, 0, st>>>(g, axis, nbi, nbj, P.px, P.py, P.pz, vc, ci.start, f);
+    void (*const ks[2][3])(Geo, int, int, const double*, const double*, const double*, const double*, const int32_t*,
+                           double*) = {{k_p2g_brick<0, 0>, k_p2g_brick<0, 1>, k_p2g_brick<0, 2>},
+                                       {k_p2g_brick<1, 0>, k_p2g_brick<1, 1>, k_p2g_brick<1, 2>}};
+    ks[kernel == 1][axis]<<<nbi * nbj * nbk, P2G_BI * P2G_BJ * P2G_BK, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, vc,
+                                                                              ci.start, f);
 }
 
 // ---------------------------------------------------------------------------
