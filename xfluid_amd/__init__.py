@@ -144,6 +144,16 @@ def _chk(rc, what):
     return rc
 
 
+def _torch_done(tensors):
+    """Waits for torch's current stream on the tensors' device.  The library
+    works on a non-blocking stream of its own, which does not wait for torch's
+    streams: a tensor that torch is still filling (or has not yet zeroed)
+    must be complete before the library reads or writes it.  Every entry point
+    returns with the library's own stream drained, so nothing is needed after."""
+    import torch
+    torch.cuda.current_stream(tensors[0].device).synchronize()
+
+
 def default_params(**kw):
     p = XfParams()
     lib().xf_params_default(C.byref(p))
@@ -305,6 +315,7 @@ class FluidSimulator:
             for a in arrs:
                 if not a.is_contiguous() or str(a.dtype) != "torch.float64":
                     raise ValueError(f"{what}: device arrays must be contiguous float64")
+            _torch_done(arrs)
             return _chk(fn(self.h, arrs[0].data_ptr(), arrs[1].data_ptr(), arrs[2].data_ptr(), 1), what)
         for a in arrs:
             if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
@@ -324,6 +335,7 @@ class FluidSimulator:
     def _dev(t, what, dtype="torch.float64"):
         if not getattr(t, "is_cuda", False) or not t.is_contiguous() or str(t.dtype) != dtype:
             raise ValueError(f"{what}: expected a contiguous {dtype} device tensor")
+        _torch_done((t,))
         return t.data_ptr()
 
     def set_particles_device(self, pos, vel):
@@ -358,6 +370,7 @@ class FluidSimulator:
             for a in arrays:
                 if not a.is_contiguous() or str(a.dtype) != "torch.float64":
                     raise ValueError(f"{what}: device arrays must be contiguous float64")
+            _torch_done(arrays)
             return [a.data_ptr() for a in arrays], 1
         for a in arrays:
             if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
