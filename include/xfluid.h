@@ -119,7 +119,8 @@ typedef struct xf_solve_stats {
                                  redone with per-hyperplane sweeps                  */
     double residual;          /* final inf-norm residual (FP64)                    */
     double rhs_norm;          /* inf-norm of the right-hand side                   */
-    double ms;                /* wall time of the solve on the device, ms          */
+    double ms;                /* timing on: wall time of the solve on the device,
+                                 ms (events around it); 0 with timing off          */
     double sweep_ms;          /* timing on, pipelined sweeps: summed device time of
                                  the MIC(0) sweep launches of every 5th
                                  preconditioner application, ms                    */

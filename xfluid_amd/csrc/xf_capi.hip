@@ -292,10 +292,14 @@ int st_forces(xf_sim* s, double dt) {  // FluidSimulator.cpp:429-499
 // (fu, fv, fw), then PCG into pw.x.  With rhs_override the RHS is replaced by
 // rhs_override (device, N values) after the stencil codes are built.
 int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* fw, const double* rhs_override) {
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, s->st));
+    // (events only with timing on: an event waited for after the solve's own
+    // wait costs another round trip to the device)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, s->st));
+    }
     launch_stencil_rhs(s->g, s->mat, fu, fv, fw, s->code, s->pw.b, s->st, s->prm.pressure_solver);
     if (rhs_override)
         HIPCHK(hipMemcpyAsync(s->pw.b, rhs_override, s->g.N * 8, hipMemcpyDeviceToDevice, s->st));
@@ -327,12 +331,16 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
             s->last.fallback = 1;
         }
     }
-    HIPCHK(hipEventRecord(e1, s->st));
-    HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (s->timing) {
+        HIPCHK(hipEventRecord(e1, s->st));
+        HIPCHK(hipEventSynchronize(e1));
+        hipEventElapsedTime(&ms, e0, e1);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    } else if (sweep < 2) {
+        HIPCHK(hipStreamSynchronize(s->st));
+    }
     HIPCHK(hipGetLastError());
     s->last.status = r.status;
     s->last.iterations = r.iterations;

@@ -335,7 +335,7 @@ void launch_source(const Geo& g, Particles& P, uint8_t* mat, const int32_t* cell
 #define P2G_BK 4
 #endif
 #ifndef P2G_CAP
-#define P2G_CAP 1024  // 32 KB of staged particles: 4 workgroups (16 waves) per CU
+#define P2G_CAP 1024  // 34 KB of staged particles: 4 workgroups (16 waves) per CU
 #endif
 #define P2G_BI 8
 #define P2G_MAXC 12  // cells along i in a slab row (8 faces + support)
@@ -348,7 +348,11 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
     double* __restrict__ out) {
 #pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
     typedef double d2 __attribute__((ext_vector_type(2)));
-    __shared__ d2 lp[2 * P2G_CAP];  // slab particle q: {x, y} at 2q, {z, v} at 2q + 1
+    // slab particle q: {x, y} at 2q + q/8, {z, v} after it.  A cell's particles
+    // follow one another and most cells hold 8, so lanes one face apart read 8
+    // particles (256 bytes) apart: without the 16-byte step after every 8
+    // particles they would all read the same LDS banks.
+    __shared__ d2 lp[2 * P2G_CAP + P2G_CAP / 8];
     __shared__ int raw[P2G_BK + 3][P2G_BJ + 3][P2G_MAXC + 1];  // start[] of the slabs' cells, per row
     __shared__ int roff[P2G_BK + 3][P2G_BJ + 4];               // LDS offset of each slab row, then the slab's total
     constexpr int axis = AX;
@@ -437,8 +441,9 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
                 for (int u = 0; u < UNR; ++u) {
                     const int q = q0 + u * NB;
                     if (q < m) {
-                        lp[2 * q] = d2{vx[u], vy[u]};
-                        lp[2 * q + 1] = d2{vz[u], vv[u]};
+                        const int e = 2 * q + (q >> 3);
+                        lp[e] = d2{vx[u], vy[u]};
+                        lp[e + 1] = d2{vz[u], vv[u]};
                     }
                 }
             }
@@ -455,12 +460,17 @@ __global__ __launch_bounds__(P2G_BI * P2G_BJ * P2G_BK) void k_p2g_brick(
                     for (; p + 4 <= pb; p += 4) {
                         d2 a[4], c[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) { a[u] = lp[2 * (p + u)]; c[u] = lp[2 * (p + u) + 1]; }
+                        for (int u = 0; u < 4; ++u) {
+                            const int e = 2 * (p + u) + ((p + u) >> 3);
+                            a[u] = lp[e];
+                            c[u] = lp[e + 1];
+                        }
 #pragma unroll
                         for (int u = 0; u < 4; ++u) pair(a[u].x, a[u].y, c[u].x, c[u].y);
                     }
                     for (; p < pb; ++p) {
-                        const d2 a = lp[2 * p], c = lp[2 * p + 1];
+                        const int e = 2 * p + (p >> 3);
+                        const d2 a = lp[e], c = lp[e + 1];
                         pair(a.x, a.y, c.x, c.y);
                     }
                 } else {
@@ -578,7 +588,7 @@ __global__ void k_density(Geo g, const double* __restrict__ px, const double* __
 #define DB_BI 8
 #define DB_BJ 8
 #define DB_BK 4
-#define DB_CAP 1024  // staged particles: 24 KB
+#define DB_CAP 1024  // staged particles: 27 KB
 #define DB_MAXC 12   // candidate cells per row (i) and rows per slab (j)
 #define DB_MAXS 8    // candidate slabs (k)
 template <int D>
@@ -587,8 +597,11 @@ __global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
     const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi, double R) {
 #pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
     typedef double d2 __attribute__((ext_vector_type(2)));
-    __shared__ d2 lxy[DB_CAP];
-    __shared__ double lz[DB_CAP];
+    // slab particle q at q + q/8 in both arrays: lanes one cell apart read about 8
+    // particles apart, which without the step after every 8 particles would land
+    // on the same LDS banks
+    __shared__ d2 lxy[DB_CAP + DB_CAP / 8];
+    __shared__ double lz[DB_CAP + DB_CAP / 8];
     __shared__ int raw[DB_MAXS][DB_MAXC][DB_MAXC + 1];
     __shared__ int roff[DB_MAXS][DB_MAXC + 1];
     const long b = xcd_block(blockIdx.x, gridDim.x);
@@ -636,12 +649,18 @@ __global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
             e = ys[a] - y; ey2[a] = e * e;
             e = zs[a] - z; ez2[a] = e * e;
         }
+        // the kernel polynomial in Horner form, y + z shared by the row of
+        // vertices: nothing divides by these sums (unlike P2G's weight sums), so
+        // the form's rounding stays at ~1e-16 of the kernel's peak
         for (int c = 0; c < D; ++c)
-            for (int bb = 0; bb < D; ++bb)
+            for (int bb = 0; bb < D; ++bb) {
+                const double yz = ey2[bb] + ez2[c];
                 for (int a = 0; a < D; ++a) {
-                    double s = ex2[a] + ey2[bb] + ez2[c];
-                    if (s < W.r2) acc[c][bb][a] += wyvill(W, s);
+                    const double s = ex2[a] + yz;
+                    const double w = fma(fma(fma(-W.c0, s, W.c1), s, -W.c2), s, 1.0);
+                    acc[c][bb][a] += s < W.r2 ? w : 0.0;
                 }
+            }
     };
     // the particles of cells [i0, i1] in row (j, k), from global memory
     auto row_global = [&](int j, int k, int i0, int i1) __attribute__((always_inline)) {
@@ -701,7 +720,11 @@ __global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
 #pragma unroll
                     for (int u = 0; u < UNR; ++u) {
                         const int q = q0 + u * NB;
-                        if (q < m) { lxy[q] = d2{X[u], Y[u]}; lz[q] = Z[u]; }
+                        if (q < m) {
+                            const int e = q + (q >> 3);
+                            lxy[e] = d2{X[u], Y[u]};
+                            lz[e] = Z[u];
+                        }
                     }
                 }
             }
@@ -717,13 +740,18 @@ __global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
                         d2 a[4];
                         double z[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) { a[u] = lxy[p + u]; z[u] = lz[p + u]; }
+                        for (int u = 0; u < 4; ++u) {
+                            const int e = p + u + ((p + u) >> 3);
+                            a[u] = lxy[e];
+                            z[u] = lz[e];
+                        }
 #pragma unroll
                         for (int u = 0; u < 4; ++u) one(a[u].x, a[u].y, z[u]);
                     }
                     for (; p < pb; ++p) {
-                        const d2 a = lxy[p];
-                        one(a.x, a.y, lz[p]);
+                        const int e = p + (p >> 3);
+                        const d2 a = lxy[e];
+                        one(a.x, a.y, lz[e]);
                     }
                 }
             }
