@@ -501,20 +501,34 @@ void launch_p2g(const Geo& g, const Particles& P, const CellIndex& ci, double* f
 // Particle-blob density, DirectedDistanceField::updateFeild
 // (directeddistancefield.cpp:122-169, 300-328): phi(v) = sum_p W(|x_v - p|^2)
 // over the (d*n+1)^3 sub-vertex lattice, x_v = (v + 1/2) dx/d.  One thread
-// gathers the d^3 vertices owned by one coarse cell; the per-axis squared
-// offsets are computed once per particle and summed in the reference's
-// x,y,z order, so every pair weight equals the reference's.
+// gathers the d^3 vertices owned by one coarse cell, over the particles of the
+// cells within R of them (rows of the cell-sorted arrays, in k, j, particle
+// index order); the per-axis squared offsets are computed once per particle.
+// A workgroup is one wavefront of 8x8 coarse cells of one k layer, so
+// neighbouring lanes read overlapping runs of particles through the caches.
+// (Staging each k-slab's particles in LDS instead, for bricks of 8x8x4 cells
+// with a barrier per slab, left the wavefronts that do not need a slab idle at
+// its barriers: 740 against 464 us on the 128^3 dam break.)  Only the summation
+// order and the kernel polynomial's form (Horner, with fused multiply-adds)
+// differ from the reference, so the result equals it to rounding.
+#define DB_BI 8
+#define DB_BJ 8
+#define DB_BK 1
 template <int D>
-__global__ void k_density(Geo g, const double* __restrict__ px, const double* __restrict__ py,
-                          const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi,
-                          double R) {
+__global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density(Geo g, int nbi, int nbj,
+                                                                   const double* __restrict__ px,
+                                                                   const double* __restrict__ py,
+                                                                   const double* __restrict__ pz,
+                                                                   const int32_t* __restrict__ start,
+                                                                   double* __restrict__ phi, double R) {
 #pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
-    const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
-    long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= n1) return;
-    const int ci = (int)(t % (g.ni + 1)), cj = (int)((t / (g.ni + 1)) % (g.nj + 1)),
-              ck = (int)(t / ((long)(g.ni + 1) * (g.nj + 1)));
-    const int vi = D * g.ni + 1, vj = D * g.nj + 1, vk = D * g.nk + 1;
+    const long b = xcd_block(blockIdx.x, gridDim.x);
+    const int bi = (int)(b % nbi), bj = (int)((b / nbi) % nbj), bk = (int)(b / ((long)nbi * nbj));
+    const int tid = threadIdx.x;
+    const int ci = bi * DB_BI + tid % DB_BI, cj = bj * DB_BJ + (tid / DB_BI) % DB_BJ,
+              ck = bk * DB_BK + tid / (DB_BI * DB_BJ);
+    if (ci > g.ni || cj > g.nj || ck > g.nk) return;
+    const int vi = D * g.ni + 1, vj = D * g.nj + 1;
     const int ni_v = (ci == g.ni) ? 1 : D, nj_v = (cj == g.nj) ? 1 : D, nk_v = (ck == g.nk) ? 1 : D;
     const double dx = g.dx, subdx = dx / (double)D;
     const Wyvill W = make_wyvill(R);
@@ -535,111 +549,6 @@ __global__ void k_density(Geo g, const double* __restrict__ px, const double* __
     range(xs[0], xs[ni_v - 1], g.ni, &li, &hi_);
     range(ys[0], ys[nj_v - 1], g.nj, &lj, &hj);
     range(zs[0], zs[nk_v - 1], g.nk, &lk, &hk);
-    double acc[D][D][D];
-    for (int a = 0; a < D; ++a) for (int b = 0; b < D; ++b) for (int c = 0; c < D; ++c) acc[a][b][c] = 0.0;
-    for (int k = lk; k <= hk; ++k)
-        for (int j = lj; j <= hj; ++j) {
-            long row = (long)g.ni * j + g.nn * k;
-            int a0 = start[row + li], b0 = start[row + hi_ + 1];
-            auto one = [&](double x, double y, double z) __attribute__((always_inline)) {
-                double ex2[D], ey2[D], ez2[D];
-                for (int a = 0; a < D; ++a) {
-                    double e = xs[a] - x; ex2[a] = e * e;
-                    e = ys[a] - y; ey2[a] = e * e;
-                    e = zs[a] - z; ez2[a] = e * e;
-                }
-                for (int c = 0; c < D; ++c)
-                    for (int b = 0; b < D; ++b)
-                        for (int a = 0; a < D; ++a) {
-                            double s = ex2[a] + ey2[b] + ez2[c];
-                            if (s < W.r2) acc[c][b][a] += wyvill(W, s);
-                        }
-            };
-            // four particles' positions requested together, then added in index order
-            int p = a0;
-            for (; p + 4 <= b0; p += 4) {
-                double X[4], Y[4], Z[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) { X[u] = px[p + u]; Y[u] = py[p + u]; Z[u] = pz[p + u]; }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) one(X[u], Y[u], Z[u]);
-            }
-            for (; p < b0; ++p) one(px[p], py[p], pz[p]);
-        }
-    for (int c = 0; c < nk_v; ++c)
-        for (int b = 0; b < nj_v; ++b)
-            for (int a = 0; a < ni_v; ++a) {
-                long v = (D * ci + a) + (long)vi * ((D * cj + b) + (long)vj * (D * ck + c));
-                phi[v] = acc[c][b][a];
-            }
-    (void)vk;
-}
-
-// The same density gather over bricks of 8x8x4 coarse cells (one thread per
-// coarse cell, as k_density): the workgroup first reads the cell starts of
-// every k-slab its cells' candidate ranges touch, then walks those slabs, copying
-// each slab's particles (one contiguous run per cell row of the sorted arrays)
-// into LDS, from which every thread adds the rows inside its own range.  Each
-// particle is fetched from memory once per brick instead of once per thread.
-// Sums run in the same order as k_density (k, j, particle index).  A brick whose
-// candidate range exceeds the LDS row tables (blob radii of more than about 2
-// cells) runs k_density's loop; a slab with more particles than fit is read
-// from global memory in the same order.
-#define DB_BI 8
-#define DB_BJ 8
-#define DB_BK 4
-#define DB_CAP 1024  // staged particles: 27 KB
-#define DB_MAXC 12   // candidate cells per row (i) and rows per slab (j)
-#define DB_MAXS 8    // candidate slabs (k)
-template <int D>
-__global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
-    Geo g, int nbi, int nbj, const double* __restrict__ px, const double* __restrict__ py,
-    const double* __restrict__ pz, const int32_t* __restrict__ start, double* __restrict__ phi, double R) {
-#pragma clang fp contract(fast)  // a tolerance kernel: its sums run in another order than the reference's anyway
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    // slab particle q at q + q/8 in both arrays: lanes one cell apart read about 8
-    // particles apart, which without the step after every 8 particles would land
-    // on the same LDS banks
-    __shared__ d2 lxy[DB_CAP + DB_CAP / 8];
-    __shared__ double lz[DB_CAP + DB_CAP / 8];
-    __shared__ int raw[DB_MAXS][DB_MAXC][DB_MAXC + 1];
-    __shared__ int roff[DB_MAXS][DB_MAXC + 1];
-    const long b = xcd_block(blockIdx.x, gridDim.x);
-    const int bi = (int)(b % nbi), bj = (int)((b / nbi) % nbj), bk = (int)(b / ((long)nbi * nbj));
-    const int F[3] = {bi * DB_BI, bj * DB_BJ, bk * DB_BK}, BS[3] = {DB_BI, DB_BJ, DB_BK};
-    const int tid = threadIdx.x;
-    const int ci = F[0] + tid % DB_BI, cj = F[1] + (tid / DB_BI) % DB_BJ, ck = F[2] + tid / (DB_BI * DB_BJ);
-    const int nc[3] = {g.ni, g.nj, g.nk};
-    const bool valid = ci <= g.ni && cj <= g.nj && ck <= g.nk;
-    const int vi = D * g.ni + 1, vj = D * g.nj + 1;
-    const int ni_v = (ci == g.ni) ? 1 : D, nj_v = (cj == g.nj) ? 1 : D, nk_v = (ck == g.nk) ? 1 : D;
-    const double dx = g.dx, subdx = dx / (double)D;
-    const Wyvill W = make_wyvill(R);
-    double xs[D], ys[D], zs[D];
-    for (int a = 0; a < D; ++a) {
-        xs[a] = idx2pos(D * ci + a, subdx);
-        ys[a] = idx2pos(D * cj + a, subdx);
-        zs[a] = idx2pos(D * ck + a, subdx);
-    }
-    // candidate cells of coarse cell c along one axis: those overlapping
-    // [x_lo - R, x_hi + R] (k_density's rule); monotone in c
-    auto range = [&](int c, int n, int* lo, int* hi) {
-        const int nv = c == n ? 1 : D;
-        const double lo_pos = idx2pos(D * c, subdx), hi_pos = idx2pos(D * c + nv - 1, subdx);
-        *lo = (int)floor((lo_pos - R) / dx - 1e-7);
-        *hi = (int)floor((hi_pos + R) / dx + 1e-7);
-        if (*lo < 0) *lo = 0;
-        if (*hi > n - 1) *hi = n - 1;
-    };
-    int lo[3], hi[3], blo[3], bhi[3];
-    const int cc[3] = {ci, cj, ck};
-    for (int a = 0; a < 3; ++a) {
-        int t;
-        range(cc[a] <= nc[a] ? cc[a] : nc[a], nc[a], &lo[a], &hi[a]);
-        range(F[a], nc[a], &blo[a], &t);
-        const int last = F[a] + BS[a] - 1 < nc[a] ? F[a] + BS[a] - 1 : nc[a];
-        range(last, nc[a], &t, &bhi[a]);
-    }
     double acc[D][D][D];
     for (int a = 0; a < D; ++a) for (int c = 0; c < D; ++c) for (int e = 0; e < D; ++e) acc[a][c][e] = 0.0;
     auto one = [&](double x, double y, double z) __attribute__((always_inline)) {
@@ -662,123 +571,37 @@ __global__ __launch_bounds__(DB_BI * DB_BJ * DB_BK) void k_density_brick(
                 }
             }
     };
-    // the particles of cells [i0, i1] in row (j, k), from global memory
-    auto row_global = [&](int j, int k, int i0, int i1) __attribute__((always_inline)) {
-        const long row = (long)g.ni * j + g.nn * k;
-        const int a0 = start[row + i0], b0 = start[row + i1 + 1];
-        int p = a0;
-        for (; p + 4 <= b0; p += 4) {
-            double X[4], Y[4], Z[4];
+    for (int k = lk; k <= hk; ++k)
+        for (int j = lj; j <= hj; ++j) {
+            const long row = (long)g.ni * j + g.nn * k;
+            const int a0 = start[row + li], b0 = start[row + hi_ + 1];
+            // four particles' positions requested together, then added in index order
+            int p = a0;
+            for (; p + 4 <= b0; p += 4) {
+                double X[4], Y[4], Z[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { X[u] = px[p + u]; Y[u] = py[p + u]; Z[u] = pz[p + u]; }
+                for (int u = 0; u < 4; ++u) { X[u] = px[p + u]; Y[u] = py[p + u]; Z[u] = pz[p + u]; }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) one(X[u], Y[u], Z[u]);
-        }
-        for (; p < b0; ++p) one(px[p], py[p], pz[p]);
-    };
-    const int ncell = bhi[0] - blo[0] + 1, nrow = bhi[1] - blo[1] + 1, nsl = bhi[2] - blo[2] + 1;
-    if (ncell > DB_MAXC || nrow > DB_MAXC || nsl > DB_MAXS) {
-        // (uniform) the row tables cannot hold this brick's range
-        if (valid)
-            for (int k = lo[2]; k <= hi[2]; ++k)
-                for (int j = lo[1]; j <= hi[1]; ++j) row_global(j, k, lo[0], hi[0]);
-    } else {
-        const int per = nrow * (ncell + 1);
-        for (int q = tid; q < nsl * per; q += blockDim.x) {
-            const int ks = q / per, rem = q - ks * per, jr = rem / (ncell + 1), ic = rem - jr * (ncell + 1);
-            raw[ks][jr][ic] = start[(long)g.ni * (blo[1] + jr) + g.nn * (blo[2] + ks) + blo[0] + ic];
-        }
-        __syncthreads();
-        if (tid < nsl) {
-            int a = 0;
-            for (int jr = 0; jr < nrow; ++jr) {
-                roff[tid][jr] = a;
-                a += raw[tid][jr][ncell] - raw[tid][jr][0];
+                for (int u = 0; u < 4; ++u) one(X[u], Y[u], Z[u]);
             }
-            roff[tid][nrow] = a;
+            for (; p < b0; ++p) one(px[p], py[p], pz[p]);
         }
-        __syncthreads();
-        constexpr int NB = DB_BI * DB_BJ * DB_BK, UNR = 4;
-        for (int ks = 0; ks < nsl; ++ks) {
-            const int k = blo[2] + ks, m = roff[ks][nrow];
-            if (m == 0) continue;  // (uniform) no particles in the slab
-            const bool fits = m <= DB_CAP;
-            if (fits) {
-                // slab particle q: row jr with roff[jr] <= q < roff[jr + 1]
-                for (int q0 = tid; q0 < m; q0 += UNR * NB) {
-                    double X[UNR], Y[UNR], Z[UNR];
-#pragma unroll
-                    for (int u = 0; u < UNR; ++u) {
-                        const int q = q0 + u * NB;
-                        if (q < m) {
-                            int jr = 0;
-                            while (jr + 1 < nrow && roff[ks][jr + 1] <= q) ++jr;
-                            const int a = raw[ks][jr][0] + (q - roff[ks][jr]);
-                            X[u] = px[a]; Y[u] = py[a]; Z[u] = pz[a];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < UNR; ++u) {
-                        const int q = q0 + u * NB;
-                        if (q < m) {
-                            const int e = q + (q >> 3);
-                            lxy[e] = d2{X[u], Y[u]};
-                            lz[e] = Z[u];
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            if (valid && k >= lo[2] && k <= hi[2]) {
-                for (int j = lo[1]; j <= hi[1]; ++j) {
-                    if (!fits) { row_global(j, k, lo[0], hi[0]); continue; }
-                    const int jr = j - blo[1];
-                    const int base = roff[ks][jr] - raw[ks][jr][0];
-                    const int pa = base + raw[ks][jr][lo[0] - blo[0]], pb = base + raw[ks][jr][hi[0] + 1 - blo[0]];
-                    int p = pa;
-                    for (; p + 4 <= pb; p += 4) {
-                        d2 a[4];
-                        double z[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int e = p + u + ((p + u) >> 3);
-                            a[u] = lxy[e];
-                            z[u] = lz[e];
-                        }
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) one(a[u].x, a[u].y, z[u]);
-                    }
-                    for (; p < pb; ++p) {
-                        const int e = p + (p >> 3);
-                        const d2 a = lxy[e];
-                        one(a.x, a.y, lz[e]);
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (!valid) return;
     for (int c = 0; c < nk_v; ++c)
         for (int bb = 0; bb < nj_v; ++bb)
             for (int a = 0; a < ni_v; ++a) {
-                long v = (D * ci + a) + (long)vi * ((D * cj + bb) + (long)vj * (D * ck + c));
+                const long v = (D * ci + a) + (long)vi * ((D * cj + bb) + (long)vj * (D * ck + c));
                 phi[v] = acc[c][bb][a];
             }
 }
 
 void launch_density(const Geo& g, const Particles& P, const CellIndex& ci, double* phi, int d, double radius,
                     hipStream_t st) {
-    const long n1 = (long)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
-    if (d == 2) {
-        const int nbi = (g.ni + DB_BI) / DB_BI, nbj = (g.nj + DB_BJ) / DB_BJ, nbk = (g.nk + DB_BK) / DB_BK;
-        k_density_brick<2><<<nbi * nbj * nbk, DB_BI * DB_BJ * DB_BK, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, ci.start,
-                                                                           phi, radius);
-        return;
-    }
-    if (d == 1) k_density<1><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
-    else if (d == 3) k_density<3><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
-    else k_density<4><<<xf_blocks(n1), XF_BLOCK, 0, st>>>(g, P.px, P.py, P.pz, ci.start, phi, radius);
+    const int nbi = (g.ni + DB_BI) / DB_BI, nbj = (g.nj + DB_BJ) / DB_BJ, nbk = (g.nk + DB_BK) / DB_BK;
+    const int nb = nbi * nbj * nbk, nt = DB_BI * DB_BJ * DB_BK;
+    if (d == 1) k_density<1><<<nb, nt, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, ci.start, phi, radius);
+    else if (d == 2) k_density<2><<<nb, nt, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, ci.start, phi, radius);
+    else if (d == 3) k_density<3><<<nb, nt, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, ci.start, phi, radius);
+    else k_density<4><<<nb, nt, 0, st>>>(g, nbi, nbj, P.px, P.py, P.pz, ci.start, phi, radius);
 }
 
 // ---------------------------------------------------------------------------
