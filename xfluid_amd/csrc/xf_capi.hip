@@ -365,7 +365,7 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
 int st_solve(xf_sim* s) { return st_solve_from(s, s->u, s->v, s->w, nullptr); }
 int st_apply(xf_sim* s) {
     double* fa[3] = {s->u, s->v, s->w};
-    for (int a = 0; a < 3; ++a) launch_apply_pressure(s->g, s->mat, s->pw.x, fa[a], a, s->st, s->prm.pressure_solver);
+    launch_apply_pressure(s->g, s->mat, s->pw.x, fa[0], fa[1], fa[2], s->st, s->prm.pressure_solver);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -373,7 +373,7 @@ int st_extrapolate(xf_sim* s) {  // FluidSimulator.cpp:511-519
     int layers = (int)(s->prm.cfl + 5);
     if (layers > 120) return set_err(-1, "extrapolation layers %d exceed 120", layers);
     double* fa[3] = {s->u, s->v, s->w};
-    for (int a = 0; a < 3; ++a) launch_extrapolate(s->g, s->mat, fa[a], s->marker, a, layers, s->st);
+    launch_extrapolate(s->g, s->mat, fa[0], fa[1], fa[2], s->marker, s->maxface(), layers, s->st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -456,7 +456,7 @@ int xf_sim_create(int ni, int nj, int nk, xf_sim** out) {
               dalloc(&s->w, s->nw()) == hipSuccess && dalloc(&s->us, s->nu()) == hipSuccess &&
               dalloc(&s->vs, s->nv()) == hipSuccess && dalloc(&s->ws, s->nw()) == hipSuccess &&
               dalloc(&s->mat, N) == hipSuccess && dalloc(&s->code, N) == hipSuccess &&
-              dalloc(&s->marker, s->maxface()) == hipSuccess && dalloc(&s->ci.count, N + 1) == hipSuccess &&
+              dalloc(&s->marker, 3 * s->maxface()) == hipSuccess && dalloc(&s->ci.count, N + 1) == hipSuccess &&
               dalloc(&s->ci.start, N + 2) == hipSuccess && dalloc(&s->ci.cursor, N + 1) == hipSuccess &&
               dalloc(&s->pw.x, N) == hipSuccess && dalloc(&s->pw.r, N) == hipSuccess &&
               dalloc(&s->pw.s, N) == hipSuccess && dalloc(&s->pw.s2, N) == hipSuccess &&

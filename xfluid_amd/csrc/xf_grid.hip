@@ -163,12 +163,10 @@ void launch_stencil_rhs(const Geo& g, const uint8_t* mat, const double* u, const
 
 // ---------------------------------------------------------------------------
 // applyPressure (variational_pressuresolver.cpp:120-204), factor 1.0; cells
-// outside the grid count as solid; air ghost pressure = -p(fluid).
-__global__ void k_apply_pressure(Geo g, const uint8_t* mat, const double* p, double* f, int axis) {
-    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    long nf = (long)si * sj * sk;
-    if (t >= nf) return;
+// outside the grid count as solid; air ghost pressure = -p(fluid).  Face t of
+// the axis's face array.
+__device__ inline void apply_face(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, long t) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1);
     int i, j, k;
     unflat3(t, si, sj, &i, &j, &k);
     int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
@@ -191,11 +189,9 @@ __global__ void k_apply_pressure(Geo g, const uint8_t* mat, const double* p, dou
 // outside cells count as solid; an air side has pressure 0; a solid side takes
 // the fluid pressure plus (minus) the face velocity, so the update leaves the
 // face at u - ((p + u) - p).
-__global__ void k_apply_pressure_normal(Geo g, const uint8_t* mat, const double* p, double* f, int axis) {
-    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    long nf = (long)si * sj * sk;
-    if (t >= nf) return;
+__device__ inline void apply_face_normal(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis,
+                                         long t) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1);
     int i, j, k;
     unflat3(t, si, sj, &i, &j, &k);
     int di = axis == 0 ? -1 : 0, dj = axis == 1 ? -1 : 0, dk = axis == 2 ? -1 : 0;
@@ -215,13 +211,34 @@ __global__ void k_apply_pressure_normal(Geo g, const uint8_t* mat, const double*
     }
     f[t] = fv - 1.0 * (p2 - p1);
 }
-void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st,
-                           int op) {
-    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
-    if (op == OP_NORMAL)
-        k_apply_pressure_normal<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, p, f, axis);
-    else
-        k_apply_pressure<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, p, f, axis);
+__host__ __device__ inline long face_count(const Geo& g, int axis) {
+    return (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
+}
+// thread t of a launch over the three face arrays: its axis and its face in that array
+__device__ inline bool face3(const Geo& g, long t, int* axis, long* tf) {
+    const long n0 = face_count(g, 0), n1 = face_count(g, 1), n2 = face_count(g, 2);
+    if (t < n0) { *axis = 0; *tf = t; return true; }
+    t -= n0;
+    if (t < n1) { *axis = 1; *tf = t; return true; }
+    t -= n1;
+    *axis = 2;
+    *tf = t;
+    return t < n2;
+}
+template <int OP>
+__global__ void k_apply_pressure(Geo g, const uint8_t* mat, const double* p, double* fu, double* fv, double* fw) {
+    int axis;
+    long t;
+    if (!face3(g, blockIdx.x * (long)blockDim.x + threadIdx.x, &axis, &t)) return;
+    double* f = axis == 0 ? fu : axis == 1 ? fv : fw;
+    if (OP == OP_NORMAL) apply_face_normal(g, mat, p, f, axis, t);
+    else apply_face(g, mat, p, f, axis, t);
+}
+void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* fu, double* fv, double* fw,
+                           hipStream_t st, int op) {
+    const long n = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
+    if (op == OP_NORMAL) k_apply_pressure<OP_NORMAL><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, mat, p, fu, fv, fw);
+    else k_apply_pressure<OP_VARIATIONAL><<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, mat, p, fu, fv, fw);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,11 +249,8 @@ void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, do
 // reads only finalised layer L-1 values, so one gather kernel per layer gives
 // the same values.  In-place marking is safe: an unmarked face only looks for
 // neighbours marked L-1, and markers change from -1 to L within the launch.
-__global__ void k_extrap_init(Geo g, const uint8_t* mat, int8_t* mark, int axis) {
-    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    long nf = (long)si * sj * sk;
-    if (t >= nf) return;
+__device__ inline void extrap_init_face(const Geo& g, const uint8_t* mat, int8_t* mark, int axis, long t) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1);
     int i, j, k;
     unflat3(t, si, sj, &i, &j, &k);
     int oi = axis == 0, oj = axis == 1, ok = axis == 2;
@@ -244,12 +258,21 @@ __global__ void k_extrap_init(Geo g, const uint8_t* mat, int8_t* mark, int axis)
     bool b = mat_at(g, mat, i - oi, j - oj, k - ok, XF_AIR) == XF_FLUID;
     mark[t] = (a || b) ? 0 : -1;
 }
-
-__global__ void k_extrap_layer(Geo g, double* f, int8_t* mark, int axis, int L) {
-    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
+__global__ void k_extrap_init(Geo g, const uint8_t* mat, int8_t* mark, int axis) {
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    long nf = (long)si * sj * sk;
-    if (t >= nf) return;
+    if (t >= face_count(g, axis)) return;
+    extrap_init_face(g, mat, mark, axis, t);
+}
+// the three face arrays' markers at mark + axis * mstride
+__global__ void k_extrap_init3(Geo g, const uint8_t* mat, int8_t* mark, long mstride) {
+    int axis;
+    long t;
+    if (!face3(g, blockIdx.x * (long)blockDim.x + threadIdx.x, &axis, &t)) return;
+    extrap_init_face(g, mat, mark + axis * mstride, axis, t);
+}
+
+__device__ inline void extrap_layer_face(const Geo& g, double* f, int8_t* mark, int axis, int L, long t) {
+    const int si = g.ni + (axis == 0), sj = g.nj + (axis == 1), sk = g.nk + (axis == 2);
     if (mark[t] != -1) return;
     int i, j, k;
     unflat3(t, si, sj, &i, &j, &k);
@@ -268,6 +291,14 @@ __global__ void k_extrap_layer(Geo g, double* f, int8_t* mark, int axis, int L) 
         f[t] = sum / (double)count;
         mark[t] = (int8_t)L;
     }
+}
+// layer L of the three face arrays in one launch (each array only reads its own
+// faces and marker)
+__global__ void k_extrap_layer(Geo g, double* fu, double* fv, double* fw, int8_t* mark, long mstride, int L) {
+    int axis;
+    long t;
+    if (!face3(g, blockIdx.x * (long)blockDim.x + threadIdx.x, &axis, &t)) return;
+    extrap_layer_face(g, axis == 0 ? fu : axis == 1 ? fv : fw, mark + axis * mstride, axis, L, t);
 }
 
 // ---- adjoint of the extrapolation (fixed material) ----
@@ -328,9 +359,10 @@ void launch_extrapolate_adjoint(const Geo& g, const uint8_t* mat, double* gf, in
     k_extrap_drop<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(nf, gf, marker);
 }
 
-void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
-                        hipStream_t st) {
-    long nf = (long)(g.ni + (axis == 0)) * (g.nj + (axis == 1)) * (g.nk + (axis == 2));
-    k_extrap_init<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, mat, marker, axis);
-    for (int L = 1; L <= layers; ++L) k_extrap_layer<<<xf_blocks(nf), XF_BLOCK, 0, st>>>(g, f, marker, axis, L);
+void launch_extrapolate(const Geo& g, const uint8_t* mat, double* fu, double* fv, double* fw, int8_t* marker,
+                        long mstride, int layers, hipStream_t st) {
+    const long n = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
+    k_extrap_init3<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, mat, marker, mstride);
+    for (int L = 1; L <= layers; ++L)
+        k_extrap_layer<<<xf_blocks(n), XF_BLOCK, 0, st>>>(g, fu, fv, fw, marker, mstride, L);
 }

@@ -34,10 +34,13 @@ void launch_forces(const Geo& g, const uint8_t* mat, double* f, int axis, double
 // op: OP_VARIATIONAL or OP_NORMAL
 void launch_stencil_rhs(const Geo& g, const uint8_t* mat, const double* u, const double* v, const double* w,
                         uint16_t* code, double* b, hipStream_t st, int op = OP_VARIATIONAL);
-void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* f, int axis, hipStream_t st,
-                           int op = OP_VARIATIONAL);
-void launch_extrapolate(const Geo& g, const uint8_t* mat, double* f, int8_t* marker, int axis, int layers,
-                        hipStream_t st);
+// the three face arrays in one launch each (per face as the reference's
+// per-axis loops)
+void launch_apply_pressure(const Geo& g, const uint8_t* mat, const double* p, double* fu, double* fv, double* fw,
+                           hipStream_t st, int op);
+// marker: three arrays of mstride bytes (one per face array)
+void launch_extrapolate(const Geo& g, const uint8_t* mat, double* fu, double* fv, double* fw, int8_t* marker,
+                        long mstride, int layers, hipStream_t st);
 // VJP of launch_extrapolate for gradient gf on one face array (in place)
 void launch_extrapolate_adjoint(const Geo& g, const uint8_t* mat, double* gf, int8_t* marker, int axis, int layers,
                                 hipStream_t st);

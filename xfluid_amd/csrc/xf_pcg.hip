@@ -475,6 +475,25 @@ __global__ void k_dot(BoxRange bx, const T* __restrict__ a, const T* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+// Start of a PCG solve: r = (T) rsrc, x = 0, z = 0 and the partials of max |r|
+// (the copy, fills and reduction of separate kernels, in one pass).
+template <class T, class Src>
+__global__ void k_pcg_start(long n, T* __restrict__ r, const Src* __restrict__ rsrc, T* __restrict__ x,
+                            T* __restrict__ z, double* partial) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        const T v = (T)rsrc[c];
+        r[c] = v;
+        x[c] = (T)0;
+        z[c] = (T)0;
+        const double a = fabs((double)v);
+        m = a > m ? a : m;
+    }
+    m = block_reduce<double, true>(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
 template <class T>
 __global__ void k_absmax(long n, const T* __restrict__ a, double* partial) {
     __shared__ double sh[16];
@@ -616,13 +635,14 @@ void launch_finish_res(const FinRes& f, hipStream_t st) {
 
 // ---------------------------------------------------------------------------
 // One PCG solve (pcg_solver.h:243-285) of A x = rhs with T-precision vectors.
-// x must be zeroed by the caller; rhs is consumed as the initial residual r.
+// x starts at zero and r at rhs (rounded to T), both set here.
 // Returns the solver state after the converged (or last) iteration.  A
 // converged solve returns as soon as the host has seen its status: the
 // iterations enqueued after it, which return at once, may still be in flight
 // on the stream, ahead of whatever the caller enqueues next.
-template <class T>
-static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* x, T* r, T* s, T* s2, T* z, T* q,
+template <class T, class Src>
+static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, const Src* rhs, T* x, T* r, T* s, T* s2,
+                           T* z, T* q,
                      double* partial, int half, PcgScalars* S, PcgScalars* S_host, hipEvent_t* chunk_ev,
                      double tol_factor,
                      int max_iter, int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
@@ -642,8 +662,7 @@ static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, T* 
         if (sk) w8_dot_tiles<T>(g, sk->ax, sk->box, z, r, partial, S, st);
         else k_dot<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, z, r, partial, S);
     };
-    k_fill<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, z, (T)0);
-    k_absmax<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, partial);
+    k_pcg_start<T, Src><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, r, rhs, x, z, partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, partial, RED_BLOCKS, S, tol_factor, 0);
     if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial)) dot_zr();
     k_finish<<<1, 1024, 0, st>>>(FIN_RHO0, partial, ndot, S, tol_factor, 0);
@@ -728,9 +747,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         pcg_mic0(g, code, w.inv, tau, sigma, box, st);
     if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
     if (precision == 0) {
-        k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
-        k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
-        const PcgScalars hs = pcg_core<double>(g, code, w.inv, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial, w.npartial, w.S, w.S_host, w.chunk_ev, tol_factor,
+        const PcgScalars hs = pcg_core<double, double>(g, code, w.inv, w.b, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial,
+                                                       w.npartial, w.S, w.S_host, w.chunk_ev, tol_factor,
                          max_iter,
                          sweep_kernel, &w.sk64, w.sweep_flags, &w.epoch, &w.timer, st);
         res.status = hs.status;
@@ -747,9 +765,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     PcgScalars* S_outer_h = w.S_host + 1;
     k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.inv32, w.inv, nullptr);
     if (sweep_kernel == 9) sweep_prepare_w8<float>(g, code, w.inv32, w.sk32, st);
-    k_fill<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, 0.0);
-    k_copy<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, nullptr);
-    k_absmax<double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.partial);
+    // x = 0, r = b, max |r| (w.z, unused by the mixed solve, is cleared on the way)
+    k_pcg_start<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, w.x, w.z, w.partial);
     k_finish<<<1, 1024, 0, st>>>(FIN_RES0, w.partial, RED_BLOCKS, S_outer, tol_factor, 0, S_outer_h);
     hipStreamSynchronize(st);
     res.rhs_norm = S_outer_h->res0;
@@ -765,9 +782,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         // FP32 recurrence drifting from the true residual), whichever is less.
         const double need = 0.5 * tol / res.residual;
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
-        k_copy<float, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r32, w.r, nullptr);
-        k_fill<float><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x32, 0.0f);
-        const PcgScalars hs = pcg_core<float>(g, code, w.inv32, w.x32, w.r32, w.s32, w.s2_32, w.z32, w.q32, w.partial, w.npartial, w.S, w.S_host,
+        const PcgScalars hs = pcg_core<float, double>(g, code, w.inv32, w.r, w.x32, w.r32, w.s32, w.s2_32, w.z32,
+                                                      w.q32, w.partial, w.npartial, w.S, w.S_host,
                         w.chunk_ev, itol,
                         max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
         total += hs.iters;
