@@ -745,8 +745,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     res.box_cells = (long)(box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (box[5] - box[4] + 1);
     if (!(sweep_kernel == 9 && mic0_w8(g, code, w.inv, tau, sigma, w.sk64, w.sweep_flags, &w.epoch, st)))
         pcg_mic0(g, code, w.inv, tau, sigma, box, st);
-    if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
     if (precision == 0) {
+        if (sweep_kernel == 9) sweep_prepare_w8<double>(g, code, w.inv, w.sk64, st);
         const PcgScalars hs = pcg_core<double, double>(g, code, w.inv, w.b, w.x, w.r, w.s, w.s2, w.z, w.q, w.partial,
                                                        w.npartial, w.S, w.S_host, w.chunk_ev, tol_factor,
                          max_iter,
