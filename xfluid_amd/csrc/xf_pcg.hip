@@ -302,32 +302,48 @@ template <class T>
 __device__ inline T spmv_cell(const Geo& g, uint16_t cc, long c, const T* __restrict__ x) {
     return spmv_cell_g<T>(g, cc, c, [x](long e) __attribute__((always_inline)) { return x[e]; });
 }
+// The seven operand cells of row c in column order (-k, -j, -i, c, +i, +j, +k),
+// indices clamped into the grid: every operand is read unconditionally, so the
+// loads of several cells can be in flight together.
+__device__ inline void spmv_cols(const Geo& g, long c, long e[7]) {
+    e[0] = c >= g.nn ? c - g.nn : c;
+    e[1] = c >= g.ni ? c - g.ni : c;
+    e[2] = c >= 1 ? c - 1 : c;
+    e[3] = c;
+    e[4] = c + 1 < g.N ? c + 1 : c;
+    e[5] = c + g.ni < g.N ? c + g.ni : c;
+    e[6] = c + g.nn < g.N ? c + g.nn : c;
+}
+// row c of A times the operands x[0..6] (spmv_cols order): the present entries
+// added in column order
+template <class T>
+__device__ inline T spmv_row(uint16_t cc, const T x[7]) {
+    T y = 0;
+    int cl = code_dir(cc, D_ZM);
+    if (present(cl)) y += (T)code_coef(cl) * x[0];
+    cl = code_dir(cc, D_YM);
+    if (present(cl)) y += (T)code_coef(cl) * x[1];
+    cl = code_dir(cc, D_XM);
+    if (present(cl)) y += (T)code_coef(cl) * x[2];
+    double dg = code_diag(cc);
+    if (dg != 0.0) y += (T)dg * x[3];
+    cl = code_dir(cc, D_XP);
+    if (present(cl)) y += (T)code_coef(cl) * x[4];
+    cl = code_dir(cc, D_YP);
+    if (present(cl)) y += (T)code_coef(cl) * x[5];
+    cl = code_dir(cc, D_ZP);
+    if (present(cl)) y += (T)code_coef(cl) * x[6];
+    return y;
+}
 // the same with the operand x[e] given by get(e)
 template <class T, class Get>
 __device__ inline T spmv_cell_g(const Geo& g, uint16_t cc, long c, Get get) {
-    // the seven operands are read unconditionally (indices clamped into the
-    // grid), so the loads of several cells can be in flight together; the
-    // present entries are then added in column order
-    const T xkm = get(c >= g.nn ? c - g.nn : c), xjm = get(c >= g.ni ? c - g.ni : c), xim = get(c >= 1 ? c - 1 : c);
-    const T xc = get(c);
-    const T xip = get(c + 1 < g.N ? c + 1 : c), xjp = get(c + g.ni < g.N ? c + g.ni : c);
-    const T xkp = get(c + g.nn < g.N ? c + g.nn : c);
-    T y = 0;
-    int cl = code_dir(cc, D_ZM);
-    if (present(cl)) y += (T)code_coef(cl) * xkm;
-    cl = code_dir(cc, D_YM);
-    if (present(cl)) y += (T)code_coef(cl) * xjm;
-    cl = code_dir(cc, D_XM);
-    if (present(cl)) y += (T)code_coef(cl) * xim;
-    double dg = code_diag(cc);
-    if (dg != 0.0) y += (T)dg * xc;
-    cl = code_dir(cc, D_XP);
-    if (present(cl)) y += (T)code_coef(cl) * xip;
-    cl = code_dir(cc, D_YP);
-    if (present(cl)) y += (T)code_coef(cl) * xjp;
-    cl = code_dir(cc, D_ZP);
-    if (present(cl)) y += (T)code_coef(cl) * xkp;
-    return y;
+    long e[7];
+    spmv_cols(g, c, e);
+    T x[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) x[i] = get(e[i]);
+    return spmv_row<T>(cc, x);
 }
 
 template <class T>
@@ -427,6 +443,23 @@ __global__ void k_axpy_max(BoxRange bx, T* __restrict__ x, T* __restrict__ r, co
     __shared__ double sh[16];
     __shared__ double alpha_sh;
     if (S->done) return;
+    const unsigned stride = gridDim.x * blockDim.x;
+    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
+    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    BoxWalk w;
+    bw_init(bx, t, w);
+    // the operands of this thread's first PF cells are requested before the
+    // spmv's partials, so the two round trips to memory overlap (an absent
+    // cell reads cell 0 and is not written)
+    constexpr int PF = 2;
+    long c0[PF];
+    T x0[PF], r0[PF], s0[PF], z0[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        c0[u] = t + u * stride < bx.n ? bw_cell(bx, w) : 0;
+        bw_next(bx, w);
+        x0[u] = x[c0[u]]; r0[u] = r[c0[u]]; s0[u] = s[c0[u]]; z0[u] = z[c0[u]];
+    }
     const double sz = reduce_partials<false>(pin, npin, sh);
     if (threadIdx.x == 0) {
         alpha_sh = S->rho_next / sz;
@@ -437,12 +470,17 @@ __global__ void k_axpy_max(BoxRange bx, T* __restrict__ x, T* __restrict__ r, co
     const double alpha = alpha_sh;
     const T a = (T)alpha, na = (T)(-alpha);
     double m = 0.0;
-    const unsigned stride = gridDim.x * blockDim.x;
-    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
-    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    BoxWalk w;
-    bw_init(bx, t, w);
-    for (; t < bx.n; t += stride) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        if (t + u * stride < bx.n) {
+            x[c0[u]] = x0[u] + a * s0[u];
+            const T rv = r0[u] + na * z0[u];
+            r[c0[u]] = rv;
+            const double av = fabs((double)rv);
+            m = av > m ? av : m;
+        }
+    }
+    for (t += PF * stride; t < bx.n; t += stride) {
         const long c = bw_cell(bx, w);
         bw_next(bx, w);
         x[c] += a * s[c];
@@ -520,6 +558,29 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
     __shared__ double sh[16];
     __shared__ double beta_sh;
     if (S->done) return;
+    const unsigned stride = gridDim.x * blockDim.x;
+    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
+    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    BoxWalk w;
+    bw_init(bx, t, w);
+    // the codes and operands (z, s_old of the seven columns) of this thread's
+    // first PF cells are requested before the dot(z, r) partials, so the two
+    // round trips to memory overlap (an absent cell reads cell 0 and is not
+    // written)
+    constexpr int PF = 2;
+    long c0[PF];
+    uint16_t cc0[PF];
+    T z0[PF][7], o0[PF][7];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        c0[u] = t + u * stride < bx.n ? bw_cell(bx, w) : 0;
+        bw_next(bx, w);
+        cc0[u] = code[c0[u]];
+        long e[7];
+        spmv_cols(g, c0[u], e);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) { z0[u][i] = z[e[i]]; o0[u][i] = s_old[e[i]]; }
+    }
     const double v = reduce_partials<false>(pin, npin, sh);
     if (threadIdx.x == 0) {
         beta_sh = v / S->rho;
@@ -529,12 +590,19 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
     const T beta = (T)beta_sh;
     auto sv = [=](long e) __attribute__((always_inline)) { return z[e] + beta * s_old[e]; };
     double acc = 0.0;
-    const unsigned stride = gridDim.x * blockDim.x;
-    // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
-    unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    BoxWalk w;
-    bw_init(bx, t, w);
-    for (; t + 3 * stride < bx.n; t += 4 * stride) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        if (t + u * stride < bx.n) {
+            T xs[7];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) xs[i] = z0[u][i] + beta * o0[u][i];
+            const T yv = spmv_row<T>(cc0[u], xs);
+            s_new[c0[u]] = xs[3];
+            y[c0[u]] = yv;
+            acc += (double)xs[3] * (double)yv;
+        }
+    }
+    for (t += PF * stride; t + 3 * stride < bx.n; t += 4 * stride) {
         long cs[4];
         uint16_t cc[4];
         T yv[4], sc[4];
