@@ -190,6 +190,11 @@ int  xf_sim_last_solve(const xf_sim* s, xf_solve_stats* st);
 int  xf_sim_set_timing(xf_sim* s, int on);
 int  xf_sim_last_timing(const xf_sim* s, xf_step_timing* t);
 int  xf_sim_synchronize(xf_sim* s);
+/* Testing the sweep-timeout fallback (no counterpart in the reference): sets
+ * the pipelined sweeps' error word, as a timed-out wait would, so that every
+ * pipelined wait of the next solve gives up at once; that solve is then redone
+ * with per-hyperplane sweeps (xf_solve_stats.fallback = 1). */
+int  xf_sim_debug_fail_sweeps(xf_sim* s);
 
 /* ---- state transfer (the reference keeps these as members,
  *      FluidSimulator.h:179-197; here they live in HBM) -------------------- */

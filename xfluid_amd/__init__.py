@@ -103,6 +103,7 @@ def lib():
     L.xf_sim_set_timing.argtypes = [V, C.c_int]
     L.xf_sim_last_timing.argtypes = [V, C.POINTER(XfStepTiming)]
     L.xf_sim_synchronize.argtypes = [V]
+    L.xf_sim_debug_fail_sweeps.argtypes = [V]
     L.xf_sim_num_particles.restype = C.c_int64
     L.xf_sim_num_particles.argtypes = [V]
     L.xf_sim_set_particles.argtypes = [V, C.c_int64] + [_dp] * 6
@@ -256,6 +257,11 @@ class FluidSimulator:
 
     def synchronize(self):
         _chk(self.L.xf_sim_synchronize(self.h), "synchronize")
+
+    def debug_fail_sweeps(self):
+        """Test hook: the next solve's pipelined sweeps give up at once (as after a
+        timed-out wait) and the solve is redone with per-hyperplane sweeps."""
+        _chk(self.L.xf_sim_debug_fail_sweeps(self.h), "debug_fail_sweeps")
 
     def num_particles(self):
         return self.L.xf_sim_num_particles(self.h)

@@ -318,11 +318,15 @@ int st_solve_from(xf_sim* s, const double* fu, const double* fv, const double* f
     PcgResult r = pcg_solve(s->g, s->code, s->pw, p.pcg_tol, p.pcg_max_iter, p.mic_tau, p.mic_sigma, p.precision,
                             p.mixed_inner_tol, p.mixed_max_outer, sweep, s->st);
     if (sweep >= 2) {
-        // (a pinned destination: the copy joins the stream, and one wait covers the
-        // solve's last kernels and the copy)
-        HIPCHK(hipMemcpyAsync(s->pw.err_host, s->pw.sweep_flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
-        HIPCHK(hipStreamSynchronize(s->st));
-        if (*(volatile int*)s->pw.err_host) {  // a pipelined wait timed out: redo the solve with per-hyperplane launches
+        int err = r.sweep_err;
+        if (!r.err_known) {
+            // (a pinned destination: the copy joins the stream, and one wait covers the
+            // solve's last kernels and the copy)
+            HIPCHK(hipMemcpyAsync(s->pw.err_host, s->pw.sweep_flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+            err = *(volatile int*)s->pw.err_host;
+        }
+        if (err) {  // a pipelined wait timed out: redo the solve with per-hyperplane launches
             fprintf(stderr, "xfluid: pipelined sweep wait timed out; solving with per-plane sweeps\n");
             HIPCHK(hipMemsetAsync(s->pw.sweep_flags, 0, sizeof(int), s->st));
             tm.used = 0;
@@ -896,6 +900,13 @@ int xf_sim_last_timing(const xf_sim* s, xf_step_timing* t) {
 }
 int xf_sim_synchronize(xf_sim* s) {
     if (check(s)) return -1;
+    HIPCHK(hipStreamSynchronize(s->st));
+    return 0;
+}
+int xf_sim_debug_fail_sweeps(xf_sim* s) {
+    if (check(s)) return -1;
+    // (byte 0 of the error word: the value 1)
+    HIPCHK(hipMemsetAsync(s->pw.sweep_flags, 1, 1, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     return 0;
 }

@@ -99,6 +99,11 @@ struct PcgResult {
     int status, iterations, outer;
     double residual, rhs_norm;
     long box_cells;  // cells of the active rows' bounding box
+    // err_known: sweep_err holds the pipelined sweeps' error word as the solve
+    // ended (the mixed solve reads it with its last residual; otherwise the
+    // caller reads it)
+    int sweep_err;
+    bool err_known;
 };
 // Assemble the stencil codes and right-hand side, factor, and solve A x = b.
 PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_factor, int max_iter, double tau,

@@ -657,11 +657,13 @@ __global__ void k_add_f2d(long n, double* __restrict__ x, const float* __restric
 enum { FIN_RES0, FIN_RHO0, FIN_ALPHA, FIN_RES, FIN_BETA };
 // mirror (optional): pinned host copy of S, written at the end (the host reads
 // it after synchronising the stream, with no copy kernel in between)
+// eflag_out (optional, pinned): receives *eflag, the sweeps' error word, with the mirror
 __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, double tol_factor, int it,
-                         PcgScalars* mirror = nullptr) {
+                         PcgScalars* mirror = nullptr, const int* eflag = nullptr, int* eflag_out = nullptr) {
     __shared__ double sh[16];
     if (what != FIN_RES0 && S->done) {
         if (mirror && threadIdx.x == 0) *mirror = *S;
+        if (eflag_out && threadIdx.x == 0) *eflag_out = *eflag;
         return;
     }
     const bool mx = (what == FIN_RES0 || what == FIN_RES);
@@ -695,6 +697,7 @@ __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, 
             break;
     }
     if (mirror) *mirror = *S;
+    if (eflag_out) *eflag_out = *eflag;
 }
 
 void launch_finish_res(const FinRes& f, hipStream_t st) {
@@ -858,8 +861,12 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         ++outer;
         k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
         k_residual<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.b, w.x, w.r, w.partial);
-        k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0, S_outer_h);
+        // (with the solve's error word: every sweep of the solve ran before this)
+        k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0, S_outer_h, w.sweep_flags,
+                                     w.err_host);
         hipStreamSynchronize(st);
+        res.sweep_err = *(volatile int*)w.err_host;
+        res.err_known = true;
         res.residual = S_outer_h->res;
         if (S_outer_h->res <= tol) { res.status = 0; break; }
         if (hs.status == 2) { res.status = 2; break; }
