@@ -838,20 +838,18 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
     if (sweep_kernel == 9) sweep_prepare_w8<float>(g, code, w.inv32, w.sk32, st);
     // x = 0, r = b, max |r| (w.z, unused by the mixed solve, is cleared on the way)
     k_pcg_start<double, double><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.r, w.b, w.x, w.z, w.partial);
-    k_finish<<<1, 1024, 0, st>>>(FIN_RES0, w.partial, RED_BLOCKS, S_outer, tol_factor, 0, S_outer_h);
-    hipStreamSynchronize(st);
-    res.rhs_norm = S_outer_h->res0;
-    res.residual = S_outer_h->res0;
-    const double tol = S_outer_h->tol;
+    // (no wait here: the host learns max |b| and the tolerance from the mirror of
+    // the first refinement step below, which copies all of S_outer)
+    k_finish<<<1, 1024, 0, st>>>(FIN_RES0, w.partial, RED_BLOCKS, S_outer, tol_factor, 0);
+    double tol = 0.0;
     int total = 0, outer = 0;
-    res.status = 0;
-    if (S_outer_h->res0 == 0) { res.iterations = 0; res.outer = 0; return res; }
     res.status = 1;
     while (outer < max_outer && total < max_iter * max_outer) {
         // The inner solve reduces its residual by inner_tol, or only by what is
         // still missing to the outer tolerance (with a factor 2 of margin for the
-        // FP32 recurrence drifting from the true residual), whichever is less.
-        const double need = 0.5 * tol / res.residual;
+        // FP32 recurrence drifting from the true residual), whichever is less;
+        // for the first one, what is missing is tol_factor itself.
+        const double need = outer == 0 ? 0.5 * tol_factor : 0.5 * tol / res.residual;
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
         const PcgScalars hs = pcg_core<float, double>(g, code, w.inv32, w.r, w.x32, w.r32, w.s32, w.s2_32, w.z32,
                                                       w.q32, w.partial, w.npartial, w.S, w.S_host,
@@ -867,6 +865,8 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         hipStreamSynchronize(st);
         res.sweep_err = *(volatile int*)w.err_host;
         res.err_known = true;
+        res.rhs_norm = S_outer_h->res0;
+        tol = S_outer_h->tol;
         res.residual = S_outer_h->res;
         if (S_outer_h->res <= tol) { res.status = 0; break; }
         if (hs.status == 2) { res.status = 2; break; }
