@@ -442,7 +442,11 @@ __global__ void k_axpy_max(BoxRange bx, T* __restrict__ x, T* __restrict__ r, co
                            PcgScalars* S) {
     __shared__ double sh[16];
     __shared__ double alpha_sh;
-    if (S->done) return;
+    // the solver state is read with the operands below (not ahead of them, one
+    // round trip each): a finished solve returns after their requests, before
+    // any store
+    const int done = S->done;
+    const double rho_next = S->rho_next;
     const unsigned stride = gridDim.x * blockDim.x;
     // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
     unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -460,11 +464,12 @@ __global__ void k_axpy_max(BoxRange bx, T* __restrict__ x, T* __restrict__ r, co
         bw_next(bx, w);
         x0[u] = x[c0[u]]; r0[u] = r[c0[u]]; s0[u] = s[c0[u]]; z0[u] = z[c0[u]];
     }
+    if (done) return;
     const double sz = reduce_partials<false>(pin, npin, sh);
     if (threadIdx.x == 0) {
-        alpha_sh = S->rho_next / sz;
+        alpha_sh = rho_next / sz;
         // (rho_next is this iteration's rho; nothing reads rho while this runs)
-        if (blockIdx.x == 0) { S->sz = sz; S->alpha = alpha_sh; S->rho = S->rho_next; }
+        if (blockIdx.x == 0) { S->sz = sz; S->alpha = alpha_sh; S->rho = rho_next; }
     }
     __syncthreads();
     const double alpha = alpha_sh;
@@ -557,7 +562,10 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
                            const double* __restrict__ pin, int npin, double* partial, PcgScalars* S, BoxRange bx) {
     __shared__ double sh[16];
     __shared__ double beta_sh;
-    if (S->done) return;
+    // the solver state is read with the operands below, and a finished solve
+    // returns after their requests, before any store (as in k_axpy_max)
+    const int was = S->done;
+    const double rho = S->rho;
     const unsigned stride = gridDim.x * blockDim.x;
     // (each XCD walks a contiguous part of the box: neighbours' rows stay in its L2)
     unsigned t = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -581,9 +589,10 @@ __global__ void k_spmv_upd(Geo g, const uint16_t* __restrict__ code, const T* __
 #pragma unroll
         for (int i = 0; i < 7; ++i) { z0[u][i] = z[e[i]]; o0[u][i] = s_old[e[i]]; }
     }
+    if (was) return;
     const double v = reduce_partials<false>(pin, npin, sh);
     if (threadIdx.x == 0) {
-        beta_sh = v / S->rho;
+        beta_sh = v / rho;
         if (blockIdx.x == 0) { S->rho_new = v; S->beta = beta_sh; S->rho_next = v; }
     }
     __syncthreads();
