@@ -757,8 +757,8 @@ static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, con
     // chunk c (whose last residual step writes its status into the pinned slot
     // S_host[c & 1 ? 2 : 0]), so the GPU does not idle while the host checks.
     // After convergence the kernels of the chunk in flight return at once.
-    auto launch_chunk = [&](int c) {
-        int end = it + CH < max_iter ? it + CH : max_iter;
+    auto launch_chunk = [&](int c, int n) {
+        int end = it + n < max_iter ? it + n : max_iter;
         PcgScalars* slot = S_host + ((c & 1) ? 2 : 0);
         for (; it < end; ++it) {
             // (the spmv and dot partials go to `partial`, the residual's to `partial2`)
@@ -779,17 +779,50 @@ static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, con
         }
         hipEventRecord(chunk_ev[c & 1], st);
     };
+    // Chunk sizes: CH iterations, fewer once the decay of max |r| between the
+    // last two chunks predicts convergence sooner (one iteration of margin), so
+    // that few iterations are enqueued past it (each one's three launches
+    // return at once but still cost about 4 us each).  When the chunks enqueued
+    // so far are predicted to reach it, no further chunk is enqueued until the
+    // last one's status is in; if it has not converged then, the GPU idles for
+    // the host's turnaround while the next chunk is enqueued.
+    int launched = 0, want = max_iter;
+    int end_it[2] = {0, 0};  // iteration count at the end of chunk c, at [c & 1]
+    auto enqueue = [&](int n) {
+        const int c = launched++;
+        launch_chunk(c, n);
+        end_it[c & 1] = it;
+        want -= n;
+    };
+    enqueue(CH);
     int chunk = 0;
-    launch_chunk(0);
+    double prev_res = -1.0;
+    int prev_it = 0;
     // the host reads each chunk's slot into its own copy: the chunk enqueued
     // next may already be writing the other slot
     PcgScalars hs{};
     for (;;) {
-        if (it < max_iter) launch_chunk(chunk + 1);
+        // at most two chunks in flight: this one and the next
+        if (launched == chunk + 1 && it < max_iter && want > 0) enqueue(want < CH ? want : CH);
         hipEventSynchronize(chunk_ev[chunk & 1]);
         hs = S_host[(chunk & 1) ? 2 : 0];
-        if (hs.done || chunk * CH + CH >= max_iter) break;
+        if (hs.done || (it >= max_iter && launched == chunk + 1)) break;
+        // iterations still needed after this chunk's test, from the decay rate of
+        // max |r| since the previous chunk's (all of max_iter while unknown)
+        int need = max_iter;
+        if (prev_res > 0.0 && hs.res > 0.0 && hs.res < prev_res && hs.it > prev_it && hs.tol > 0.0) {
+            const double rate = std::log(hs.res / prev_res) / (double)(hs.it - prev_it);
+            const double k = std::log(hs.tol / hs.res) / rate;
+            need = k < (double)max_iter ? (int)std::ceil(k) + 1 : max_iter;
+        }
+        prev_res = hs.res;
+        prev_it = hs.it;
+        // (iterations already enqueued after this chunk)
+        const int ahead = launched > chunk + 1 ? end_it[(chunk + 1) & 1] - end_it[chunk & 1] : 0;
+        want = need - ahead;
         ++chunk;
+        // nothing in flight (the prediction fell short): enqueue at least one iteration
+        if (launched == chunk && it < max_iter) enqueue(want > 1 ? (want < CH ? want : CH) : 1);
     }
     if (!hs.done) {
         // (the stream is idle after this wait: no kernel writes the slots then)
