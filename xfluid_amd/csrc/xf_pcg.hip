@@ -13,6 +13,7 @@
 // but not the reference's serial order: PCG iterates match to rounding).
 #include <vector>
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include "xf_internal.h"
 #include <climits>
@@ -399,9 +400,13 @@ void pcg_spmv(const Geo& g, const uint16_t* code, const double* x, double* y, hi
 }
 
 // r = b - A x (FP64), partial max |r| -- the refinement residual.
+// guard (optional): runs only once that solve has converged (enqueued ahead of
+// knowing it, see pcg_core's `post`)
 __global__ void k_residual(Geo g, const uint16_t* __restrict__ code, const double* __restrict__ b,
-                           const double* __restrict__ x, double* __restrict__ r, double* partial) {
+                           const double* __restrict__ x, double* __restrict__ r, double* partial,
+                           const PcgScalars* guard = nullptr) {
     __shared__ double sh[16];
+    if (guard && !guard->done) return;
     double m = 0.0;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
         double v = b[c] - spmv_cell<double>(g, code[c], c, x);
@@ -656,7 +661,9 @@ template <class T>
 __global__ void k_fill(long n, T* __restrict__ d, T v) {
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) d[c] = v;
 }
-__global__ void k_add_f2d(long n, double* __restrict__ x, const float* __restrict__ c) {
+__global__ void k_add_f2d(long n, double* __restrict__ x, const float* __restrict__ c,
+                          const PcgScalars* guard = nullptr) {
+    if (guard && !guard->done) return;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         x[i] += (double)c[i];
 }
@@ -668,8 +675,10 @@ enum { FIN_RES0, FIN_RHO0, FIN_ALPHA, FIN_RES, FIN_BETA };
 // it after synchronising the stream, with no copy kernel in between)
 // eflag_out (optional, pinned): receives *eflag, the sweeps' error word, with the mirror
 __global__ void k_finish(int what, const double* partial, int n, PcgScalars* S, double tol_factor, int it,
-                         PcgScalars* mirror = nullptr, const int* eflag = nullptr, int* eflag_out = nullptr) {
+                         PcgScalars* mirror = nullptr, const int* eflag = nullptr, int* eflag_out = nullptr,
+                         const PcgScalars* guard = nullptr) {
     __shared__ double sh[16];
+    if (guard && !guard->done) return;
     if (what != FIN_RES0 && S->done) {
         if (mirror && threadIdx.x == 0) *mirror = *S;
         if (eflag_out && threadIdx.x == 0) *eflag_out = *eflag;
@@ -720,13 +729,21 @@ void launch_finish_res(const FinRes& f, hipStream_t st) {
 // converged solve returns as soon as the host has seen its status: the
 // iterations enqueued after it, which return at once, may still be in flight
 // on the stream, ahead of whatever the caller enqueues next.
+// post (optional): enqueues the caller's work after the solve, each kernel of it
+// guarded by S->done (it runs only if the solve has converged by then).  It is
+// called once, right after the chunk that is predicted to converge, so that
+// this work follows the solve with no wait for the host; *post_ran tells the
+// caller whether it was enqueued behind the converged chunk (else the caller
+// enqueues it itself).
 template <class T, class Src>
 static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, const Src* rhs, T* x, T* r, T* s, T* s2,
                            T* z, T* q,
                      double* partial, int half, PcgScalars* S, PcgScalars* S_host, hipEvent_t* chunk_ev,
                      double tol_factor,
                      int max_iter, int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
-                     SweepTimer* tm, hipStream_t st) {
+                     SweepTimer* tm, hipStream_t st, const std::function<void()>* post = nullptr,
+                     bool* post_ran = nullptr) {
+    if (post_ran) *post_ran = false;
     const long N = g.N;
     const int tm_start = tm ? tm->used : 0;
     if (tm) tm->k = 0;
@@ -801,11 +818,17 @@ static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, con
     // the host reads each chunk's slot into its own copy: the chunk enqueued
     // next may already be writing the other slot
     PcgScalars hs{};
+    int post_at = 0;  // chunks enqueued ahead of the caller's work (0: not enqueued)
     for (;;) {
         // at most two chunks in flight: this one and the next
         if (launched == chunk + 1 && it < max_iter && want > 0) enqueue(want < CH ? want : CH);
+        if (post && !post_at && (want <= 0 || it >= max_iter)) {
+            (*post)();
+            post_at = launched;
+        }
         hipEventSynchronize(chunk_ev[chunk & 1]);
         hs = S_host[(chunk & 1) ? 2 : 0];
+        if (hs.done && post_ran) *post_ran = chunk < post_at;
         if (hs.done || (it >= max_iter && launched == chunk + 1)) break;
         // iterations still needed after this chunk's test, from the decay rate of
         // max |r| since the previous chunk's (all of max_iter while unknown)
@@ -893,17 +916,26 @@ PcgResult pcg_solve(const Geo& g, const uint16_t* code, PcgWork& w, double tol_f
         // for the first one, what is missing is tol_factor itself.
         const double need = outer == 0 ? 0.5 * tol_factor : 0.5 * tol / res.residual;
         const double itol = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
+        // the refinement step: x += x32, r = b - A x and its max |r| (with the
+        // solve's error word: every sweep that ran did so before it; the
+        // iterations enqueued after convergence return at once).  Guarded by the
+        // inner solve's status, it is enqueued behind the chunk predicted to
+        // converge, so that it follows the solve without a wait for the host.
+        auto refine = [&](const PcgScalars* guard) {
+            k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32, guard);
+            k_residual<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.b, w.x, w.r, w.partial, guard);
+            k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0, S_outer_h, w.sweep_flags,
+                                         w.err_host, guard);
+        };
+        const std::function<void()> post = [&]() { refine(w.S); };
+        bool post_ran = false;
         const PcgScalars hs = pcg_core<float, double>(g, code, w.inv32, w.r, w.x32, w.r32, w.s32, w.s2_32, w.z32,
                                                       w.q32, w.partial, w.npartial, w.S, w.S_host,
                         w.chunk_ev, itol,
-                        max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st);
+                        max_iter, sweep_kernel, &w.sk32, w.sweep_flags, &w.epoch, &w.timer, st, &post, &post_ran);
         total += hs.iters;
         ++outer;
-        k_add_f2d<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(N, w.x, w.x32);
-        k_residual<<<RED_BLOCKS, XF_BLOCK, 0, st>>>(g, code, w.b, w.x, w.r, w.partial);
-        // (with the solve's error word: every sweep of the solve ran before this)
-        k_finish<<<1, 1024, 0, st>>>(FIN_RES, w.partial, RED_BLOCKS, S_outer, 0.0, 0, S_outer_h, w.sweep_flags,
-                                     w.err_host);
+        if (!post_ran) refine(nullptr);
         hipStreamSynchronize(st);
         res.sweep_err = *(volatile int*)w.err_host;
         res.err_known = true;
