@@ -240,9 +240,14 @@ __global__ void k_bwd_plane(Geo g, int t, int kmin, const uint16_t* __restrict__
 // partials of dot(z, r) (returns true then; the caller runs w8_dot_tiles
 // otherwise, which gives the same partials)
 template <class T>
+// chunk_end (optional): an event to record at the end of the application; when
+// the pipelined sweep runs untimed it rides on the sweep's own dispatch (no
+// marker packet, whose cache release idles the GPU for about 6 us) and
+// *chunk_rec is set, else the caller records it.
 static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T* r, T* q, T* z, const PcgScalars* S,
                       int sweep_kernel, const SweepSkew<T>* sk, int* flags, unsigned long long* epoch,
-                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr, FinRes* fin = nullptr) {
+                      SweepTimer* tm, hipStream_t st, double* dot_partial = nullptr, FinRes* fin = nullptr,
+                      hipEvent_t chunk_end = nullptr, bool* chunk_rec = nullptr) {
     // a pending residual test runs first: inside the 8x8-tile schedule's fused
     // kernel, else as k_finish here
     if (fin && !(sweep_kernel == 9 && sk)) {
@@ -256,8 +261,10 @@ static bool precond_t(const Geo& g, const uint16_t* code, const T* inv, const T*
             a = tm->ev[tm->used];
             b = tm->ev[tm->used + 1];
         }
-        if (sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, b, st, dot_partial, fin)) {
+        const bool ride = !b && chunk_end;
+        if (sweep_precond_w8<T>(g, *sk, r, z, S, flags, epoch, a, ride ? chunk_end : b, st, dot_partial, fin)) {
             if (a) tm->used += 2;
+            if (ride && chunk_rec) *chunk_rec = true;
             return dot_partial != nullptr;
         }
     }
@@ -777,6 +784,7 @@ static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, con
     auto launch_chunk = [&](int c, int n) {
         int end = it + n < max_iter ? it + n : max_iter;
         PcgScalars* slot = S_host + ((c & 1) ? 2 : 0);
+        bool rec = false;  // the chunk's event rode on its last sweep
         for (; it < end; ++it) {
             // (the spmv and dot partials go to `partial`, the residual's to `partial2`)
             if (it == 0) {
@@ -791,10 +799,11 @@ static PcgScalars pcg_core(const Geo& g, const uint16_t* code, const T* inv, con
             k_axpy_max<T><<<RED_BLOCKS, XF_BLOCK, 0, st>>>(bxr, x, r, sc, q, it == 0 ? partial : partial2, RED_BLOCKS,
                                                             it == 0 ? partial2 : partial, S);
             FinRes fin{it == 0 ? partial2 : partial, RED_BLOCKS, it, S, it == end - 1 ? slot : nullptr, false};
-            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &fin))
+            if (!precond_t<T>(g, code, inv, r, q, z, S, sweep_kernel, sk, flags, epoch, tm, st, partial, &fin,
+                              it == end - 1 ? chunk_ev[c & 1] : nullptr, &rec))
                 dot_zr();
         }
-        hipEventRecord(chunk_ev[c & 1], st);
+        if (!rec) hipEventRecord(chunk_ev[c & 1], st);
     };
     // Chunk sizes: CH iterations, fewer once the decay of max |r| between the
     // last two chunks predicts convergence sooner (one iteration of margin), so
