@@ -96,9 +96,7 @@ __global__ void k_fill(long n, T* __restrict__ d, T v);
 __global__ void k_active_box(Geo g, const uint16_t* __restrict__ code, int* box) {
     __shared__ int red[6][XF_BLOCK / 64];
     int v[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
-    // grid-stride: few blocks, so few atomics
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < g.N; c += (long)gridDim.x * blockDim.x) {
-        const uint16_t cc = code[c];
+    auto visit = [&](long c, uint16_t cc) __attribute__((always_inline)) {
         bool act = code_diag(cc) != 0.0;
         for (int d = 0; d < 6; ++d) act = act || present(code_dir(cc, d));
         if (act) {  // (32-bit index arithmetic: N < 2^30)
@@ -108,6 +106,23 @@ __global__ void k_active_box(Geo g, const uint16_t* __restrict__ code, int* box)
             v[2] = j < v[2] ? j : v[2]; v[3] = j > v[3] ? j : v[3];
             v[4] = k < v[4] ? k : v[4]; v[5] = k > v[5] ? k : v[5];
         }
+    };
+    // grid-stride: few blocks, so few atomics; a thread's codes are loaded U at a
+    // time, unconditionally (one round trip to memory for all of them, not one
+    // per cell; a cell past the end visits the last cell again, which changes
+    // no bound)
+    constexpr int U = 8;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long c0 = blockIdx.x * (long)blockDim.x + threadIdx.x; c0 < g.N; c0 += U * stride) {
+        long cs[U];
+        uint16_t ccs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cs[u] = c0 + u * stride < g.N ? c0 + u * stride : g.N - 1;
+            ccs[u] = code[cs[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) visit(cs[u], ccs[u]);
     }
     for (int o = 32; o > 0; o >>= 1)
         for (int d = 0; d < 6; ++d) {
